@@ -1,0 +1,42 @@
+"""Synthetic workloads of the adapt hot path (host-side field evaluation).
+
+Fields are evaluated on the host in float64 with numpy from the current
+vertex coordinates, exactly as the reference evaluates analytic fields on
+the host (eval_field.cpp:15-49); the arrays are then handed to the mesh.
+"""
+import math
+
+import numpy as np
+
+
+def sine_warp_2d(coords, amp=0.1):
+    """u(x, y) = amp*sin(2 pi x)*cos(2 pi y) in both x and y components."""
+    x, y = coords[:, 0], coords[:, 1]
+    u = amp * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    w = np.zeros_like(coords)
+    w[:, 0] = u
+    w[:, 1] = u
+    return w
+
+
+def rotation_warp(coords, rotation, mid=(0.5, 0.5, 0.0), flatten_z=True):
+    """The swirl of test_warp.cpp:34-50 / test_warp_perf.cpp:47-64: points
+    within radius 0.5 of the axis rotate by rotation*2*(0.5-r)."""
+    x = coords - np.asarray(mid)[None, :]
+    if flatten_z:
+        x[:, 2] = 0
+    polar_a = np.arctan2(x[:, 1], x[:, 0])
+    polar_r = np.sqrt(np.sum(x * x, axis=1))
+    rot_a = np.where(polar_r < 0.5, rotation * (2 * (0.5 - polar_r)), 0.0)
+    dest_a = polar_a + rot_a
+    dst = np.zeros_like(x)
+    dst[:, 0] = np.cos(dest_a) * polar_r
+    dst[:, 1] = np.sin(dest_a) * polar_r
+    return dst - x
+
+
+def uniform_size(nverts, h):
+    return np.full(nverts, float(h))
+
+
+PI_REF = 3.14159265359  # algebra.hpp:8

@@ -1,0 +1,88 @@
+/* CPU oracle for the Omega_h v1 adapt-pass hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a from-scratch serial C restatement of
+ * the reference's algorithm (FP64, isotropic "adapt_size" metric, serial
+ * loop build semantics).  Only tests/, bench.py's cpu_baseline leg and
+ * __graft_entry__.smoke() load it, and only as the checker.  The product
+ * (omega_h_v1_amd/) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.
+ */
+#ifndef OHV1_ORACLE_H
+#define OHV1_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mesh orc_mesh;
+
+/* Kuhn box generator: n^dim cubes (squares), each split like the
+ * reference's single box (tables.cpp:29-47). Caller frees with orc_free_buf. */
+void orc_box(unsigned dim, unsigned n, unsigned* nverts, double** coords,
+    unsigned* nelems, unsigned** conn);
+void orc_free_buf(void* p);
+
+orc_mesh* orc_new_mesh(unsigned dim, unsigned nverts, double const* coords,
+    unsigned nelems, unsigned const* conn);
+void orc_free(orc_mesh* m);
+void orc_derive_model_full(orc_mesh* m);
+void orc_derive_model(orc_mesh* m, double crease_angle);
+void orc_set_full(orc_mesh* m);
+
+unsigned orc_dim(orc_mesh* m);
+unsigned orc_count(orc_mesh* m, unsigned dim);
+void orc_get_down(orc_mesh* m, unsigned h, unsigned l, unsigned* out);
+unsigned orc_up_nuses(orc_mesh* m, unsigned l, unsigned h);
+void orc_get_up(orc_mesh* m, unsigned l, unsigned h, unsigned* offs,
+    unsigned* adj, unsigned* dirs);
+unsigned orc_star_nadj(orc_mesh* m, unsigned l, unsigned h);
+void orc_get_star(orc_mesh* m, unsigned l, unsigned h, unsigned* offs,
+    unsigned* adj);
+void orc_get_dual(orc_mesh* m, unsigned* out);
+int orc_get_tag(orc_mesh* m, unsigned dim, char const* name, int want_f64,
+    void* out);
+unsigned orc_ntags(orc_mesh* m, unsigned dim);
+char const* orc_tag_name(orc_mesh* m, unsigned dim, unsigned i);
+void orc_add_field(orc_mesh* m, unsigned dim, char const* name,
+    unsigned ncomps, double const* data);
+void orc_free_field(orc_mesh* m, unsigned dim, char const* name);
+
+double orc_min_quality(orc_mesh* m);
+void orc_qualities(orc_mesh* m, double* out);
+void orc_measure_edges(orc_mesh* m, double* out);
+void orc_find_indset(orc_mesh* m, unsigned ent_dim,
+    unsigned const* candidates, double const* goodness, unsigned* out);
+
+unsigned orc_refine_by_size(orc_mesh* m, double qual_floor);
+void orc_uniformly_refine(orc_mesh* m);
+unsigned orc_coarsen_by_size(orc_mesh* m, double qual_floor, double size_floor);
+unsigned orc_coarsen_slivers(orc_mesh* m, double qual_floor, unsigned nlayers);
+unsigned orc_swap_slivers(orc_mesh* m, double good_qual, unsigned nlayers);
+int orc_adapt(orc_mesh* m, double size_floor, double good_qual,
+    unsigned nlayers, unsigned max_ops);
+int orc_adapt_timed(orc_mesh* m, double size_floor, double good_qual,
+    unsigned nlayers, unsigned max_ops, double* stats);
+unsigned orc_warp_to_limit(orc_mesh* m, double qual_floor);
+
+/* pure kernels, for known-answer tests */
+double orc_tet_quality(double const* x12);
+double orc_triangle_quality(double const* x9);
+double orc_triangle_xy_quality(double const* x9);
+double orc_doubles_min(double const* a, unsigned n);
+double orc_doubles_max(double const* a, unsigned n);
+unsigned orc_choose_edge_swap(unsigned ring_size, double const* edge_x6,
+    double const* ring_x, double* quality);
+unsigned orc_swap_nint_edges(unsigned ring_size);
+void orc_swap_int_edges(unsigned ring_size, unsigned code, unsigned* out);
+void orc_invert_map(unsigned nin, unsigned const* in, unsigned nout,
+    unsigned* out, unsigned* offsets);
+
+/* verbosity of the per-pass prints (reference prints them from rank 0) */
+void orc_set_verbose(int v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
