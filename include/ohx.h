@@ -1,0 +1,159 @@
+/* Kernel-level C ABI and mesh seam of the MI355X adapt library.
+ *
+ * Two layers sit below include/omega_h.h:
+ *
+ * 1. ohx_* kernel entry points: plain device pointers, sizes and a HIP
+ *    stream (passed as void*, NULL = the library stream).  Each replaces one
+ *    data-parallel loop of the reference and cites it.  They enqueue work on
+ *    the stream; entry points with a host return value synchronise.  On a
+ *    bad argument or a device error they return OHX_ERROR (or a NaN for
+ *    double results) and ohx_last_error() says why.
+ *
+ * 2. ohx_mesh_* seam: the reference's internal pass functions (refine.hpp,
+ *    coarsen.hpp, swap.hpp, adapt.hpp, warp_to_limit.hpp, derive_model.hpp)
+ *    over a device-resident mesh, with host arrays in and out.  Tests and
+ *    the benchmark drive the product through this layer, in the same shape
+ *    as the oracle's orc_* and the reference harness's ref_* functions.
+ *
+ * There is no CPU fallback: every call needs a visible HIP device.
+ */
+#ifndef OHX_H
+#define OHX_H
+
+#include "omega_h.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OHX_ERROR 0xffffffffu
+
+/* last error message of this thread, or NULL */
+char const* ohx_last_error(void) OSH_PUBLIC;
+void ohx_clear_error(void) OSH_PUBLIC;
+
+/* ------------------------------------------------- kernel entry points */
+
+/* elem_quality_kern / element_qualities (quality.cpp:13-45): FP64 quality
+ * of every element (tets: Shewchuk V/A_rms^{3/2}; triangles: signed z-area
+ * form), bit-identical to the reference's x86-64 build. */
+unsigned ohx_element_qualities_f64(unsigned dim, unsigned nelems,
+    unsigned const* verts_of_elems, double const* coords, double* out,
+    void* stream) OSH_PUBLIC;
+
+/* min_element_quality (quality.cpp:47-57) with the serial doubles_min
+ * (doubles.cpp:111-118): the minimum over elements 1..nelems-1. */
+double ohx_min_quality_f64(unsigned dim, unsigned nelems,
+    unsigned const* verts_of_elems, double const* coords, void* stream) OSH_PUBLIC;
+
+/* measure_edge (size.cpp:57-88): |x1-x0| / ((h0+h1)/2) per edge */
+unsigned ohx_measure_edges_f64(unsigned nedges, unsigned const* verts_of_edges,
+    double const* coords, double const* size, double* out, void* stream) OSH_PUBLIC;
+
+/* uints_exscan (ints.cpp:25-36): out[0]=0, out[i+1]=a[0]+...+a[i] (n+1 values) */
+unsigned ohx_exscan_u32(unsigned const* a, unsigned n, unsigned* out, void* stream) OSH_PUBLIC;
+
+/* up_from_down (up_from_down.cpp:24-47) over invert_map
+ * (invert_map.cpp:118-136): offsets (nlows+1), highs and directions
+ * (nhighs*deg each) sorted by (high, direction) per low. */
+unsigned ohx_up_from_down(unsigned high_dim, unsigned low_dim, unsigned nhighs,
+    unsigned nlows, unsigned const* lows_of_highs, unsigned* offsets,
+    unsigned* highs_of_lows, unsigned* directions, void* stream) OSH_PUBLIC;
+
+/* find_indset (indset.cpp:95-117) on a CSR graph: state[i] = 1 for members
+ * of the independent set.  Returns the number of Jacobi rounds run (in
+ * batches), or OHX_ERROR if the set did not converge within 100 rounds. */
+unsigned ohx_find_indset(unsigned n, unsigned const* offsets, unsigned const* adj,
+    unsigned const* filter, double const* goodness, unsigned* state,
+    void* stream) OSH_PUBLIC;
+
+/* FP64 self-test of the device arithmetic the parity claims rest on:
+ * sqrt_out[i] = sqrt(a[i]), div_out[i] = a[i] / b[i], computed with the
+ * library's compile flags, to be compared bitwise with the host. */
+unsigned ohx_fp64_selftest(unsigned n, double const* a, double const* b,
+    double* sqrt_out, double* div_out, void* stream) OSH_PUBLIC;
+
+/* ----------------------------------------------------------- mesh seam */
+typedef struct ohx_mesh ohx_mesh;
+
+/* a reduced (vertices + elements) serial mesh from host arrays */
+ohx_mesh* ohx_mesh_new(unsigned dim, unsigned nverts, double const* coords,
+    unsigned nelems, unsigned const* conn) OSH_PUBLIC;
+/* Kuhn box: n^dim cubes (squares) split as new_box_mesh (mesh.cpp:69-83,
+ * tables.cpp:15-47), generated on the device */
+ohx_mesh* ohx_mesh_new_box(unsigned dim, unsigned n) OSH_PUBLIC;
+ohx_mesh* ohx_mesh_clone(ohx_mesh* m) OSH_PUBLIC;
+void ohx_mesh_free(ohx_mesh* m) OSH_PUBLIC;
+/* mesh_derive_model(m, PI/4) + mesh_set_rep(m, MESH_FULL)
+ * (derive_model.cpp:444-448, test_warp.cpp:108-110) */
+void ohx_mesh_derive_model_full(ohx_mesh* m) OSH_PUBLIC;
+void ohx_mesh_set_full(ohx_mesh* m) OSH_PUBLIC;
+
+unsigned ohx_mesh_count(ohx_mesh* m, unsigned dim) OSH_PUBLIC;
+void ohx_mesh_get_down(ohx_mesh* m, unsigned h, unsigned l, unsigned* out) OSH_PUBLIC;
+unsigned ohx_mesh_up_nuses(ohx_mesh* m, unsigned l, unsigned h) OSH_PUBLIC;
+void ohx_mesh_get_up(ohx_mesh* m, unsigned l, unsigned h, unsigned* offs,
+    unsigned* adj, unsigned* dirs) OSH_PUBLIC;
+unsigned ohx_mesh_star_nadj(ohx_mesh* m, unsigned l, unsigned h) OSH_PUBLIC;
+void ohx_mesh_get_star(ohx_mesh* m, unsigned l, unsigned h, unsigned* offs,
+    unsigned* adj) OSH_PUBLIC;
+void ohx_mesh_get_dual(ohx_mesh* m, unsigned* out) OSH_PUBLIC;
+/* returns ncomps, or -1 if there is no such tag of that kind */
+int ohx_mesh_get_tag(ohx_mesh* m, unsigned dim, char const* name, int want_f64,
+    void* out) OSH_PUBLIC;
+unsigned ohx_mesh_ntags(ohx_mesh* m, unsigned dim) OSH_PUBLIC;
+char const* ohx_mesh_tag_name(ohx_mesh* m, unsigned dim, unsigned i) OSH_PUBLIC;
+void ohx_mesh_add_field(ohx_mesh* m, unsigned dim, char const* name,
+    unsigned ncomps, double const* data) OSH_PUBLIC;
+void ohx_mesh_free_field(ohx_mesh* m, unsigned dim, char const* name) OSH_PUBLIC;
+/* uniform field on every vertex, set on the device */
+void ohx_mesh_set_uniform_field(ohx_mesh* m, char const* name, double value) OSH_PUBLIC;
+
+double ohx_mesh_min_quality(ohx_mesh* m) OSH_PUBLIC;
+void ohx_mesh_qualities(ohx_mesh* m, double* out) OSH_PUBLIC;
+void ohx_mesh_measure_edges(ohx_mesh* m, double* out) OSH_PUBLIC;
+void ohx_mesh_find_indset(ohx_mesh* m, unsigned ent_dim,
+    unsigned const* candidates, double const* goodness, unsigned* out) OSH_PUBLIC;
+
+/* refine.cpp:17-35, coarsen.cpp:23-70, swap.cpp:138-154 */
+unsigned ohx_mesh_refine_by_size(ohx_mesh* m, double qual_floor) OSH_PUBLIC;
+void ohx_mesh_uniformly_refine(ohx_mesh* m) OSH_PUBLIC;
+unsigned ohx_mesh_coarsen_by_size(ohx_mesh* m, double qual_floor,
+    double size_floor) OSH_PUBLIC;
+unsigned ohx_mesh_coarsen_slivers(ohx_mesh* m, double qual_floor,
+    unsigned nlayers) OSH_PUBLIC;
+unsigned ohx_mesh_swap_slivers(ohx_mesh* m, double good_qual,
+    unsigned nlayers) OSH_PUBLIC;
+/* mesh_adapt (adapt.cpp:82-94); -1 where the reference aborts after
+ * max_ops operations */
+int ohx_mesh_adapt(ohx_mesh* m, double size_floor, double good_qual,
+    unsigned nlayers, unsigned max_ops) OSH_PUBLIC;
+/* as ohx_mesh_adapt; stats[0] passes that changed the mesh, [1] the sum of
+ * their input element counts, [2] seconds inside them, [3] seconds total */
+int ohx_mesh_adapt_timed(ohx_mesh* m, double size_floor, double good_qual,
+    unsigned nlayers, unsigned max_ops, double* stats) OSH_PUBLIC;
+/* mesh_warp_to_limit (warp_to_limit.cpp:61-81) */
+unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qual_floor) OSH_PUBLIC;
+
+/* the osh_t handle of a seam mesh (same object) */
+osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;
+
+/* ------------------------------------------------------ instrumentation */
+void ohx_set_verbose(int on) OSH_PUBLIC;
+/* HIP-event timing of named kernels on the stream they run on */
+void ohx_profile_enable(int on) OSH_PUBLIC;
+void ohx_profile_reset(void) OSH_PUBLIC;
+/* total milliseconds and launches of a timed kernel; returns 0 if unseen */
+int ohx_profile_get(char const* name, double* ms, unsigned long long* launches) OSH_PUBLIC;
+unsigned long long ohx_device_bytes_high_water(void) OSH_PUBLIC;
+void ohx_device_reset_high_water(void) OSH_PUBLIC;
+void ohx_synchronize(void) OSH_PUBLIC;
+/* interior-edge table of the swap triangulations (edge_swap_edges.cpp) */
+unsigned ohx_swap_nint_edges(unsigned ring_size) OSH_PUBLIC;
+void ohx_swap_int_edges(unsigned ring_size, unsigned code, unsigned* out) OSH_PUBLIC;
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif

@@ -1,0 +1,672 @@
+// C ABI: the public osh_* drop-in, the ohx_mesh_* seam and the ohx_*
+// kernel entry points.  Exceptions never cross the ABI: the seam and kernel
+// layers record ohx_last_error() and return a sentinel, the osh_* layer
+// aborts like the reference's asserts.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "../../include/ohx.h"
+#include "ohx_adapt.hpp"
+#include "ohx_geom.hpp"
+#include "ohx_prims.hpp"
+#include "ohx_tables.hpp"
+
+namespace ohx {
+std::string const& last_error();
+void set_last_error(std::string const& s);
+void prof_enable(bool on);
+void prof_reset();
+bool prof_get(char const* name, double* ms, u64* launches);
+}  // namespace ohx
+
+using namespace ohx;
+
+struct ohx_mesh {
+  Mesh m;
+  // per-dimension caches handed out through osh_* getters
+  Buf<u32> own_rank[4], own_id[4];
+  Buf<unsigned long> global[4];
+  std::vector<std::string> field_names;
+  ohx_mesh(u32 d, bool f) : m(d, f) {}
+};
+
+static thread_local std::string t_err;
+static thread_local bool t_has_err = false;
+
+template <class F, class R>
+static R guard(R bad, F&& f) {
+  try {
+    return f();
+  } catch (std::exception const& e) {
+    t_err = e.what();
+    t_has_err = true;
+    return bad;
+  }
+}
+
+template <class F>
+static void guard_void(F&& f) {
+  try {
+    f();
+  } catch (std::exception const& e) {
+    t_err = e.what();
+    t_has_err = true;
+  }
+}
+
+// the public layer mirrors the reference's assert/abort behaviour
+template <class F>
+static auto osh_guard(F&& f) -> decltype(f()) {
+  try {
+    return f();
+  } catch (std::exception const& e) {
+    std::fprintf(stderr, "omega_h (MI355X): %s\n", e.what());
+    std::abort();
+  }
+}
+
+static ohx_mesh* M(osh_t p) { return reinterpret_cast<ohx_mesh*>(p); }
+
+// ------------------------------------------------------- Kuhn generator
+static ohx_mesh* new_box(u32 dim, u32 n) {
+  OHX_REQUIRE(dim == 2 || dim == 3, "box dimension must be 2 or 3");
+  OHX_REQUIRE(n >= 1, "box needs at least one cell per axis");
+  u32 np = n + 1;
+  u64 nv64 = (dim == 3) ? (u64) np * np * np : (u64) np * np;
+  u64 nc64 = (dim == 3) ? (u64) n * n * n : (u64) n * n;
+  OHX_REQUIRE(nc64 * (dim == 3 ? 6 : 2) * (dim + 1) < (1ull << 32), "box too large for 32-bit indices");
+  u32 nv = (u32) nv64, nc = (u32) nc64;
+  u32 per = (dim == 3) ? 6 : 2;
+  auto* mm = new ohx_mesh(dim, false);
+  Mesh& m = mm->m;
+  Buf<double> x((size_t) 3 * nv);
+  double* xp = x.get();
+  double dn = (double) n;
+  parallel_for(nv, OHX_LAMBDA(u32 v) {
+    u32 i = v % np, j = (v / np) % np, k = v / (np * np);
+    xp[3 * (size_t) v + 0] = (double) i / dn;
+    xp[3 * (size_t) v + 1] = (double) j / dn;
+    xp[3 * (size_t) v + 2] = (dim == 3) ? (double) k / dn : 0.0;
+  });
+  Buf<u32> conn((size_t) nc * per * (dim + 1));
+  u32* cp = conn.get();
+  if (dim == 3) {
+    parallel_for(nc, OHX_LAMBDA(u32 c) {
+      // corners and split of box_3d (tables.cpp:29-47)
+      u32 const cx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+      u32 const t[24] = {0, 1, 2, 6, 2, 3, 0, 6, 0, 3, 7, 6, 7, 4, 0, 6, 0, 4, 5, 6, 5, 1, 0, 6};
+      u32 ci = c % n, cj = (c / n) % n, ck = c / (n * n);
+      u32 vid[8];
+      for (u32 a = 0; a < 8; ++a) vid[a] = (ci + cx[a]) + np * ((cj + cy[a]) + np * (ck + cz[a]));
+      for (u32 a = 0; a < 24; ++a) cp[(size_t) c * 24 + a] = vid[t[a]];
+    });
+  } else {
+    parallel_for(nc, OHX_LAMBDA(u32 c) {
+      u32 const cx[4] = {0, 1, 1, 0}, cy[4] = {0, 0, 1, 1};  // box_2d (tables.cpp:15-27)
+      u32 const t[6] = {0, 1, 2, 2, 3, 0};
+      u32 ci = c % n, cj = c / n;
+      u32 vid[4];
+      for (u32 a = 0; a < 4; ++a) vid[a] = (ci + cx[a]) + np * (cj + cy[a]);
+      for (u32 a = 0; a < 6; ++a) cp[(size_t) c * 6 + a] = vid[t[a]];
+    });
+  }
+  set_ents(m, 0, nv, Buf<u32>());
+  set_ents(m, dim, nc * per, std::move(conn));
+  add_tag_f64(m, 0, "coordinates", 3, std::move(x));
+  return mm;
+}
+
+static ohx_mesh* clone_mesh(ohx_mesh* src) {
+  Mesh& s = src->m;
+  auto* mm = new ohx_mesh(s.dim, s.full);
+  Mesh& m = mm->m;
+  for (u32 d = 0; d <= s.dim; ++d) {
+    if (!has_dim(s, d)) continue;
+    u32 n = s.counts[d];
+    if (d == 0)
+      set_ents(m, 0, n, Buf<u32>());
+    else
+      set_ents(m, d, n, copy(s.down[d][0].get(), n * (d + 1)));
+    for (auto& tp : s.tags[d]) {
+      Tag& t = *tp;
+      if (t.type == TagType::F64)
+        add_tag_f64(m, d, t.name.c_str(), t.ncomps, copy_f64(t.f.get(), n * t.ncomps), t.transfer);
+      else
+        add_tag_u32(m, d, t.name.c_str(), t.ncomps, copy(t.u.get(), n * t.ncomps), t.transfer);
+    }
+  }
+  return mm;
+}
+
+// identity size (size.cpp:90-135): longest adjacent edge per vertex
+static void identity_size(Mesh& m, char const* name) {
+  Graph const& g = ask_star(m, 0, 1);
+  u32 nv = count(m, 0);
+  double const* x = coords(m);
+  Buf<double> out(nv);
+  double* o = out.get();
+  u32 const* go = g.offsets.get();
+  u32 const* ga = g.adj.get();
+  parallel_for(nv, OHX_LAMBDA(u32 i) {
+    V3 a = ld3(x + 3 * (size_t) i);
+    double mx = 0;
+    for (u32 j = go[i]; j < go[i + 1]; ++j) {
+      double l = sqrt(sqdist(ld3(x + 3 * (size_t) ga[j]), a));
+      if (l > mx) mx = l;
+    }
+    o[i] = mx;
+  });
+  add_tag_f64(m, 0, name, 1, std::move(out));
+}
+
+extern "C" {
+
+char const* ohx_last_error(void) { return t_has_err ? t_err.c_str() : nullptr; }
+void ohx_clear_error(void) {
+  t_has_err = false;
+  t_err.clear();
+}
+
+// ------------------------------------------------------------ kernel ABI
+unsigned ohx_element_qualities_f64(unsigned dim, unsigned n, unsigned const* ev, double const* x,
+                                   double* out, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    OHX_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    StreamScope sc((hipStream_t) s);
+    Buf<double> q = element_qualities(dim, n, ev, x);
+    d2d(out, q.get(), n);
+    return 0u;
+  });
+}
+
+double ohx_min_quality_f64(unsigned dim, unsigned n, unsigned const* ev, double const* x, void* s) {
+  return guard(std::numeric_limits<double>::quiet_NaN(), [&]() -> double {
+    OHX_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    StreamScope sc((hipStream_t) s);
+    return min_element_quality(dim, n, ev, x);
+  });
+}
+
+unsigned ohx_measure_edges_f64(unsigned ne, unsigned const* ev, double const* x, double const* h,
+                               double* out, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    StreamScope sc((hipStream_t) s);
+    parallel_for(ne, OHX_LAMBDA(u32 i) {
+      u32 a = ev[2 * i], b = ev[2 * i + 1];
+      double len = sqrt(sqdist(ld3(x + 3 * (size_t) b), ld3(x + 3 * (size_t) a)));
+      double want = (h[a] + h[b]) / 2;
+      out[i] = len / want;
+    });
+    return 0u;
+  });
+}
+
+unsigned ohx_exscan_u32(unsigned const* a, unsigned n, unsigned* out, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    StreamScope sc((hipStream_t) s);
+    exscan(a, n, out);
+    return 0u;
+  });
+}
+
+unsigned ohx_up_from_down(unsigned hd, unsigned ld, unsigned nh, unsigned nl, unsigned const* lh,
+                          unsigned* offs, unsigned* adj, unsigned* dirs, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    OHX_REQUIRE(hd <= 3 && ld < hd, "bad dimensions");
+    StreamScope sc((hipStream_t) s);
+    Up u;
+    up_from_down(hd, ld, nh, nl, lh, u);
+    u32 nuses = nh * h_down_deg[hd][ld];
+    d2d(offs, u.offsets.get(), (size_t) nl + 1);
+    d2d(adj, u.adj.get(), nuses);
+    d2d(dirs, u.dirs.get(), nuses);
+    return 0u;
+  });
+}
+
+unsigned ohx_find_indset(unsigned n, unsigned const* offs, unsigned const* adj, unsigned const* filter,
+                         double const* good, unsigned* state, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    StreamScope sc((hipStream_t) s);
+    // a one-dimensional "mesh" whose vertex star is the given graph
+    Mesh m(1, true);
+    set_ents(m, 0, n, Buf<u32>());
+    auto g = std::make_unique<Graph>();
+    g->offsets = copy(offs, n + 1);
+    u32 nadj = read_at(offs, n);
+    g->adj = copy(adj, nadj);
+    m.star[0][1] = std::move(g);
+    Buf<u32> st = find_indset(m, 0, filter, good);
+    d2d(state, st.get(), n);
+    return 0u;
+  });
+}
+
+unsigned ohx_fp64_selftest(unsigned n, double const* a, double const* b, double* so, double* dq, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    StreamScope sc((hipStream_t) s);
+    parallel_for(n, OHX_LAMBDA(u32 i) {
+      so[i] = sqrt(a[i]);
+      dq[i] = a[i] / b[i];
+    });
+    return 0u;
+  });
+}
+
+// ------------------------------------------------------------ mesh seam
+ohx_mesh* ohx_mesh_new(unsigned dim, unsigned nv, double const* x, unsigned ne, unsigned const* conn) {
+  return guard((ohx_mesh*) nullptr, [&]() -> ohx_mesh* {
+    OHX_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    auto* mm = new ohx_mesh(dim, false);
+    set_ents(mm->m, 0, nv, Buf<u32>());
+    set_ents(mm->m, dim, ne, to_device(conn, (size_t) ne * (dim + 1)));
+    add_tag_f64(mm->m, 0, "coordinates", 3, to_device(x, (size_t) 3 * nv));
+    return mm;
+  });
+}
+
+ohx_mesh* ohx_mesh_new_box(unsigned dim, unsigned n) {
+  return guard((ohx_mesh*) nullptr, [&]() { return new_box(dim, n); });
+}
+
+ohx_mesh* ohx_mesh_clone(ohx_mesh* m) {
+  return guard((ohx_mesh*) nullptr, [&]() { return clone_mesh(m); });
+}
+
+void ohx_mesh_free(ohx_mesh* m) {
+  guard_void([&]() { delete m; });
+}
+
+void ohx_mesh_derive_model_full(ohx_mesh* m) {
+  guard_void([&]() {
+    derive_model(m->m, REF_PI / 4);
+    m->m.full = true;
+  });
+}
+
+void ohx_mesh_set_full(ohx_mesh* m) { m->m.full = true; }
+
+unsigned ohx_mesh_count(ohx_mesh* m, unsigned d) {
+  return guard(OHX_ERROR, [&]() { return count(m->m, d); });
+}
+
+void ohx_mesh_get_down(ohx_mesh* m, unsigned h, unsigned l, unsigned* out) {
+  guard_void([&]() {
+    u32 n = count(m->m, h) * h_down_deg[h][l];
+    to_host(out, ask_down(m->m, h, l), n);
+  });
+}
+
+unsigned ohx_mesh_up_nuses(ohx_mesh* m, unsigned l, unsigned h) {
+  return guard(OHX_ERROR, [&]() {
+    Up const& u = ask_up(m->m, l, h);
+    return read_at(u.offsets.get(), count(m->m, l));
+  });
+}
+
+void ohx_mesh_get_up(ohx_mesh* m, unsigned l, unsigned h, unsigned* offs, unsigned* adj, unsigned* dirs) {
+  guard_void([&]() {
+    Up const& u = ask_up(m->m, l, h);
+    u32 n = count(m->m, l);
+    to_host(offs, u.offsets.get(), n + 1);
+    to_host(adj, u.adj.get(), offs[n]);
+    to_host(dirs, u.dirs.get(), offs[n]);
+  });
+}
+
+unsigned ohx_mesh_star_nadj(ohx_mesh* m, unsigned l, unsigned h) {
+  return guard(OHX_ERROR, [&]() {
+    Graph const& g = ask_star(m->m, l, h);
+    return read_at(g.offsets.get(), count(m->m, l));
+  });
+}
+
+void ohx_mesh_get_star(ohx_mesh* m, unsigned l, unsigned h, unsigned* offs, unsigned* adj) {
+  guard_void([&]() {
+    Graph const& g = ask_star(m->m, l, h);
+    u32 n = count(m->m, l);
+    to_host(offs, g.offsets.get(), n + 1);
+    to_host(adj, g.adj.get(), offs[n]);
+  });
+}
+
+void ohx_mesh_get_dual(ohx_mesh* m, unsigned* out) {
+  guard_void([&]() {
+    u32 d = m->m.dim;
+    to_host(out, ask_dual(m->m), (size_t) count(m->m, d) * h_down_deg[d][d - 1]);
+  });
+}
+
+int ohx_mesh_get_tag(ohx_mesh* m, unsigned d, char const* name, int want_f64, void* out) {
+  return guard(-1, [&]() -> int {
+    Tag* t = find_tag(m->m, d, name);
+    if (!t) return -1;
+    if ((t->type == TagType::F64) != (want_f64 != 0)) return -1;
+    if (out) {
+      size_t n = (size_t) count(m->m, d) * t->ncomps;
+      if (t->type == TagType::F64)
+        to_host((double*) out, t->f.get(), n);
+      else
+        to_host((u32*) out, t->u.get(), n);
+    }
+    return (int) t->ncomps;
+  });
+}
+
+unsigned ohx_mesh_ntags(ohx_mesh* m, unsigned d) { return (unsigned) m->m.tags[d].size(); }
+
+char const* ohx_mesh_tag_name(ohx_mesh* m, unsigned d, unsigned i) {
+  return m->m.tags[d][i]->name.c_str();
+}
+
+void ohx_mesh_add_field(ohx_mesh* m, unsigned d, char const* name, unsigned nc, double const* data) {
+  guard_void([&]() {
+    size_t n = (size_t) count(m->m, d) * nc;
+    add_tag_f64(m->m, d, name, nc, to_device(data, n));
+  });
+}
+
+void ohx_mesh_free_field(ohx_mesh* m, unsigned d, char const* name) {
+  guard_void([&]() { remove_tag(m->m, d, name); });
+}
+
+void ohx_mesh_set_uniform_field(ohx_mesh* m, char const* name, double v) {
+  guard_void([&]() {
+    if (find_tag(m->m, 0, name)) remove_tag(m->m, 0, name);
+    add_tag_f64(m->m, 0, name, 1, filled_f64(count(m->m, 0), v));
+  });
+}
+
+double ohx_mesh_min_quality(ohx_mesh* m) {
+  return guard(std::numeric_limits<double>::quiet_NaN(), [&]() { return mesh_min_quality(m->m); });
+}
+
+void ohx_mesh_qualities(ohx_mesh* m, double* out) {
+  guard_void([&]() {
+    Buf<double> q = mesh_qualities(m->m);
+    to_host(out, q.get(), count(m->m, m->m.dim));
+  });
+}
+
+void ohx_mesh_measure_edges(ohx_mesh* m, double* out) {
+  guard_void([&]() {
+    Buf<double> s = measure_edges(m->m);
+    to_host(out, s.get(), count(m->m, 1));
+  });
+}
+
+void ohx_mesh_find_indset(ohx_mesh* m, unsigned ed, unsigned const* cand, double const* good, unsigned* out) {
+  guard_void([&]() {
+    u32 n = count(m->m, ed);
+    Buf<u32> c = to_device(cand, n);
+    Buf<double> g = to_device(good, n);
+    Buf<u32> s = find_indset(m->m, ed, c.get(), g.get());
+    to_host(out, s.get(), n);
+  });
+}
+
+unsigned ohx_mesh_refine_by_size(ohx_mesh* m, double qf) {
+  return guard(OHX_ERROR, [&]() { return refine_by_size(m->m, qf); });
+}
+
+void ohx_mesh_uniformly_refine(ohx_mesh* m) {
+  guard_void([&]() { uniformly_refine(m->m); });
+}
+
+unsigned ohx_mesh_coarsen_by_size(ohx_mesh* m, double qf, double sf) {
+  return guard(OHX_ERROR, [&]() { return coarsen_by_size(m->m, qf, sf); });
+}
+
+unsigned ohx_mesh_coarsen_slivers(ohx_mesh* m, double qf, unsigned nl) {
+  return guard(OHX_ERROR, [&]() { return coarsen_slivers(m->m, qf, nl); });
+}
+
+unsigned ohx_mesh_swap_slivers(ohx_mesh* m, double gq, unsigned nl) {
+  return guard(OHX_ERROR, [&]() { return swap_slivers(m->m, gq, nl); });
+}
+
+int ohx_mesh_adapt(ohx_mesh* m, double sf, double gq, unsigned nl, unsigned mo) {
+  return guard(-2, [&]() { return adapt(m->m, sf, gq, nl, mo, nullptr); });
+}
+
+int ohx_mesh_adapt_timed(ohx_mesh* m, double sf, double gq, unsigned nl, unsigned mo, double* stats) {
+  return guard(-2, [&]() {
+    AdaptStats st;
+    int r = adapt(m->m, sf, gq, nl, mo, &st);
+    stats[0] = st.npass;
+    stats[1] = st.nelems;
+    stats[2] = st.tpass;
+    stats[3] = st.ttotal;
+    return r;
+  });
+}
+
+unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qf) {
+  return guard(OHX_ERROR, [&]() { return warp_to_limit(m->m, qf); });
+}
+
+osh_t ohx_mesh_osh(ohx_mesh* m) { return reinterpret_cast<osh_t>(m); }
+
+void ohx_set_verbose(int on) { g_verbose = on; }
+void ohx_profile_enable(int on) { prof_enable(on != 0); }
+void ohx_profile_reset(void) {
+  guard_void([&]() { prof_reset(); });
+}
+int ohx_profile_get(char const* name, double* ms, unsigned long long* launches) {
+  return guard(0, [&]() { return prof_get(name, ms, launches) ? 1 : 0; });
+}
+unsigned long long ohx_device_bytes_high_water(void) { return dev_bytes_high_water(); }
+void ohx_device_reset_high_water(void) { dev_reset_high_water(); }
+void ohx_synchronize(void) {
+  guard_void([&]() { sync(); });
+}
+unsigned ohx_swap_nint_edges(unsigned rs) {
+  return guard(OHX_ERROR, [&]() { return swap_nint_edges(rs); });
+}
+void ohx_swap_int_edges(unsigned rs, unsigned code, unsigned* out) {
+  guard_void([&]() { swap_int_edges_host(rs, code, out); });
+}
+
+// ---------------------------------------------------------- public osh_*
+void osh_init(int*, char***) {
+  osh_guard([]() {
+    rt();
+    return 0;
+  });
+}
+
+void osh_fini(void) {
+  try {
+    dev_trim();
+  } catch (...) {
+  }
+}
+
+void osh_free(osh_t m) { delete M(m); }
+
+osh_t osh_read_vtk(char const*) {
+  std::fprintf(stderr, "omega_h (MI355X): osh_read_vtk is not available in this build\n");
+  std::abort();
+}
+
+void osh_write_vtk(osh_t, char const*) {
+  std::fprintf(stderr, "omega_h (MI355X): osh_write_vtk is not available in this build\n");
+  std::abort();
+}
+
+osh_t osh_new(unsigned dim) {
+  return reinterpret_cast<osh_t>(new ohx_mesh(dim, true));
+}
+
+unsigned* osh_build_ents(osh_t m, unsigned d, unsigned n) {
+  return osh_guard([&]() -> unsigned* {
+    Mesh& mm = M(m)->m;
+    if (d == 0) {
+      set_ents(mm, 0, n, Buf<u32>());
+      return nullptr;
+    }
+    Buf<u32> b((size_t) n * h_down_deg[d][0]);
+    u32* p = b.get();
+    set_ents(mm, d, n, std::move(b));
+    return p;
+  });
+}
+
+unsigned osh_dim(osh_t m) { return M(m)->m.dim; }
+unsigned osh_nelems(osh_t m) { return osh_count(m, M(m)->m.dim); }
+unsigned osh_nverts(osh_t m) { return osh_count(m, 0); }
+unsigned osh_count(osh_t m, unsigned d) {
+  return osh_guard([&]() { return count(M(m)->m, d); });
+}
+
+unsigned const* osh_down(osh_t m, unsigned h, unsigned l) {
+  return osh_guard([&]() { return ask_down(M(m)->m, h, l); });
+}
+unsigned const* osh_up(osh_t m, unsigned l, unsigned h) {
+  return osh_guard(
+      [&]() -> unsigned const* { return ask_up(M(m)->m, l, h).adj.get(); });
+}
+unsigned const* osh_up_offs(osh_t m, unsigned l, unsigned h) {
+  return osh_guard(
+      [&]() -> unsigned const* { return ask_up(M(m)->m, l, h).offsets.get(); });
+}
+unsigned const* osh_up_dirs(osh_t m, unsigned l, unsigned h) {
+  return osh_guard(
+      [&]() -> unsigned const* { return ask_up(M(m)->m, l, h).dirs.get(); });
+}
+unsigned const* osh_star(osh_t m, unsigned l, unsigned h) {
+  return osh_guard(
+      [&]() -> unsigned const* { return ask_star(M(m)->m, l, h).adj.get(); });
+}
+unsigned const* osh_star_offs(osh_t m, unsigned l, unsigned h) {
+  return osh_guard(
+      [&]() -> unsigned const* { return ask_star(M(m)->m, l, h).offsets.get(); });
+}
+double const* osh_coords(osh_t m) {
+  return osh_guard([&]() { return coords(M(m)->m); });
+}
+
+unsigned const* osh_own_rank(osh_t m, unsigned d) {
+  return osh_guard([&]() -> unsigned const* {
+    ohx_mesh* mm = M(m);
+    mm->own_rank[d] = filled(count(mm->m, d), 0);
+    return mm->own_rank[d].get();
+  });
+}
+unsigned const* osh_own_id(osh_t m, unsigned d) {
+  return osh_guard([&]() -> unsigned const* {
+    ohx_mesh* mm = M(m);
+    mm->own_id[d] = iota(count(mm->m, d));
+    return mm->own_id[d].get();
+  });
+}
+unsigned long const* osh_global(osh_t m, unsigned d) {
+  return osh_guard([&]() -> unsigned long const* {
+    ohx_mesh* mm = M(m);
+    u32 n = count(mm->m, d);
+    mm->global[d] = Buf<unsigned long>(n);
+    unsigned long* g = mm->global[d].get();
+    parallel_for(n, OHX_LAMBDA(u32 i) { g[i] = i; });
+    return g;
+  });
+}
+
+double* osh_new_field(osh_t m, unsigned d, char const* name, unsigned nc, enum osh_transfer tt) {
+  return osh_guard([&]() -> double* {
+    Mesh& mm = M(m)->m;
+    if (Tag* t = find_tag(mm, d, name)) return t->f.get();
+    return add_tag_f64(mm, d, name, nc, Buf<double>((size_t) count(mm, d) * nc), (int) tt).f.get();
+  });
+}
+double* osh_get_field(osh_t m, unsigned d, char const* name) {
+  return osh_guard([&]() -> double* {
+    Tag* t = find_tag(M(m)->m, d, name);
+    OHX_REQUIRE(t && t->type == TagType::F64, "no such field");
+    return t->f.get();
+  });
+}
+void osh_free_field(osh_t m, unsigned d, char const* name) {
+  osh_guard([&]() {
+    remove_tag(M(m)->m, d, name);
+    return 0;
+  });
+}
+unsigned osh_nfields(osh_t m, unsigned d) {
+  unsigned n = 0;
+  for (auto& t : M(m)->m.tags[d]) n += (t->type == TagType::F64);
+  return n;
+}
+char const* osh_field(osh_t m, unsigned d, unsigned i) {
+  unsigned n = 0;
+  for (auto& t : M(m)->m.tags[d])
+    if (t->type == TagType::F64 && i == n++) return t->name.c_str();
+  return nullptr;
+}
+unsigned osh_components(osh_t m, unsigned d, char const* name) {
+  Tag* t = find_tag(M(m)->m, d, name);
+  return t ? t->ncomps : 0;
+}
+unsigned* osh_new_label(osh_t m, unsigned d, char const* name, unsigned nc) {
+  return osh_guard([&]() -> unsigned* {
+    Mesh& mm = M(m)->m;
+    if (Tag* t = find_tag(mm, d, name)) return t->u.get();
+    return add_tag_u32(mm, d, name, nc, Buf<u32>((size_t) count(mm, d) * nc)).u.get();
+  });
+}
+unsigned* osh_get_label(osh_t m, unsigned d, char const* name) {
+  return osh_guard([&]() -> unsigned* {
+    Tag* t = find_tag(M(m)->m, d, name);
+    OHX_REQUIRE(t && t->type == TagType::U32, "no such label");
+    return t->u.get();
+  });
+}
+void osh_free_label(osh_t m, unsigned d, char const* name) { osh_free_field(m, d, name); }
+
+unsigned long* osh_new_global(osh_t m, unsigned d) {
+  return const_cast<unsigned long*>(osh_global(m, d));
+}
+void osh_accumulate_to_owner(osh_t, unsigned, char const*) {}
+void osh_conform(osh_t, unsigned, char const*) {}
+
+void osh_mark_classified(osh_t m, unsigned ent_dim, unsigned class_dim, unsigned class_id, unsigned* marked) {
+  osh_guard([&]() {
+    Buf<u32> mk = mark_class(M(m)->m, ent_dim, class_dim, class_id);
+    u32 const* k = mk.get();
+    parallel_for(count(M(m)->m, ent_dim), OHX_LAMBDA(u32 i) {
+      if (k[i]) marked[i] = 1;
+    });
+    sync();
+    return 0;
+  });
+}
+
+void osh_ghost(osh_t, unsigned nlayers) {
+  if (nlayers) {
+    std::fprintf(stderr, "omega_h (MI355X): ghost layers need MPI ranks; this build holds the mesh on one GPU\n");
+    std::abort();
+  }
+}
+unsigned osh_num_ghost_layers(osh_t) { return 0; }
+
+unsigned osh_adapt(osh_t m, double sf, double gq, unsigned nl, unsigned mp) {
+  return osh_guard([&]() -> unsigned {
+    int r = adapt(M(m)->m, sf, gq, nl, mp, nullptr);
+    if (r < 0) {
+      std::fprintf(stderr, "mesh_adapt could not succeed after %u operations\n", mp);
+      std::abort();
+    }
+    return (unsigned) r;
+  });
+}
+
+void osh_identity_size(osh_t m, char const* name) {
+  osh_guard([&]() {
+    identity_size(M(m)->m, name);
+    return 0;
+  });
+}
+
+}  // extern "C"

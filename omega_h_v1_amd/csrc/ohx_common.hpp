@@ -1,0 +1,179 @@
+// Common definitions of the MI355X (gfx950) adapt-path library.
+//
+// One process, one GPU, one HIP stream per library instance.  Every mesh
+// array lives in HBM; the host keeps only counts and control flow.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace ohx {
+
+using u32 = unsigned;
+using u64 = unsigned long long;
+constexpr u32 INVALID = ~0u;
+constexpr u32 MAX_UP = 128;
+constexpr u32 MAX_EDGE_SWAP = 7;
+constexpr double REF_PI = 3.14159265359;  // algebra.hpp:8
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void fail(char const* what, char const* file, int line);
+#define OHX_CHECK(x)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess)                                                  \
+      ::ohx::fail(hipGetErrorString(e_), __FILE__, __LINE__);              \
+  } while (0)
+#define OHX_REQUIRE(c, msg)                                                \
+  do {                                                                     \
+    if (!(c)) ::ohx::fail(msg, __FILE__, __LINE__);                        \
+  } while (0)
+
+// --------------------------------------------------------------- runtime
+// The library's stream and a small pinned staging area for the scalars the
+// host reads back (scan totals, reduction results, convergence flags).
+struct Runtime {
+  hipStream_t stream = nullptr;
+  void* pinned = nullptr;  // 4 KiB of hipHostMalloc'd memory
+  int device = 0;
+};
+Runtime& rt();
+// a caller-provided stream for the duration of one kernel-ABI call
+extern hipStream_t g_stream_override;
+inline hipStream_t stream() { return g_stream_override ? g_stream_override : rt().stream; }
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(g_stream_override) { if (s) g_stream_override = s; }
+  // temporaries of the call go back to the arena, which reuses blocks in
+  // library-stream order: finish the caller's stream before that
+  ~StreamScope() {
+    if (g_stream_override && g_stream_override != prev) (void) hipStreamSynchronize(g_stream_override);
+    g_stream_override = prev;
+  }
+};
+void sync();
+
+// ------------------------------------------------------------------ arena
+// Caching allocator over hipMalloc: power-of-two size classes, free lists
+// reused in stream order (all work is on one stream, so a block freed by
+// the host may be handed to the next launch without an event).
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+size_t dev_bytes_in_use();
+size_t dev_bytes_high_water();
+void dev_reset_high_water();
+void dev_trim();
+
+template <class T>
+class Buf {
+ public:
+  Buf() = default;
+  // always allocates (like malloc(0) in the reference), so an existing
+  // but empty array stays distinguishable from an absent one
+  explicit Buf(size_t n) : n_(n) { p_ = static_cast<T*>(dev_alloc(n ? n * sizeof(T) : sizeof(T))); }
+  ~Buf() { release(); }
+  Buf(Buf const&) = delete;
+  Buf& operator=(Buf const&) = delete;
+  Buf(Buf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  Buf& operator=(Buf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  void release() {
+    if (p_) dev_free(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+  explicit operator bool() const { return p_ != nullptr; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// -------------------------------------------------------- host <-> device
+template <class T>
+void to_host(T* dst, T const* src, size_t n) {
+  if (!n) return;
+  OHX_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream()));
+  sync();
+}
+template <class T>
+Buf<T> to_device(T const* src, size_t n) {
+  Buf<T> b(n);
+  if (n) {
+    OHX_CHECK(hipMemcpyAsync(b.get(), src, n * sizeof(T), hipMemcpyHostToDevice, stream()));
+    sync();
+  }
+  return b;
+}
+template <class T>
+std::vector<T> host_copy(T const* src, size_t n) {
+  std::vector<T> v(n);
+  to_host(v.data(), src, n);
+  return v;
+}
+// read one device scalar (one D2H round trip)
+template <class T>
+T read_at(T const* p, size_t i) {
+  T* h = static_cast<T*>(rt().pinned);
+  OHX_CHECK(hipMemcpyAsync(h, p + i, sizeof(T), hipMemcpyDeviceToHost, stream()));
+  sync();
+  return *h;
+}
+template <class T>
+void d2d(T* dst, T const* src, size_t n) {
+  if (!n) return;
+  OHX_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+}
+
+// ---------------------------------------------------------------- launch
+constexpr u32 BLOCK = 256;
+inline u32 nblocks(size_t n, u32 block = BLOCK) { return static_cast<u32>((n + block - 1) / block); }
+void check_launch(char const* name);
+
+template <class F>
+__global__ __launch_bounds__(BLOCK) void for_kernel(u32 n, F f) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) f(i);
+}
+// one thread per index; the body is a __device__ lambda capturing by value
+template <class F>
+void parallel_for(u32 n, F f) {
+  if (!n) return;
+  hipLaunchKernelGGL(for_kernel<F>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, f);
+  check_launch("parallel_for");
+}
+#define OHX_LAMBDA [=] __device__
+
+// ---------------------------------------------------------------- timing
+// Per-kernel-name HIP event timers, enabled by ohx_profile_enable().
+void prof_begin(char const* name);
+void prof_end(char const* name);
+struct ProfScope {
+  char const* name;
+  explicit ProfScope(char const* n) : name(n) { prof_begin(n); }
+  ~ProfScope() { prof_end(name); }
+};
+
+}  // namespace ohx

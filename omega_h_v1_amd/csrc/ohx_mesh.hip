@@ -1,0 +1,1032 @@
+// Mesh container, lazy topology derivation, quality kernels and marks.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ohx_geom.hpp"
+#include "ohx_mesh.hpp"
+#include "ohx_prims.hpp"
+#include "ohx_tables.hpp"
+
+namespace ohx {
+
+// ------------------------------------------------------------- bookkeeping
+void set_ents(Mesh& m, u32 d, u32 n, Buf<u32>&& verts) {
+  OHX_REQUIRE(!m.down[d][0] || d == 0, "entities of this dimension were already set");
+  m.counts[d] = n;
+  if (d) m.down[d][0] = std::move(verts);
+}
+
+bool has_dim(Mesh& m, u32 d) { return d == 0 || bool(m.down[d][0]); }
+
+u32 count(Mesh& m, u32 d) {
+  if (d > m.dim) return 0;
+  if (d) ask_down(m, d, 0);
+  return m.counts[d];
+}
+
+void overwrite(Mesh& m, Mesh&& w) {
+  m.dim = w.dim;
+  m.full = w.full;
+  for (u32 i = 0; i < 4; ++i) {
+    m.counts[i] = w.counts[i];
+    m.tags[i] = std::move(w.tags[i]);
+    for (u32 j = 0; j < 4; ++j) {
+      m.down[i][j] = std::move(w.down[i][j]);
+      m.up[i][j] = std::move(w.up[i][j]);
+      m.star[i][j] = std::move(w.star[i][j]);
+    }
+  }
+  m.dual = std::move(w.dual);
+}
+
+Tag* find_tag(Mesh& m, u32 d, char const* name) {
+  for (auto& t : m.tags[d])
+    if (t->name == name) return t.get();
+  return nullptr;
+}
+
+static Tag& push_tag(Mesh& m, u32 d, char const* name, TagType type, u32 ncomps, int transfer) {
+  OHX_REQUIRE(!find_tag(m, d, name), "tag already exists");
+  auto t = std::make_unique<Tag>();
+  t->name = name;
+  t->type = type;
+  t->ncomps = ncomps;
+  t->transfer = transfer;
+  m.tags[d].push_back(std::move(t));
+  return *m.tags[d].back();
+}
+
+Tag& add_tag_u32(Mesh& m, u32 d, char const* name, u32 ncomps, Buf<u32>&& data, int transfer) {
+  Tag& t = push_tag(m, d, name, TagType::U32, ncomps, transfer);
+  t.u = std::move(data);
+  return t;
+}
+
+Tag& add_tag_f64(Mesh& m, u32 d, char const* name, u32 ncomps, Buf<double>&& data, int transfer) {
+  Tag& t = push_tag(m, d, name, TagType::F64, ncomps, transfer);
+  t.f = std::move(data);
+  return t;
+}
+
+void remove_tag(Mesh& m, u32 d, char const* name) {
+  auto& v = m.tags[d];
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i]->name == name) {
+      v.erase(v.begin() + i);
+      return;
+    }
+  fail("removing a tag that does not exist", __FILE__, __LINE__);
+}
+
+double const* coords(Mesh& m) {
+  Tag* t = find_tag(m, 0, "coordinates");
+  OHX_REQUIRE(t && t->type == TagType::F64, "mesh has no coordinates");
+  return t->f.get();
+}
+
+u32 const* u32_tag(Mesh& m, u32 d, char const* name) {
+  Tag* t = find_tag(m, d, name);
+  OHX_REQUIRE(t && t->type == TagType::U32, "missing integer tag");
+  return t->u.get();
+}
+
+// ---------------------------------------------------------------- quality
+// One thread per element: gather the vertex coordinates, evaluate the
+// reference's quality formula in FP64 (quality.cpp:13-45).
+template <u32 DIM>
+__global__ __launch_bounds__(BLOCK) void elem_quality_f64(u32 n, u32 const* __restrict__ ev,
+                                                          double const* __restrict__ x,
+                                                          double* __restrict__ out) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  V3 p[DIM + 1];
+#pragma unroll
+  for (u32 j = 0; j <= DIM; ++j) p[j] = ld3(x + 3 * (size_t) ev[(size_t) i * (DIM + 1) + j]);
+  out[i] = element_quality(DIM, p);
+}
+
+// Fused quality + minimum over elements begin..n-1 (no per-element store).
+template <u32 DIM>
+__global__ __launch_bounds__(BLOCK) void elem_min_quality_f64(u32 n, u32 begin,
+                                                              u32 const* __restrict__ ev,
+                                                              double const* __restrict__ x,
+                                                              double* __restrict__ part) {
+  __shared__ double lds[BLOCK / 64];
+  double v = 1.7976931348623157e308;
+  for (u32 i = begin + blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    V3 p[DIM + 1];
+#pragma unroll
+    for (u32 j = 0; j <= DIM; ++j) p[j] = ld3(x + 3 * (size_t) ev[(size_t) i * (DIM + 1) + j]);
+    double q = element_quality(DIM, p);
+    if (q < v) v = q;
+  }
+#pragma unroll
+  for (u32 d = 32; d >= 1; d >>= 1) {
+    double o = __shfl_down(v, d, 64);
+    if (o < v) v = o;
+  }
+  if ((threadIdx.x & 63u) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = lds[0];
+    for (u32 w = 1; w < BLOCK / 64; ++w)
+      if (lds[w] < r) r = lds[w];
+    part[blockIdx.x] = r;
+  }
+}
+
+Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x) {
+  Buf<double> q(n);
+  if (!n) return q;
+  ProfScope ps("elem_quality_f64");
+  if (dim == 3)
+    hipLaunchKernelGGL(elem_quality_f64<3>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, q.get());
+  else
+    hipLaunchKernelGGL(elem_quality_f64<2>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, q.get());
+  check_launch("elem_quality_f64");
+  return q;
+}
+
+Buf<double> mesh_qualities(Mesh& m) {
+  return element_qualities(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m));
+}
+
+// min over elements 1..n-1: the serial build's doubles_min starts at 1
+// (doubles.cpp:111-118), and mesh_min_quality goes through it
+double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x) {
+  if (n <= 1) return 1.7976931348623157e308;
+  u32 nb = std::min<u32>(nblocks(n - 1), 1024);
+  Buf<double> part(nb);
+  {
+    ProfScope ps("elem_min_quality_f64");
+    if (dim == 3)
+      hipLaunchKernelGGL(elem_min_quality_f64<3>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, part.get());
+    else
+      hipLaunchKernelGGL(elem_min_quality_f64<2>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, part.get());
+    check_launch("elem_min_quality_f64");
+  }
+  return min_f64(part.get(), nb, 0);
+}
+
+double mesh_min_quality(Mesh& m) {
+  return min_element_quality(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m));
+}
+
+// measure_edge (size.cpp:57-88): |x1 - x0| / ((h0 + h1) / 2)
+__global__ __launch_bounds__(BLOCK) void measure_edge_f64(u32 n, u32 const* __restrict__ ev,
+                                                          double const* __restrict__ x,
+                                                          double const* __restrict__ h,
+                                                          double* __restrict__ out) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  u32 a = ev[2 * i], b = ev[2 * i + 1];
+  double len = sqrt(sqdist(ld3(x + 3 * (size_t) b), ld3(x + 3 * (size_t) a)));
+  double want = (h[a] + h[b]) / 2;
+  out[i] = len / want;
+}
+
+Buf<double> measure_edges(Mesh& m) {
+  u32 ne = count(m, 1);
+  Tag* st = find_tag(m, 0, "adapt_size");
+  OHX_REQUIRE(st && st->type == TagType::F64, "mesh has no adapt_size field");
+  Buf<double> out(ne);
+  if (ne) {
+    ProfScope ps("measure_edge_f64");
+    hipLaunchKernelGGL(measure_edge_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+                       ask_down(m, 1, 0), coords(m), st->f.get(), out.get());
+    check_launch("measure_edge_f64");
+  }
+  return out;
+}
+
+// --------------------------------------------------------- up_from_down
+// Upward adjacency as a stable counting sort of uses by low entity
+// (invert_map.cpp:72-136 + up_from_down.cpp:9-47): per low, the uses in
+// increasing use index (= high * deg + direction).  Counts and slots come
+// from atomics; each bucket is then sorted, which makes the result exact.
+__global__ __launch_bounds__(BLOCK) void upfd_count(u32 nuses, u32 const* __restrict__ in,
+                                                    u32* __restrict__ counts) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < nuses) atomicAdd(&counts[in[i]], 1u);
+}
+
+__global__ __launch_bounds__(BLOCK) void upfd_fill(u32 nuses, u32 const* __restrict__ in,
+                                                   u32 const* __restrict__ offs, u32* __restrict__ cursor,
+                                                   u32* __restrict__ out) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nuses) return;
+  u32 d = in[i];
+  u32 j = atomicAdd(&cursor[d], 1u);
+  out[offs[d] + j] = i;
+}
+
+__global__ __launch_bounds__(BLOCK) void upfd_sort_split(u32 nlows, u32 deg, u32 const* __restrict__ offs,
+                                                         u32* __restrict__ adj, u32* __restrict__ dirs) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nlows) return;
+  u32 f = offs[i], e = offs[i + 1];
+  // insertion sort of the bucket (a dozen to a few dozen entries)
+  for (u32 j = f + 1; j < e; ++j) {
+    u32 v = adj[j];
+    u32 k = j;
+    while (k > f && adj[k - 1] > v) {
+      adj[k] = adj[k - 1];
+      --k;
+    }
+    adj[k] = v;
+  }
+  for (u32 j = f; j < e; ++j) {
+    u32 both = adj[j];
+    u32 h = both / deg;
+    adj[j] = h;
+    dirs[j] = both - h * deg;
+  }
+}
+
+void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_highs, Up& u) {
+  ProfScope ps("up_from_down");
+  u32 deg = h_down_deg[hd][ld];
+  u32 nuses = nhighs * deg;
+  Buf<u32> counts = filled(nlows, 0);
+  if (nuses)
+    hipLaunchKernelGGL(upfd_count, dim3(nblocks(nuses)), dim3(BLOCK), 0, stream(), nuses, lows_of_highs,
+                       counts.get());
+  u.offsets = exscan(counts.get(), nlows);
+  fill(counts.get(), nlows, 0);
+  u.adj = Buf<u32>(nuses);
+  u.dirs = Buf<u32>(nuses);
+  if (nuses)
+    hipLaunchKernelGGL(upfd_fill, dim3(nblocks(nuses)), dim3(BLOCK), 0, stream(), nuses, lows_of_highs,
+                       u.offsets.get(), counts.get(), u.adj.get());
+  if (nlows)
+    hipLaunchKernelGGL(upfd_sort_split, dim3(nblocks(nlows)), dim3(BLOCK), 0, stream(), nlows, deg,
+                       u.offsets.get(), u.adj.get(), u.dirs.get());
+  check_launch("up_from_down");
+}
+
+// ---------------------------------------------------------- reflect_down
+// For every high entity and each of its canonical lows, find the low with
+// those vertices among the lows adjacent to the first vertex
+// (reflect_down.cpp:13-81, find_by_verts.hpp:14-76).  The answer is unique
+// in a conforming mesh, so the search order does not matter.
+template <u32 HD, u32 LD>
+__global__ __launch_bounds__(BLOCK) void reflect_down_kernel(u32 nh, u32 const* __restrict__ vh,
+                                                             u32 const* __restrict__ vl,
+                                                             u32 const* __restrict__ uoff,
+                                                             u32 const* __restrict__ uadj,
+                                                             u32 const* __restrict__ udir,
+                                                             u32* __restrict__ out, u32* __restrict__ err) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nh) return;
+  constexpr u32 VPH = HD + 1, VPL = LD + 1;
+  constexpr u32 LPH = (HD == 3) ? (LD == 1 ? 6 : 4) : 3;
+  u32 hv[VPH];
+#pragma unroll
+  for (u32 k = 0; k < VPH; ++k) hv[k] = vh[(size_t) i * VPH + k];
+#pragma unroll
+  for (u32 j = 0; j < LPH; ++j) {
+    u32 const* lc = canon(HD, LD, 0, j);
+    u32 w0 = hv[lc[0]], w1 = hv[lc[1]], w2 = (VPL == 3) ? hv[lc[VPL - 1]] : 0u;
+    u32 found = INVALID;
+    for (u32 k = uoff[w0]; k < uoff[w0 + 1]; ++k) {
+      u32 e = uadj[k], d = udir[k];
+      if (VPL == 2) {
+        if (vl[2 * (size_t) e + (1 - d)] == w1) {
+          found = e;
+          break;
+        }
+      } else {
+        u32 const* t = vl + 3 * (size_t) e;
+        u32 a = t[(d + 1) % 3], b = t[(d + 2) % 3];
+        if ((a == w1 && b == w2) || (a == w2 && b == w1)) {
+          found = e;
+          break;
+        }
+      }
+    }
+    if (found == INVALID) *err = 1u;
+    out[(size_t) i * LPH + j] = found;
+  }
+}
+
+static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
+  u32 nh = count(m, hd);
+  u32 const* vh = ask_down(m, hd, 0);
+  Up const& lv = ask_up(m, 0, ld);
+  u32 const* vl = ask_down(m, ld, 0);
+  u32 lph = h_down_deg[hd][ld];
+  Buf<u32> out((size_t) nh * lph);
+  if (!nh) return out;
+  u32* err = scratch_words();
+  fill(err, 1, 0);
+  ProfScope ps("reflect_down");
+  dim3 g(nblocks(nh)), b(BLOCK);
+  if (hd == 3 && ld == 1)
+    hipLaunchKernelGGL((reflect_down_kernel<3, 1>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
+                       lv.adj.get(), lv.dirs.get(), out.get(), err);
+  else if (hd == 3 && ld == 2)
+    hipLaunchKernelGGL((reflect_down_kernel<3, 2>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
+                       lv.adj.get(), lv.dirs.get(), out.get(), err);
+  else if (hd == 2 && ld == 1)
+    hipLaunchKernelGGL((reflect_down_kernel<2, 1>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
+                       lv.adj.get(), lv.dirs.get(), out.get(), err);
+  else
+    fail("reflect_down: unsupported dimensions", __FILE__, __LINE__);
+  check_launch("reflect_down");
+  OHX_REQUIRE(read_at(err, 0) == 0, "reflect_down: a sub-entity was not found (non-conforming mesh)");
+  return out;
+}
+
+// ----------------------------------------------------------------- stars
+// get_star_general (star.cpp:13-111): lows of the highs around each low,
+// first occurrence kept, in up-adjacency order
+static Graph star_general(Mesh& m, u32 ld, u32 hd) {
+  u32 nl = count(m, ld);
+  Up const& u = ask_up(m, ld, hd);
+  u32 const* lh = ask_down(m, hd, ld);
+  u32 lph = h_down_deg[hd][ld];
+  u32 const* uo = u.offsets.get();
+  u32 const* ua = u.adj.get();
+  Buf<u32> deg(nl);
+  u32* dg = deg.get();
+  // count pass: uniqueness by scanning what came before (no buffer needed)
+  parallel_for(nl, OHX_LAMBDA(u32 i) {
+    u32 size = 0;
+    for (u32 j = uo[i]; j < uo[i + 1]; ++j)
+      for (u32 k = 0; k < lph; ++k) {
+        u32 s = lh[(size_t) ua[j] * lph + k];
+        if (s == i) continue;
+        bool seen = false;
+        for (u32 j2 = uo[i]; j2 <= j && !seen; ++j2)
+          for (u32 k2 = 0; k2 < lph; ++k2) {
+            if (j2 == j && k2 == k) break;
+            if (lh[(size_t) ua[j2] * lph + k2] == s) {
+              seen = true;
+              break;
+            }
+          }
+        if (!seen) ++size;
+      }
+    dg[i] = size;
+  });
+  Graph g;
+  g.offsets = exscan(dg, nl);
+  u32 nadj = read_at(g.offsets.get(), nl);
+  g.adj = Buf<u32>(nadj);
+  u32* ga = g.adj.get();
+  u32 const* go = g.offsets.get();
+  parallel_for(nl, OHX_LAMBDA(u32 i) {
+    u32 base = go[i], size = 0;
+    for (u32 j = uo[i]; j < uo[i + 1]; ++j)
+      for (u32 k = 0; k < lph; ++k) {
+        u32 s = lh[(size_t) ua[j] * lph + k];
+        if (s == i) continue;
+        bool seen = false;
+        for (u32 t = 0; t < size; ++t)
+          if (ga[base + t] == s) {
+            seen = true;
+            break;
+          }
+        if (!seen) ga[base + size++] = s;
+      }
+  });
+  return g;
+}
+
+// get_vertex_edge_star (star.cpp:113-153)
+static Graph star_vertex_edge(Mesh& m) {
+  u32 nv = count(m, 0);
+  u32 const* ev = ask_down(m, 1, 0);
+  Up const& u = ask_up(m, 0, 1);
+  Graph g;
+  g.offsets = copy(u.offsets.get(), nv + 1);
+  u32 nadj = read_at(u.offsets.get(), nv);
+  g.adj = Buf<u32>(nadj);
+  u32* ga = g.adj.get();
+  u32 const* ua = u.adj.get();
+  u32 const* ud = u.dirs.get();
+  parallel_for(nadj, OHX_LAMBDA(u32 j) { ga[j] = ev[2 * (size_t) ua[j] + (1 - ud[j])]; });
+  return g;
+}
+
+// get_edge_triangle_star (star.cpp:155-200)
+static Graph star_edge_tri(Mesh& m) {
+  u32 ne = count(m, 1);
+  u32 const* et = ask_down(m, 2, 1);
+  Up const& u = ask_up(m, 1, 2);
+  Graph g;
+  g.offsets = Buf<u32>(ne + 1);
+  u32* go = g.offsets.get();
+  u32 const* uo = u.offsets.get();
+  parallel_for(ne + 1, OHX_LAMBDA(u32 i) { go[i] = uo[i] * 2; });
+  u32 nuse = read_at(uo, ne);
+  g.adj = Buf<u32>((size_t) nuse * 2);
+  u32* ga = g.adj.get();
+  u32 const* ua = u.adj.get();
+  u32 const* ud = u.dirs.get();
+  parallel_for(nuse, OHX_LAMBDA(u32 j) {
+    u32 t = ua[j], d = ud[j];
+    ga[2 * j + 0] = et[3 * (size_t) t + (d + 1) % 3];
+    ga[2 * j + 1] = et[3 * (size_t) t + (d + 2) % 3];
+  });
+  return g;
+}
+
+// get_edge_tet_star (star.cpp:202-278): the triangle star followed by the
+// edge opposite to this one in every adjacent tet
+static Graph star_edge_tet(Mesh& m) {
+  u32 ne = count(m, 1);
+  u32 const* et = ask_down(m, 2, 1);
+  u32 const* er = ask_down(m, 3, 1);
+  Up const& ut = ask_up(m, 1, 2);
+  Up const& ur = ask_up(m, 1, 3);
+  u32 const* to = ut.offsets.get();
+  u32 const* ta = ut.adj.get();
+  u32 const* td = ut.dirs.get();
+  u32 const* ro = ur.offsets.get();
+  u32 const* ra = ur.adj.get();
+  u32 const* rd = ur.dirs.get();
+  Buf<u32> deg(ne);
+  u32* dg = deg.get();
+  parallel_for(ne, OHX_LAMBDA(u32 i) { dg[i] = 2 * (to[i + 1] - to[i]) + (ro[i + 1] - ro[i]); });
+  Graph g;
+  g.offsets = exscan(dg, ne);
+  g.adj = Buf<u32>(read_at(g.offsets.get(), ne));
+  u32* ga = g.adj.get();
+  u32 const* go = g.offsets.get();
+  parallel_for(ne, OHX_LAMBDA(u32 i) {
+    u32 o = go[i];
+    for (u32 j = to[i]; j < to[i + 1]; ++j) {
+      u32 t = ta[j], d = td[j];
+      ga[o++] = et[3 * (size_t) t + (d + 1) % 3];
+      ga[o++] = et[3 * (size_t) t + (d + 2) % 3];
+    }
+    u32 const* oe = opp(3, 1);
+    for (u32 j = ro[i]; j < ro[i + 1]; ++j) ga[o++] = er[6 * (size_t) ra[j] + oe[rd[j]]];
+  });
+  return g;
+}
+
+// mesh_get_star (star.cpp:280-315)
+static Graph get_star(Mesh& m, u32 ld, u32 hd) {
+  if (ld == 0 && hd == m.dim && m.dim != 1 && has_dim(m, 1)) return get_star(m, 0, 1);
+  if (ld == 0 && hd == 1) return star_vertex_edge(m);
+  if (ld == 1 && hd == 2) return star_edge_tri(m);
+  if (ld == 1 && hd == 3 && has_dim(m, 2)) return star_edge_tet(m);
+  return star_general(m, ld, hd);
+}
+
+// ------------------------------------------------------------------ dual
+// element_dual_from_verts (dual.cpp:40-98)
+static Buf<u32> dual_from_verts(Mesh& m) {
+  u32 dim = m.dim, sd = dim - 1;
+  u32 ne = count(m, dim);
+  u32 const* ev = ask_down(m, dim, 0);
+  Up const& u = ask_up(m, 0, dim);
+  u32 vpe = dim + 1, spe = h_down_deg[dim][sd], vps = h_down_deg[sd][0];
+  Buf<u32> out((size_t) ne * spe);
+  u32* o = out.get();
+  u32 const* uo = u.offsets.get();
+  u32 const* ua = u.adj.get();
+  u32* err = scratch_words();
+  fill(err, 1, 0);
+  parallel_for(ne, OHX_LAMBDA(u32 i) {
+    for (u32 j = 0; j < spe; ++j) {
+      u32 const* sc = canon(dim, sd, 0, j);
+      // elements of the first side vertex other than i that also contain
+      // every other side vertex
+      u32 v0 = ev[(size_t) i * vpe + sc[0]];
+      u32 found = INVALID, nfound = 0;
+      for (u32 a = uo[v0]; a < uo[v0 + 1]; ++a) {
+        u32 c = ua[a];
+        if (c == i) continue;
+        bool all = true;
+        for (u32 k = 1; k < vps && all; ++k) {
+          u32 v = ev[(size_t) i * vpe + sc[k]];
+          bool has = false;
+          for (u32 b = uo[v]; b < uo[v + 1]; ++b)
+            if (ua[b] == c) {
+              has = true;
+              break;
+            }
+          all = has;
+        }
+        if (all) {
+          if (!nfound) found = c;
+          ++nfound;
+        }
+      }
+      if (nfound > 1) *err = 1u;
+      o[(size_t) i * spe + j] = found;
+    }
+  });
+  OHX_REQUIRE(read_at(err, 0) == 0, "dual_from_verts: non-manifold side");
+  return out;
+}
+
+// element_dual_from_sides (dual.cpp:116-156)
+static Buf<u32> dual_from_sides(Mesh& m) {
+  u32 dim = m.dim;
+  u32 ne = count(m, dim);
+  u32 spe = h_down_deg[dim][dim - 1];
+  u32 const* se = ask_down(m, dim, dim - 1);
+  Up const& u = ask_up(m, dim - 1, dim);
+  Buf<u32> out((size_t) ne * spe);
+  u32* o = out.get();
+  u32 const* uo = u.offsets.get();
+  u32 const* ua = u.adj.get();
+  parallel_for(ne, OHX_LAMBDA(u32 i) {
+    for (u32 j = 0; j < spe; ++j) {
+      u32 s = se[(size_t) i * spe + j];
+      u32 r = INVALID;
+      for (u32 k = uo[s]; k < uo[s + 1]; ++k)
+        if (ua[k] != i) {
+          r = ua[k];
+          break;
+        }
+      o[(size_t) i * spe + j] = r;
+    }
+  });
+  return out;
+}
+
+// bridge_graph_general (bridge_graph.cpp:12-80)
+void bridge_graph(u32 n, u32 const* offs, u32 const* adj, u32* p_nedges, Buf<u32>* ev, Buf<u32>* dirs) {
+  Buf<u32> deg(n);
+  u32* dg = deg.get();
+  parallel_for(n, OHX_LAMBDA(u32 i) {
+    u32 c = 0;
+    for (u32 j = offs[i]; j < offs[i + 1]; ++j)
+      if (i < adj[j]) ++c;
+    dg[i] = c;
+  });
+  Buf<u32> bo = exscan(dg, n);
+  u32 ne = read_at(bo.get(), n);
+  *ev = Buf<u32>((size_t) ne * 2);
+  if (dirs) *dirs = Buf<u32>(ne);
+  u32* e2 = ev->get();
+  u32* ed = dirs ? dirs->get() : nullptr;
+  u32 const* b = bo.get();
+  parallel_for(n, OHX_LAMBDA(u32 i) {
+    u32 e = b[i];
+    for (u32 j = offs[i]; j < offs[i + 1]; ++j)
+      if (i < adj[j]) {
+        e2[2 * (size_t) e + 0] = i;
+        e2[2 * (size_t) e + 1] = adj[j];
+        if (ed) ed[e] = j - offs[i];
+        ++e;
+      }
+  });
+  *p_nedges = ne;
+}
+
+// ------------------------------------------------------------ lazy caches
+// mesh_ask_down (mesh.cpp:136-181)
+u32 const* ask_down(Mesh& m, u32 h, u32 l) {
+  OHX_REQUIRE(h <= m.dim && l <= h, "ask_down: bad dimensions");
+  if (m.down[h][l]) return m.down[h][l].get();
+  if (l == h) {
+    m.down[h][l] = iota(m.counts[h]);
+  } else if (l > 0) {
+    m.down[h][l] = reflect_down(m, h, l);
+  } else {
+    OHX_REQUIRE(!m.full, "ask_down: missing entities in a full mesh");
+    if (h == 1 && m.dim == 3) {
+      // 3D edges from the vertex-to-vertex star across tets
+      Graph const& g = ask_star(m, 0, m.dim);
+      u32 ne;
+      Buf<u32> ev;
+      bridge_graph(m.counts[0], g.offsets.get(), g.adj.get(), &ne, &ev, nullptr);
+      set_ents(m, 1, ne, std::move(ev));
+    } else {
+      // sides: bridge the element dual, then take each side's vertices
+      // from its first element (bridge_graph.cpp:82-108, derive_sides.cpp:10-52)
+      OHX_REQUIRE(h == m.dim - 1, "ask_down: cannot derive this dimension");
+      u32 const* dual = ask_dual(m);
+      u32 dim = m.dim, ne = m.counts[dim];
+      u32 spe = h_down_deg[dim][dim - 1];
+      Buf<u32> offs(ne + 1);
+      u32* op = offs.get();
+      parallel_for(ne + 1, OHX_LAMBDA(u32 i) { op[i] = i * spe; });
+      u32 ns;
+      Buf<u32> es, ess;
+      bridge_graph(ne, op, dual, &ns, &es, &ess);
+      u32 const* ev = m.down[dim][0].get();
+      u32 vps = h_down_deg[dim - 1][0], vpe = dim + 1;
+      Buf<u32> sv((size_t) ns * vps);
+      u32* s = sv.get();
+      u32 const* e = es.get();
+      u32 const* ed = ess.get();
+      parallel_for(ns, OHX_LAMBDA(u32 i) {
+        u32 el = e[2 * (size_t) i];
+        u32 const* sc = canon(dim, dim - 1, 0, ed[i]);
+        for (u32 j = 0; j < vps; ++j) s[(size_t) i * vps + j] = ev[(size_t) el * vpe + sc[j]];
+      });
+      set_ents(m, h, ns, std::move(sv));
+    }
+  }
+  return m.down[h][l].get();
+}
+
+// mesh_ask_up (mesh.cpp:190-217)
+Up const& ask_up(Mesh& m, u32 l, u32 h) {
+  if (m.up[l][h]) return *m.up[l][h];
+  auto u = std::make_unique<Up>();
+  if (l == h) {
+    u32 n = m.counts[h];
+    u->offsets = iota(n + 1);
+    u->adj = iota(n);
+    u->dirs = filled(n, 0);
+  } else {
+    u32 const* lh = ask_down(m, h, l);
+    up_from_down(h, l, m.counts[h], m.counts[l], lh, *u);
+  }
+  m.up[l][h] = std::move(u);
+  return *m.up[l][h];
+}
+
+// mesh_ask_star (mesh.cpp:226-244)
+Graph const& ask_star(Mesh& m, u32 l, u32 h) {
+  if (m.star[l][h]) return *m.star[l][h];
+  auto g = std::make_unique<Graph>();
+  if (l == h) {
+    g->offsets = filled(m.counts[l] + 1, 0);
+    g->adj = Buf<u32>(1);
+  } else {
+    *g = get_star(m, l, h);
+  }
+  m.star[l][h] = std::move(g);
+  return *m.star[l][h];
+}
+
+// mesh_ask_dual (mesh.cpp:252-261)
+u32 const* ask_dual(Mesh& m) {
+  if (!m.dual) {
+    if (has_dim(m, m.dim - 1))
+      m.dual = dual_from_sides(m);
+    else
+      m.dual = dual_from_verts(m);
+  }
+  return m.dual.get();
+}
+
+// ----------------------------------------------------------------- marks
+Buf<u32> mark_down(Mesh& m, u32 hd, u32 ld, u32 const* mh) {
+  u32 nl = count(m, ld);
+  Up const& u = ask_up(m, ld, hd);
+  Buf<u32> out(nl);
+  u32* o = out.get();
+  u32 const* uo = u.offsets.get();
+  u32 const* ua = u.adj.get();
+  parallel_for(nl, OHX_LAMBDA(u32 i) {
+    u32 r = 0;
+    for (u32 j = uo[i]; j < uo[i + 1]; ++j)
+      if (mh[ua[j]]) {
+        r = 1;
+        break;
+      }
+    o[i] = r;
+  });
+  return out;
+}
+
+Buf<u32> mark_up(Mesh& m, u32 ld, u32 hd, u32 const* ml) {
+  u32 nh = count(m, hd);
+  u32 lph = h_down_deg[hd][ld];
+  u32 const* lh = ask_down(m, hd, ld);
+  Buf<u32> out(nh);
+  u32* o = out.get();
+  parallel_for(nh, OHX_LAMBDA(u32 i) {
+    u32 r = 0;
+    for (u32 j = 0; j < lph; ++j)
+      if (ml[lh[(size_t) i * lph + j]]) {
+        r = 1;
+        break;
+      }
+    o[i] = r;
+  });
+  return out;
+}
+
+// mesh_mark_dual_layers (mark.cpp:90-126)
+void mark_dual_layers(Mesh& m, Buf<u32>& marked, u32 nlayers) {
+  u32 dim = m.dim;
+  u32 ne = count(m, dim);
+  u32 const* dual = ask_dual(m);
+  u32 deg = h_down_deg[dim][dim - 1];
+  for (u32 l = 0; l < nlayers; ++l) {
+    Buf<u32> out(ne);
+    u32* o = out.get();
+    u32 const* in = marked.get();
+    parallel_for(ne, OHX_LAMBDA(u32 i) {
+      u32 r = in[i];
+      for (u32 j = 0; j < deg; ++j) {
+        u32 e = dual[(size_t) i * deg + j];
+        if (e != INVALID && in[e]) r = 1;
+      }
+      o[i] = r;
+    });
+    marked = std::move(out);
+  }
+}
+
+Buf<u32> mark_class(Mesh& m, u32 ent_dim, u32 target_dim, u32 target_id) {
+  u32 n = count(m, ent_dim);
+  u32 const* cd = u32_tag(m, ent_dim, "class_dim");
+  u32 const* ci = (target_id != INVALID) ? u32_tag(m, ent_dim, "class_id") : nullptr;
+  Buf<u32> out(n);
+  u32* o = out.get();
+  parallel_for(n, OHX_LAMBDA(u32 i) {
+    u32 r = 0;
+    if (cd[i] == target_dim)
+      if (target_id == INVALID || ci[i] == target_id) r = 1;
+    o[i] = r;
+  });
+  return out;
+}
+
+void unmark_boundary(Mesh& m, u32 ent_dim, u32* marked) {
+  u32 const* cd = u32_tag(m, ent_dim, "class_dim");
+  u32 dim = m.dim;
+  parallel_for(count(m, ent_dim), OHX_LAMBDA(u32 i) {
+    if (cd[i] != dim) marked[i] = 0;
+  });
+}
+
+// mesh_mark_slivers (mark.cpp:202-232): quality fused with the threshold
+template <u32 DIM>
+__global__ __launch_bounds__(BLOCK) void mark_sliver_f64(u32 n, u32 const* __restrict__ ev,
+                                                         double const* __restrict__ x, double good,
+                                                         u32* __restrict__ out) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  V3 p[DIM + 1];
+#pragma unroll
+  for (u32 j = 0; j <= DIM; ++j) p[j] = ld3(x + 3 * (size_t) ev[(size_t) i * (DIM + 1) + j]);
+  out[i] = (element_quality(DIM, p) < good) ? 1u : 0u;
+}
+
+Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers) {
+  u32 ne = count(m, m.dim);
+  Buf<u32> s(ne);
+  if (ne) {
+    ProfScope ps("mark_sliver_f64");
+    if (m.dim == 3)
+      hipLaunchKernelGGL(mark_sliver_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+                         ask_down(m, 3, 0), coords(m), good_qual, s.get());
+    else
+      hipLaunchKernelGGL(mark_sliver_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+                         ask_down(m, 2, 0), coords(m), good_qual, s.get());
+    check_launch("mark_sliver_f64");
+  }
+  mark_dual_layers(m, s, nlayers);
+  return s;
+}
+
+Buf<u32> mark_part_boundary(Mesh& m) {
+  u32 dim = m.dim;
+  u32 ns = count(m, dim - 1);
+  Up const& u = ask_up(m, dim - 1, dim);
+  u32 const* uo = u.offsets.get();
+  Buf<u32> out(ns);
+  u32* o = out.get();
+  parallel_for(ns, OHX_LAMBDA(u32 i) { o[i] = (uo[i + 1] - uo[i]) < 2 ? 1u : 0u; });
+  return out;
+}
+
+// ------------------------------------------------------ classification
+// mesh_derive_model (derive_model.cpp:24-448).  Setup, run once per mesh.
+// Normals and hinge dot products are computed on the device; the crease
+// test acos(d) > angle and the connected-component BFS run on the host with
+// the C library, as the reference runs its BFS on the host (bfs.cpp).
+static void derive_class_dim(Mesh& m, double crease_angle) {
+  u32 dim = m.dim;
+  u32 ne = count(m, dim);
+  add_tag_u32(m, dim, "class_dim", 1, filled(ne, dim));
+  Buf<u32> bs = mark_part_boundary(m);
+  u32 ns = count(m, dim - 1);
+  Buf<u32> scd(ns);
+  {
+    u32* s = scd.get();
+    u32 const* b = bs.get();
+    parallel_for(ns, OHX_LAMBDA(u32 i) { s[i] = dim - b[i]; });
+  }
+  add_tag_u32(m, dim - 1, "class_dim", 1, std::move(scd));
+  double const* x = coords(m);
+  u32 const* sv = ask_down(m, dim - 1, 0);
+  Buf<double> nrm((size_t) 3 * ns);
+  {
+    double* n = nrm.get();
+    u32 const* b = bs.get();
+    parallel_for(ns, OHX_LAMBDA(u32 i) {
+      if (!b[i]) return;
+      V3 v;
+      if (dim == 2) {
+        v = sub(ld3(x + 3 * (size_t) sv[2 * i + 1]), ld3(x + 3 * (size_t) sv[2 * i]));
+      } else {
+        V3 a = ld3(x + 3 * (size_t) sv[3 * i]);
+        v = cross(sub(ld3(x + 3 * (size_t) sv[3 * i + 1]), a), sub(ld3(x + 3 * (size_t) sv[3 * i + 2]), a));
+      }
+      double s = 1.0 / sqrt(dot(v, v));  // normalize_vector (algebra.hpp:133-139)
+      n[3 * (size_t) i + 0] = v.x * s;
+      n[3 * (size_t) i + 1] = v.y * s;
+      n[3 * (size_t) i + 2] = v.z * s;
+    });
+  }
+  Buf<u32> bh = mark_down(m, dim - 1, dim - 2, bs.get());
+  u32 nh = count(m, dim - 2);
+  Up const& hs = ask_up(m, dim - 2, dim - 1);
+  Buf<double> dots(nh);
+  {
+    double* d = dots.get();
+    double const* n = nrm.get();
+    u32 const* b = bs.get();
+    u32 const* h = bh.get();
+    u32 const* ho = hs.offsets.get();
+    u32 const* ha = hs.adj.get();
+    u32* err = scratch_words();
+    fill(err, 1, 0);
+    parallel_for(nh, OHX_LAMBDA(u32 i) {
+      if (!h[i]) {
+        d[i] = 2.0;  // not a boundary hinge: no angle
+        return;
+      }
+      u32 nadj = 0, adj[2] = {0, 0};
+      for (u32 j = ho[i]; j < ho[i + 1]; ++j)
+        if (b[ha[j]]) {
+          if (nadj < 2) adj[nadj] = ha[j];
+          else *err = 1u;
+          ++nadj;
+        }
+      d[i] = dot(ld3(n + 3 * (size_t) adj[0]), ld3(n + 3 * (size_t) adj[1]));
+    });
+    OHX_REQUIRE(read_at(err, 0) == 0, "derive_model: more than two boundary sides at a hinge");
+  }
+  std::vector<double> hd = host_copy(dots.get(), nh);
+  std::vector<u32> crease(nh);
+  for (u32 i = 0; i < nh; ++i) crease[i] = (hd[i] != 2.0) ? (std::acos(hd[i]) > crease_angle) : 0u;
+  Buf<u32> cr = to_device(crease.data(), nh);
+  Buf<u32> hcd(nh);
+  {
+    u32* o = hcd.get();
+    u32 const* h = bh.get();
+    u32 const* c = cr.get();
+    parallel_for(nh, OHX_LAMBDA(u32 i) { o[i] = dim - h[i] - c[i]; });
+  }
+  if (dim == 2) {
+    add_tag_u32(m, 0, "class_dim", 1, std::move(hcd));
+    return;
+  }
+  u32 nv = count(m, 0);
+  Up const& ve = ask_up(m, 0, 1);
+  Buf<u32> vcd(nv);
+  {
+    u32* o = vcd.get();
+    u32 const* vo = ve.offsets.get();
+    u32 const* va = ve.adj.get();
+    u32 const* c = cr.get();
+    u32 const* hc = hcd.get();
+    parallel_for(nv, OHX_LAMBDA(u32 i) {
+      u32 nin = 0, d = 3;
+      for (u32 j = vo[i]; j < vo[i + 1]; ++j) {
+        u32 e = va[j];
+        if (c[e]) ++nin;
+        if (hc[e] < d) d = hc[e];
+      }
+      o[i] = d - (nin > 2 ? 1u : 0u);
+    });
+  }
+  add_tag_u32(m, 1, "class_dim", 1, std::move(hcd));
+  add_tag_u32(m, 0, "class_dim", 1, std::move(vcd));
+}
+
+// connected components numbered by their lowest node (bfs.cpp:18-103)
+static void host_components(u32 n, std::vector<u32> const& offs, std::vector<u32> const& adj,
+                            std::vector<u32>& comp) {
+  std::vector<u32> layer(n, INVALID), queue(n);
+  comp.assign(n, 0);
+  u32 begin = 0, end = 0, c = 0;
+  for (u32 i = 0; i < n; ++i) {
+    if (layer[i] != INVALID) continue;
+    queue[end++] = i;
+    layer[i] = 0;
+    comp[i] = c;
+    while (begin != end) {
+      u32 u = queue[begin++];
+      for (u32 j = offs[u]; j < offs[u + 1]; ++j) {
+        u32 v = adj[j];
+        if (layer[v] == INVALID) {
+          queue[end++] = v;
+          layer[v] = layer[u] + 1;
+          comp[v] = comp[u];
+        }
+      }
+    }
+    ++c;
+  }
+}
+
+// set_equal_order_class_id (derive_model.cpp:270-396)
+static void equal_order_class_id(Mesh& m, u32 d) {
+  u32 n = count(m, d);
+  Buf<u32> eq = mark_class(m, d, d, INVALID);
+  Buf<u32> eqo = exscan(eq.get(), n);
+  u32 neq = read_at(eqo.get(), n);
+  Buf<u32> br = mark_class(m, d - 1, d, INVALID);
+  u32 bpe = h_down_deg[d][d - 1];
+  u32 const* be = ask_down(m, d, d - 1);
+  Up const& u = ask_up(m, d - 1, d);
+  Buf<u32> deg = filled(neq, 0);
+  u32 const* e = eqo.get();
+  u32 const* b = br.get();
+  {
+    u32* dg = deg.get();
+    parallel_for(n, OHX_LAMBDA(u32 i) {
+      if (e[i] == e[i + 1]) return;
+      u32 k = 0;
+      for (u32 j = 0; j < bpe; ++j)
+        if (b[be[(size_t) i * bpe + j]]) ++k;
+      dg[e[i]] = k;
+    });
+  }
+  Buf<u32> offs = exscan(deg.get(), neq);
+  u32 nadj = read_at(offs.get(), neq);
+  Buf<u32> adj(nadj);
+  {
+    u32* a = adj.get();
+    u32 const* o = offs.get();
+    u32 const* uo = u.offsets.get();
+    u32 const* ua = u.adj.get();
+    parallel_for(n, OHX_LAMBDA(u32 i) {
+      if (e[i] == e[i + 1]) return;
+      u32 k = o[e[i]];
+      for (u32 j = 0; j < bpe; ++j) {
+        u32 s = be[(size_t) i * bpe + j];
+        if (!b[s]) continue;
+        for (u32 l = uo[s]; l < uo[s + 1]; ++l) {
+          u32 other = ua[l];
+          if (other == i) continue;
+          if (e[other] != e[other + 1]) {
+            a[k++] = e[other];
+            break;
+          }
+        }
+      }
+    });
+  }
+  std::vector<u32> ho = host_copy(offs.get(), neq + 1), ha = host_copy(adj.get(), nadj), comp;
+  host_components(neq, ho, ha, comp);
+  Buf<u32> dc = to_device(comp.data(), neq);
+  Buf<u32> cid = filled(n, 0);
+  {
+    u32* c = cid.get();
+    u32 const* cc = dc.get();
+    parallel_for(n, OHX_LAMBDA(u32 i) {
+      if (e[i] != e[i + 1]) c[i] = cc[e[i]];
+    });
+  }
+  add_tag_u32(m, d, "class_id", 1, std::move(cid));
+}
+
+// project_class_id_onto (derive_model.cpp:398-416): last matching high wins
+static void project_class_id(Mesh& m, u32 ld) {
+  u32 nl = count(m, ld);
+  Up const& u = ask_up(m, ld, ld + 1);
+  u32 const* hcd = u32_tag(m, ld + 1, "class_dim");
+  u32 const* hci = u32_tag(m, ld + 1, "class_id");
+  u32 const* lcd = u32_tag(m, ld, "class_dim");
+  u32* lci = find_tag(m, ld, "class_id")->u.get();
+  u32 const* uo = u.offsets.get();
+  u32 const* ua = u.adj.get();
+  parallel_for(nl, OHX_LAMBDA(u32 i) {
+    for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
+      u32 h = ua[j];
+      if (hcd[h] == lcd[i]) lci[i] = hci[h];
+    }
+  });
+}
+
+void derive_model(Mesh& m, double crease_angle) {
+  derive_class_dim(m, crease_angle);
+  u32 nv = count(m, 0);
+  u32 const* vcd = u32_tag(m, 0, "class_dim");
+  Buf<u32> is0(nv);
+  {
+    u32* z = is0.get();
+    parallel_for(nv, OHX_LAMBDA(u32 i) { z[i] = vcd[i] == 0 ? 1u : 0u; });
+  }
+  Buf<u32> sc = exscan(is0.get(), nv);
+  Buf<u32> vci(nv);
+  {
+    u32* c = vci.get();
+    u32 const* z = is0.get();
+    u32 const* s = sc.get();
+    parallel_for(nv, OHX_LAMBDA(u32 i) { c[i] = z[i] ? s[i] : 0u; });
+  }
+  add_tag_u32(m, 0, "class_id", 1, std::move(vci));
+  for (u32 d = 1; d <= m.dim; ++d) equal_order_class_id(m, d);
+  for (u32 dd = 1; dd <= m.dim; ++dd) project_class_id(m, m.dim - dd);
+}
+
+}  // namespace ohx

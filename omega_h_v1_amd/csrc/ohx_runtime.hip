@@ -1,0 +1,209 @@
+// Runtime pieces: stream, pinned staging, caching HBM arena, event timers.
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "ohx_common.hpp"
+
+namespace ohx {
+
+static std::string g_last_error;
+hipStream_t g_stream_override = nullptr;
+
+void fail(char const* what, char const* file, int line) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "ohx: %s (%s:%d)", what, file, line);
+  g_last_error = buf;
+  throw Error(buf);
+}
+
+std::string const& last_error() { return g_last_error; }
+void set_last_error(std::string const& s) { g_last_error = s; }
+
+Runtime& rt() {
+  static Runtime r;
+  if (!r.stream) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+      throw Error("ohx: no HIP device visible (this library has no CPU path)");
+    OHX_CHECK(hipGetDevice(&r.device));
+    OHX_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+    OHX_CHECK(hipHostMalloc(&r.pinned, 4096, hipHostMallocDefault));
+  }
+  return r;
+}
+
+void sync() { OHX_CHECK(hipStreamSynchronize(stream())); }
+
+void check_launch(char const* name) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail(name, __FILE__, __LINE__);
+}
+
+// ------------------------------------------------------------------ arena
+namespace {
+struct Arena {
+  std::map<size_t, std::vector<void*>> free_lists;
+  std::unordered_map<void*, size_t> live;  // ptr -> class size
+  size_t in_use = 0, high = 0, cached = 0;
+};
+Arena& arena() {
+  static Arena a;
+  return a;
+}
+size_t size_class(size_t bytes) {
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+  rt();
+  Arena& a = arena();
+  size_t c = size_class(bytes ? bytes : 1);
+  void* p = nullptr;
+  auto it = a.free_lists.find(c);
+  if (it != a.free_lists.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+    a.cached -= c;
+  } else {
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      // give cached blocks back and retry once
+      (void) hipGetLastError();
+      dev_trim();
+      OHX_CHECK(hipMalloc(&p, c));
+    }
+  }
+  a.live[p] = c;
+  a.in_use += c;
+  if (a.in_use > a.high) a.high = a.in_use;
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  Arena& a = arena();
+  auto it = a.live.find(p);
+  if (it == a.live.end()) fail("dev_free of an unknown pointer", __FILE__, __LINE__);
+  size_t c = it->second;
+  a.live.erase(it);
+  a.in_use -= c;
+  a.free_lists[c].push_back(p);
+  a.cached += c;
+}
+
+void dev_trim() {
+  Arena& a = arena();
+  OHX_CHECK(hipStreamSynchronize(rt().stream));
+  for (auto& kv : a.free_lists) {
+    for (void* p : kv.second) OHX_CHECK(hipFree(p));
+    kv.second.clear();
+  }
+  a.cached = 0;
+}
+
+size_t dev_bytes_in_use() { return arena().in_use; }
+size_t dev_bytes_high_water() { return arena().high; }
+void dev_reset_high_water() { arena().high = arena().in_use; }
+
+// ---------------------------------------------------------------- timing
+namespace {
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::unordered_map<std::string, hipEvent_t> open;
+  std::map<std::string, std::pair<double, u64>> totals;  // ms, launches
+};
+Prof& prof() {
+  static Prof p;
+  return p;
+}
+hipEvent_t take_event() {
+  Prof& p = prof();
+  if (!p.pool.empty()) {
+    hipEvent_t e = p.pool.back();
+    p.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  OHX_CHECK(hipEventCreate(&e));
+  return e;
+}
+}  // namespace
+
+void prof_begin(char const* name) {
+  Prof& p = prof();
+  if (!p.on) return;
+  hipEvent_t e = take_event();
+  OHX_CHECK(hipEventRecord(e, stream()));
+  p.open[name] = e;
+}
+
+void prof_end(char const* name) {
+  Prof& p = prof();
+  if (!p.on) return;
+  auto it = p.open.find(name);
+  if (it == p.open.end()) return;
+  hipEvent_t e = take_event();
+  OHX_CHECK(hipEventRecord(e, stream()));
+  p.pending.push_back({name, it->second, e});
+  p.open.erase(it);
+}
+
+void prof_enable(bool on) {
+  Prof& p = prof();
+  p.on = on;
+}
+
+void prof_collect() {
+  Prof& p = prof();
+  if (p.pending.empty()) return;
+  sync();
+  for (auto& q : p.pending) {
+    float ms = 0;
+    OHX_CHECK(hipEventElapsedTime(&ms, q.a, q.b));
+    auto& t = p.totals[q.name];
+    t.first += ms;
+    t.second += 1;
+    p.pool.push_back(q.a);
+    p.pool.push_back(q.b);
+  }
+  p.pending.clear();
+}
+
+void prof_reset() {
+  prof_collect();
+  prof().totals.clear();
+}
+
+bool prof_get(char const* name, double* ms, u64* launches) {
+  prof_collect();
+  auto& t = prof().totals;
+  auto it = t.find(name);
+  if (it == t.end()) {
+    *ms = 0;
+    *launches = 0;
+    return false;
+  }
+  *ms = it->second.first;
+  *launches = it->second.second;
+  return true;
+}
+
+std::vector<std::string> prof_names() {
+  prof_collect();
+  std::vector<std::string> v;
+  for (auto& kv : prof().totals) v.push_back(kv.first);
+  return v;
+}
+
+}  // namespace ohx

@@ -22,6 +22,7 @@
 #include "arrays.hpp"
 #include "coarsen.hpp"
 #include "derive_model.hpp"
+#include "edge_swap.hpp"
 #include "doubles.hpp"
 #include "graph.hpp"
 #include "indset.hpp"
@@ -228,6 +229,54 @@ unsigned ref_adapt(void* m, double size_floor, double good_qual,
 unsigned ref_warp_to_limit(void* m, double qual_floor)
 {
   return mesh_warp_to_limit(M(m), qual_floor);
+}
+
+/* the reference's own swap tables and swap choice (edge_swap.cpp,
+ * edge_swap_edges.cpp) and quality formulas (quality.hpp), for
+ * known-answer fixtures */
+unsigned ref_swap_nint_edges(unsigned ring_size)
+{
+  return swap_nint_edges[ring_size];
+}
+
+void ref_swap_int_edges(unsigned ring_size, unsigned code, unsigned* out)
+{
+  unsigned k = swap_nint_edges[ring_size];
+  for (unsigned i = 0; i < 2 * k; ++i)
+    out[i] = swap_int_edges[ring_size][code][i];
+}
+
+unsigned ref_choose_edge_swap(unsigned ring_size, double const* edge_x6,
+    double const* ring_x, double* quality)
+{
+  double ex[2][3];
+  double rx[MAX_EDGE_SWAP][3];
+  memcpy(ex, edge_x6, sizeof(ex));
+  memcpy(rx, ring_x, sizeof(double) * 3 * ring_size);
+  struct swap_choice sc = choose_edge_swap(ring_size, ex, rx);
+  *quality = sc.quality;
+  return sc.code;
+}
+
+double ref_tet_quality(double const* x12)
+{
+  double x[4][3];
+  memcpy(x, x12, sizeof(x));
+  return tet_quality(x);
+}
+
+double ref_triangle_quality(double const* x9)
+{
+  double x[3][3];
+  memcpy(x, x9, sizeof(x));
+  return triangle_quality(x);
+}
+
+double ref_triangle_xy_quality(double const* x9)
+{
+  double x[3][3];
+  memcpy(x, x9, sizeof(x));
+  return triangle_xy_quality(x);
 }
 
 /* Same control flow as mesh_adapt (adapt.cpp:82-94) with satisfy_size

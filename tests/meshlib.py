@@ -59,6 +59,8 @@ _SIG = {
     "warp_to_limit": (_u, [_p, _d]),
 }
 _DOWN_DEG = [[1, 0, 0, 0], [2, 1, 0, 0], [3, 3, 1, 0], [4, 6, 4, 1]]
+# per-prefix symbol names that differ from the common name
+_ALIAS = {"ohx_mesh_": {"new_mesh": "new"}}
 
 
 class Lib:
@@ -70,20 +72,47 @@ class Lib:
         self.dll = C.CDLL(path)
         self.f = {}
         for name, (res, args) in _SIG.items():
-            fn = getattr(self.dll, prefix + name, None)
+            sym = _ALIAS.get(prefix, {}).get(name, name)
+            fn = getattr(self.dll, prefix + sym, None)
             if fn is None:
                 continue
             fn.restype = res
             fn.argtypes = args
             self.f[name] = fn
+        if prefix == "ohx_mesh_":
+            self._wrap_errors()
         adapt = getattr(self.dll, prefix + "adapt", None)
         if adapt is not None:
             adapt.restype = C.c_int if prefix != "ref_" else _u
             adapt.argtypes = [_p, _d, _d, _u, _u]
             self.f["adapt"] = adapt
+            if prefix == "ohx_mesh_":
+                self._wrap_errors_one("adapt")
 
     def has(self, name):
         return name in self.f
+
+    def _wrap_errors(self):
+        """The product records errors instead of aborting; surface them."""
+        last = self.dll.ohx_last_error
+        last.restype = C.c_char_p
+        clear = self.dll.ohx_clear_error
+
+        def wrap(fn, name):
+            def call(*a):
+                clear()
+                r = fn(*a)
+                msg = last()
+                if msg:
+                    raise RuntimeError("%s: %s" % (name, msg.decode()))
+                return r
+            return call
+        self._wrap = wrap
+        for name in list(self.f):
+            self.f[name] = wrap(self.f[name], name)
+
+    def _wrap_errors_one(self, name):
+        self.f[name] = self._wrap(self.f[name], name)
 
 
 def _ptr(a):
@@ -102,6 +131,30 @@ class Mesh:
                                    if conn.ndim == 1 else len(conn), _ptr(conn))
         if not self.h:
             raise RuntimeError("mesh creation failed in " + lib.path)
+
+    @classmethod
+    def box_on_device(cls, lib, dim, n):
+        """The product's own device-side Kuhn generator (ohx_mesh_new_box)."""
+        fn = lib.dll.ohx_mesh_new_box
+        fn.restype = C.c_void_p
+        fn.argtypes = [_u, _u]
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.dim = dim
+        self.h = fn(dim, n)
+        if not self.h:
+            raise RuntimeError("ohx_mesh_new_box failed: %s" % lib.dll.ohx_last_error())
+        return self
+
+    def clone(self):
+        fn = self.lib.dll.ohx_mesh_clone
+        fn.restype = C.c_void_p
+        fn.argtypes = [_p]
+        other = Mesh.__new__(Mesh)
+        other.lib = self.lib
+        other.dim = self.dim
+        other.h = fn(self.h)
+        return other
 
     def __del__(self):
         try:

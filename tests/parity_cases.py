@@ -1,0 +1,100 @@
+"""Scripted adapt workloads run step by step in any of the libraries.
+
+A case yields (label, mesh) after every step so two libraries can be
+compared state by state.  Fields are evaluated on the host in float64
+(numpy) from the mesh's current coordinates, as the reference evaluates
+analytic fields on the host (eval_field.cpp:15-49).
+"""
+import numpy as np
+
+import meshlib as ML
+from omega_h_v1_amd import workloads as W
+
+
+def make_mesh(lib, dim, n, on_device=False):
+    if on_device:
+        m = ML.Mesh.box_on_device(lib, dim, n)
+    else:
+        x, c = ML.box(dim, n)
+        m = ML.Mesh(lib, dim, x, c)
+    m.derive_model_full()
+    return m
+
+
+def c1_steps(lib, on_device=False):
+    """3D Kuhn box 8^3 (3072 tets), h = 0.05: one refine, one coarsen, one
+    swap pass with the osh_adapt defaults (omega_h.cpp:955-1003)."""
+    m = make_mesh(lib, 3, 8, on_device)
+    yield "derive", m
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), 0.05))
+    qf = min(m.min_quality(), 0.3)
+    m.last = m.refine_by_size(qf)
+    yield "refine", m
+    m.last = m.coarsen_by_size(qf, 1.0 / 3.0)
+    yield "coarsen", m
+    m.last = m.swap_slivers(0.3, 4)
+    yield "swap", m
+
+
+def warp_steps(lib, dim, n, h, warp, cycles=4, warp_floor=0.2, good=0.3, nlayers=4,
+               on_device=False):
+    """test_warp*.cpp-style loop: per cycle a fresh warp field, then up to
+    six warp_to_limit + mesh_adapt rounds; the swirl alternates direction."""
+    m = make_mesh(lib, dim, n, on_device)
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), h))
+    yield "setup", m
+    rot = W.PI_REF / 4
+    for cyc in range(cycles):
+        x = m.coords().reshape(-1, 3)
+        w = W.sine_warp_2d(x) if warp == "sine" else W.rotation_warp(x, rot)
+        m.add_field(0, "warp", w.ravel())
+        for it in range(6):
+            done = m.warp_to_limit(warp_floor)
+            yield "warp%d.%d" % (cyc, it), m
+            m.last = m.adapt(1.0 / 3.0, good, nlayers, 50)
+            yield "adapt%d.%d" % (cyc, it), m
+            if done:
+                break
+        m.free_field(0, "warp")
+        rot = -rot
+
+
+def c2_steps(lib, n=256, h=1.0 / 200.0, warp_floor=0.2, on_device=False):
+    """2D Kuhn square n^2 x 2 triangles, warp u = 0.1 sin(2 pi x) cos(2 pi y)
+    in both components, warp_to_limit, then mesh_adapt with h."""
+    m = make_mesh(lib, 2, n, on_device)
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), h))
+    x = m.coords().reshape(-1, 3)
+    m.add_field(0, "warp", W.sine_warp_2d(x).ravel())
+    m.last = m.warp_to_limit(warp_floor)
+    yield "warp", m
+    m.last = m.adapt(1.0 / 3.0, 0.3, 4, 50)
+    yield "adapt", m
+
+
+CASES = {
+    "c1": lambda lib, dev=False: c1_steps(lib, dev),
+    "warp2d_sine_n16": lambda lib, dev=False: warp_steps(lib, 2, 16, 1 / 12.0, "sine", on_device=dev),
+    "warp2d_rot_n32": lambda lib, dev=False: warp_steps(lib, 2, 32, 1 / 40.0, "rot", on_device=dev),
+    "warp3d_rot_n4": lambda lib, dev=False: warp_steps(lib, 3, 4, 0.15, "rot", on_device=dev),
+}
+
+
+def diff_states(a, b):
+    """None if the two mesh states are bit-identical, else a description."""
+    sa, sb = a.state(), b.state()
+    if sorted(sa) != sorted(sb):
+        return "different tag/array sets: %s vs %s" % (sorted(sa), sorted(sb))
+    for k in sorted(sa):
+        x, y = sa[k], sb[k]
+        if isinstance(x, int):
+            if x != y:
+                return "%s: %d vs %d" % (k, x, y)
+            continue
+        if x.shape != y.shape:
+            return "%s: shape %s vs %s" % (k, x.shape, y.shape)
+        if not np.array_equal(x.view(np.uint8), y.view(np.uint8)):
+            idx = np.nonzero(x.view(np.uint8).reshape(len(x), -1).any(1) if False else (x != y))[0]
+            return "%s: %d entries differ, first at %s: %s vs %s" % (
+                k, len(idx), idx[:5], x[idx[:5]], y[idx[:5]])
+    return None
