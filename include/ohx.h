@@ -143,8 +143,12 @@ void ohx_set_verbose(int on) OSH_PUBLIC;
 /* HIP-event timing of named kernels on the stream they run on */
 void ohx_profile_enable(int on) OSH_PUBLIC;
 void ohx_profile_reset(void) OSH_PUBLIC;
-/* total milliseconds and launches of a timed kernel; returns 0 if unseen */
-int ohx_profile_get(char const* name, double* ms, unsigned long long* launches) OSH_PUBLIC;
+/* total milliseconds, launches and work items (elements, edges...) of a
+ * timed kernel; returns 0 if unseen */
+int ohx_profile_get(char const* name, double* ms, unsigned long long* launches,
+    double* units) OSH_PUBLIC;
+/* newline-separated names of the timed kernels; returns the full length */
+unsigned ohx_profile_names(char* buf, unsigned cap) OSH_PUBLIC;
 unsigned long long ohx_device_bytes_high_water(void) OSH_PUBLIC;
 void ohx_device_reset_high_water(void) OSH_PUBLIC;
 void ohx_synchronize(void) OSH_PUBLIC;

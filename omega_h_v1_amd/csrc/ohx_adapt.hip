@@ -72,7 +72,8 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
     parallel_for(n, OHX_LAMBDA(u32 i) { s[i] = filter[i] ? UNKNOWN : NOT_IN_SET; });
   }
   if (!n) return a;
-  ProfScope ps("indset");
+  // per round: state in and out, goodness and the star CSR
+  ProfScope ps("indset", (double) n * 16 + (double) g.adj.size() * 4);
   // rounds are launched in batches; a round after convergence copies the
   // state unchanged, so running past the converged round is harmless
   constexpr u32 BATCH = 4, MAX_ROUNDS = 100;
@@ -206,7 +207,7 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   if (require_better) eq = mesh_qualities(m);
   Buf<double> out(ns);
   zero_words(0, 2);
-  ProfScope ps("refine_quality_f64");
+  ProfScope ps("refine_quality_f64", (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24);
   if (dim == 3)
     hipLaunchKernelGGL(refine_quality_f64<3>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
                        u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), eq.get(), qual_floor,
@@ -385,7 +386,7 @@ u32 refine_by_size(Mesh& m, double qual_floor) {
   Buf<u32> cand(ne);
   zero_words(0, 1);
   if (ne) {
-    ProfScope ps("refine_candidate_f64");
+    ProfScope ps("refine_candidate_f64", (double) ne * 12 + (double) count(m, 0) * 32);
     hipLaunchKernelGGL(refine_candidate_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
                        ask_down(m, 1, 0), coords(m), st->f.get(), cand.get(), scratch_words());
     check_launch("refine_candidate_f64");
@@ -537,7 +538,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   Buf<double> cq((size_t) 2 * ne);
   zero_words(0, 1);
   {
-    ProfScope ps("coarsen_quality_f64");
+    ProfScope ps("coarsen_quality_f64", (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28);
     if (dim == 3)
       hipLaunchKernelGGL(coarsen_quality_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, ev,
                          ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), eq.get(), qual_floor,
@@ -743,7 +744,7 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
   Buf<u32> codes(ne);
   zero_words(0, 1);
   if (ne) {
-    ProfScope ps("coarsen_size_code_f64");
+    ProfScope ps("coarsen_size_code_f64", (double) ne * 12 + (double) count(m, 0) * 32);
     hipLaunchKernelGGL(coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
                        ask_down(m, 1, 0), coords(m), st->f.get(), size_floor, codes.get(), scratch_words());
     check_launch("coarsen_size_code_f64");
@@ -1072,7 +1073,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   Buf<u32> rsz = filled(ne, 0);
   zero_words(0, 2);
   {
-    ProfScope ps("swap_quality_f64");
+    ProfScope ps("swap_quality_f64", (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24);
     hipLaunchKernelGGL(swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, u.offsets.get(),
                        u.adj.get(), u.dirs.get(), ev, vt, coords(m), eq.get(), cand.get(), q.get(), rsz.get(),
                        scratch_words());
@@ -1115,7 +1116,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
     u32 vpe = d + 1;
     Buf<u32> gen((size_t) ngen * vpe);
     {
-      ProfScope ps("swap_cavity_f64");
+      ProfScope ps("swap_cavity_f64", (double) ne * 12 + (double) ngen * vpe * 4);
       hipLaunchKernelGGL(swap_cavity_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, d, is, goff.get(),
                          u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m), gen.get());
       check_launch("swap_cavity_f64");
@@ -1252,7 +1253,7 @@ u32 warp_to_limit(Mesh& m, double qual_floor) {
   };
   axpy(f);
   while (true) {
-    double mq = min_element_quality(dim, ne, ev, xp);
+    double mq = min_element_quality(dim, ne, ev, xp, nv);
     if (!(mq < qual_floor)) break;
     if (g_verbose) std::printf("quality %f exceeds limit\n", mq);
     hit = true;

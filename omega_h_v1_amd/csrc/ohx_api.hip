@@ -18,7 +18,8 @@ std::string const& last_error();
 void set_last_error(std::string const& s);
 void prof_enable(bool on);
 void prof_reset();
-bool prof_get(char const* name, double* ms, u64* launches);
+bool prof_get(char const* name, double* ms, u64* launches, double* units);
+std::vector<std::string> prof_names();
 }  // namespace ohx
 
 using namespace ohx;
@@ -455,8 +456,19 @@ void ohx_profile_enable(int on) { prof_enable(on != 0); }
 void ohx_profile_reset(void) {
   guard_void([&]() { prof_reset(); });
 }
-int ohx_profile_get(char const* name, double* ms, unsigned long long* launches) {
-  return guard(0, [&]() { return prof_get(name, ms, launches) ? 1 : 0; });
+int ohx_profile_get(char const* name, double* ms, unsigned long long* launches, double* units) {
+  return guard(0, [&]() { return prof_get(name, ms, launches, units) ? 1 : 0; });
+}
+unsigned ohx_profile_names(char* buf, unsigned cap) {
+  return guard(0u, [&]() -> unsigned {
+    std::string all;
+    for (auto& n : prof_names()) all += n + "\n";
+    if (buf && cap) {
+      std::strncpy(buf, all.c_str(), cap - 1);
+      buf[cap - 1] = 0;
+    }
+    return (unsigned) all.size();
+  });
 }
 unsigned long long ohx_device_bytes_high_water(void) { return dev_bytes_high_water(); }
 void ohx_device_reset_high_water(void) { dev_reset_high_water(); }

@@ -168,12 +168,14 @@ void parallel_for(u32 n, F f) {
 
 // ---------------------------------------------------------------- timing
 // Per-kernel-name HIP event timers, enabled by ohx_profile_enable().
+// units: the work items of the launch (elements, edges...), summed per name
 void prof_begin(char const* name);
-void prof_end(char const* name);
+void prof_end(char const* name, double units);
 struct ProfScope {
   char const* name;
-  explicit ProfScope(char const* n) : name(n) { prof_begin(n); }
-  ~ProfScope() { prof_end(name); }
+  double units;
+  explicit ProfScope(char const* n, double u = 0) : name(n), units(u) { prof_begin(n); }
+  ~ProfScope() { prof_end(name, units); }
 };
 
 }  // namespace ohx

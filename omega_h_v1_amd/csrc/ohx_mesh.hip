@@ -136,10 +136,11 @@ __global__ __launch_bounds__(BLOCK) void elem_min_quality_f64(u32 n, u32 begin,
   }
 }
 
-Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x) {
+Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x, u32 nverts) {
   Buf<double> q(n);
   if (!n) return q;
-  ProfScope ps("elem_quality_f64");
+  // compulsory bytes: connectivity, each vertex's coordinates once, output
+  ProfScope ps("elem_quality_f64", (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) n * 8);
   if (dim == 3)
     hipLaunchKernelGGL(elem_quality_f64<3>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, q.get());
   else
@@ -149,17 +150,17 @@ Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x) {
 }
 
 Buf<double> mesh_qualities(Mesh& m) {
-  return element_qualities(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m));
+  return element_qualities(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m), count(m, 0));
 }
 
 // min over elements 1..n-1: the serial build's doubles_min starts at 1
 // (doubles.cpp:111-118), and mesh_min_quality goes through it
-double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x) {
+double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x, u32 nverts) {
   if (n <= 1) return 1.7976931348623157e308;
   u32 nb = std::min<u32>(nblocks(n - 1), 1024);
   Buf<double> part(nb);
   {
-    ProfScope ps("elem_min_quality_f64");
+    ProfScope ps("elem_min_quality_f64", (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) nb * 8);
     if (dim == 3)
       hipLaunchKernelGGL(elem_min_quality_f64<3>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, part.get());
     else
@@ -170,7 +171,7 @@ double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x) {
 }
 
 double mesh_min_quality(Mesh& m) {
-  return min_element_quality(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m));
+  return min_element_quality(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m), count(m, 0));
 }
 
 // measure_edge (size.cpp:57-88): |x1 - x0| / ((h0 + h1) / 2)
@@ -192,7 +193,7 @@ Buf<double> measure_edges(Mesh& m) {
   OHX_REQUIRE(st && st->type == TagType::F64, "mesh has no adapt_size field");
   Buf<double> out(ne);
   if (ne) {
-    ProfScope ps("measure_edge_f64");
+    ProfScope ps("measure_edge_f64", (double) ne * 16 + (double) count(m, 0) * 32);
     hipLaunchKernelGGL(measure_edge_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
                        ask_down(m, 1, 0), coords(m), st->f.get(), out.get());
     check_launch("measure_edge_f64");
@@ -245,7 +246,8 @@ __global__ __launch_bounds__(BLOCK) void upfd_sort_split(u32 nlows, u32 deg, u32
 }
 
 void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_highs, Up& u) {
-  ProfScope ps("up_from_down");
+  // compulsory bytes: the down map in, offsets and (high, direction) out
+  ProfScope ps("up_from_down", (double) nhighs * h_down_deg[hd][ld] * 12 + (double) (nlows + 1) * 4);
   u32 deg = h_down_deg[hd][ld];
   u32 nuses = nhighs * deg;
   Buf<u32> counts = filled(nlows, 0);
@@ -310,6 +312,8 @@ __global__ __launch_bounds__(BLOCK) void reflect_down_kernel(u32 nh, u32 const* 
   }
 }
 
+static u32 read_nuses(Up const& u, u32 n) { return u.adj.size(); }
+
 static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   u32 nh = count(m, hd);
   u32 const* vh = ask_down(m, hd, 0);
@@ -320,7 +324,11 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   if (!nh) return out;
   u32* err = scratch_words();
   fill(err, 1, 0);
-  ProfScope ps("reflect_down");
+  // compulsory bytes: high vertices in, low ids out, lows-of-vertex CSR and
+  // low vertices read once
+  ProfScope ps("reflect_down", (double) nh * ((hd + 1) * 4 + lph * 4) +
+                                   (double) (count(m, 0) + 1) * 4 + (double) read_nuses(lv, count(m, 0)) * 8 +
+                                   (double) count(m, ld) * (ld + 1) * 4);
   dim3 g(nblocks(nh)), b(BLOCK);
   if (hd == 3 && ld == 1)
     hipLaunchKernelGGL((reflect_down_kernel<3, 1>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
@@ -771,7 +779,7 @@ Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers) {
   u32 ne = count(m, m.dim);
   Buf<u32> s(ne);
   if (ne) {
-    ProfScope ps("mark_sliver_f64");
+    ProfScope ps("mark_sliver_f64", (double) ne * (m.dim + 1) * 4 + (double) count(m, 0) * 24 + (double) ne * 4);
     if (m.dim == 3)
       hipLaunchKernelGGL(mark_sliver_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
                          ask_down(m, 3, 0), coords(m), good_qual, s.get());

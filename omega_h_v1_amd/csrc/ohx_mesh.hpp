@@ -67,9 +67,10 @@ double const* coords(Mesh& m);
 u32 const* u32_tag(Mesh& m, u32 d, char const* name);
 
 // kernels on a mesh
-Buf<double> element_qualities(u32 dim, u32 n, u32 const* verts_of_elems, double const* x);
+// nverts only feeds the traffic accounting of the kernel timers
+Buf<double> element_qualities(u32 dim, u32 n, u32 const* verts_of_elems, double const* x, u32 nverts = 0);
 Buf<double> mesh_qualities(Mesh& m);
-double min_element_quality(u32 dim, u32 n, u32 const* verts_of_elems, double const* x);
+double min_element_quality(u32 dim, u32 n, u32 const* verts_of_elems, double const* x, u32 nverts = 0);
 double mesh_min_quality(Mesh& m);
 Buf<double> measure_edges(Mesh& m);
 

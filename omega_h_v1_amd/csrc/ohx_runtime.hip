@@ -118,10 +118,15 @@ struct Prof {
   struct Pending {
     std::string name;
     hipEvent_t a, b;
+    double units;
   };
   std::vector<Pending> pending;
   std::unordered_map<std::string, hipEvent_t> open;
-  std::map<std::string, std::pair<double, u64>> totals;  // ms, launches
+  struct Total {
+    double ms = 0, units = 0;
+    u64 launches = 0;
+  };
+  std::map<std::string, Total> totals;
 };
 Prof& prof() {
   static Prof p;
@@ -148,14 +153,14 @@ void prof_begin(char const* name) {
   p.open[name] = e;
 }
 
-void prof_end(char const* name) {
+void prof_end(char const* name, double units) {
   Prof& p = prof();
   if (!p.on) return;
   auto it = p.open.find(name);
   if (it == p.open.end()) return;
   hipEvent_t e = take_event();
   OHX_CHECK(hipEventRecord(e, stream()));
-  p.pending.push_back({name, it->second, e});
+  p.pending.push_back({name, it->second, e, units});
   p.open.erase(it);
 }
 
@@ -172,8 +177,9 @@ void prof_collect() {
     float ms = 0;
     OHX_CHECK(hipEventElapsedTime(&ms, q.a, q.b));
     auto& t = p.totals[q.name];
-    t.first += ms;
-    t.second += 1;
+    t.ms += ms;
+    t.units += q.units;
+    t.launches += 1;
     p.pool.push_back(q.a);
     p.pool.push_back(q.b);
   }
@@ -185,17 +191,19 @@ void prof_reset() {
   prof().totals.clear();
 }
 
-bool prof_get(char const* name, double* ms, u64* launches) {
+bool prof_get(char const* name, double* ms, u64* launches, double* units) {
   prof_collect();
   auto& t = prof().totals;
   auto it = t.find(name);
   if (it == t.end()) {
     *ms = 0;
     *launches = 0;
+    if (units) *units = 0;
     return false;
   }
-  *ms = it->second.first;
-  *launches = it->second.second;
+  *ms = it->second.ms;
+  *launches = it->second.launches;
+  if (units) *units = it->second.units;
   return true;
 }
 
