@@ -7,6 +7,7 @@
 #include "ohx_adapt.hpp"
 #include "ohx_geom.hpp"
 #include "ohx_prims.hpp"
+#include "ohx_scan.hpp"
 #include "ohx_tables.hpp"
 
 namespace ohx {
@@ -19,7 +20,7 @@ static double now_s() {
 }
 
 // scratch word slots (zeroed by a fill kernel before each use)
-enum { W_ANY = 0, W_ERR = 1, W_FLAGS = 8 };
+enum { W_ANY = 0, W_FLAGS = 8 };
 
 static u32 read_word(u32 k) { return read_at(scratch_words(), k); }
 static void zero_words(u32 first, u32 n) { fill(scratch_words() + first, n, 0); }
@@ -95,46 +96,9 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
 }
 
 // ------------------------------------------------------------- inherit
-// class tags of products come from their domains (inherit.cpp:14-232)
-struct Prods {
-  u32 ndoms[4] = {0, 0, 0, 0};
-  u32 const* offs[4] = {nullptr, nullptr, nullptr, nullptr};
-  Buf<u32> own[4];
-  u32 ngen[4] = {0, 0, 0, 0};  // host copies of offs[d][ndoms[d]]
-  u32 go[5] = {0, 0, 0, 0, 0};
-  void finish() {
-    go[0] = 0;
-    for (u32 i = 0; i < 4; ++i) {
-      ngen[i] = (ndoms[i] && offs[i]) ? read_at(offs[i], ndoms[i]) : 0;
-      go[i + 1] = go[i] + ngen[i];
-    }
-  }
-};
-
-static void inherit_tag(Mesh& m, Mesh& mo, u32 prod_dim, Prods const& p, char const* name) {
-  Tag* t = find_tag(m, prod_dim, name);
-  u32 w = t->ncomps;
-  Buf<u32> out((size_t) p.go[4] * w);
-  for (u32 d = 0; d < 4; ++d) {
-    u32 const* src = nullptr;
-    if (d == 0)
-      src = t->u.get();
-    else if (d <= m.dim) {
-      Tag* td = find_tag(m, d, name);
-      src = td ? td->u.get() : nullptr;
-    }
-    if (!(p.ndoms[d] && src && p.offs[d])) continue;
-    expand_into_u32(out.get() + (size_t) p.go[d] * w, src, p.offs[d], p.ndoms[d], w);
-  }
-  add_tag_u32(mo, prod_dim, name, w, std::move(out));
-}
-
-static void inherit_class(Mesh& m, Mesh& mo, u32 prod_dim, Prods const& p) {
-  if (!find_tag(m, 0, "class_dim")) return;
-  OHX_REQUIRE(m.full, "class inheritance needs a full mesh");
-  inherit_tag(m, mo, prod_dim, p, "class_dim");
-  if (find_tag(m, 0, "class_id")) inherit_tag(m, mo, prod_dim, p, "class_id");
-}
+// Class tags (class_dim, class_id) of products come from the entity they
+// were made from (inherit.cpp:14-232); the fused output kernels of each
+// pass write them next to the connectivity.
 
 // --------------------------------------------------------------- refine
 // Worst child quality of every candidate edge split
@@ -144,7 +108,7 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     u32 ns, u32 const* __restrict__ vs, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ ve, double const* __restrict__ x,
     double const* __restrict__ eq, double qual_floor, u32* __restrict__ cand, double* __restrict__ out,
-    u32* __restrict__ words) {
+    u32* __restrict__ words, u32* __restrict__ err) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= ns || !cand[i]) return;
   constexpr u32 VPE = DIM + 1;
@@ -170,7 +134,7 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
       for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) ve[(size_t) e * VPE + bv[l]]);
       p[VPE - 1] = split;
       double q = entity_quality(DIM, p);
-      if (!(q > 0)) words[W_ERR] = 1u;
+      if (!(q > 0)) *err = ERR_REFINE_QUALITY;
       if (q < minq) minq = q;
     }
   }
@@ -206,48 +170,91 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   Buf<double> eq;
   if (require_better) eq = mesh_qualities(m);
   Buf<double> out(ns);
-  zero_words(0, 2);
+  zero_words(0, 1);
   ProfScope ps("refine_quality_f64", (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24);
   if (dim == 3)
     hipLaunchKernelGGL(refine_quality_f64<3>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
                        u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), eq.get(), qual_floor,
-                       cand, out.get(), scratch_words());
+                       cand, out.get(), scratch_words(), dev_error_word());
   else
     hipLaunchKernelGGL(refine_quality_f64<2>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
                        u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), eq.get(), qual_floor,
-                       cand, out.get(), scratch_words());
+                       cand, out.get(), scratch_words(), dev_error_word());
   check_launch("refine_quality_f64");
   return out;
 }
 
-// refine_verts + refine_node (refine_common.cpp:29-54, refine_nodal.cpp:10-46)
+// Up to 8 tag arrays handled by one kernel (FP64 or u32 components).
+struct TagList {
+  u32 n = 0;
+  u32 ncomps[8];
+  u32 f64[8];
+  void const* src[8];
+  void* dst[8];
+};
+
+static void tag_list_add(TagList& L, Tag& t, void* dst) {
+  OHX_REQUIRE(L.n < 8, "too many vertex tags for one fused kernel");
+  L.ncomps[L.n] = t.ncomps;
+  L.f64[L.n] = t.type == TagType::F64;
+  L.src[L.n] = t.raw();
+  L.dst[L.n] = dst;
+  ++L.n;
+}
+
+// refine_verts + refine_node (refine_common.cpp:29-54, refine_nodal.cpp:10-46):
+// every FP64 vertex tag keeps its old values and gets, for each split edge
+// in order, the average ((0 + f(v0)) + f(v1)) / 2 of the edge's vertices.
+// One thread per output vertex, all tags in the same launch.
+__global__ __launch_bounds__(BLOCK) void refine_vertex_tags(u32 nv, u32 ns, u32 const* __restrict__ gen_off,
+                                                            u32 const* __restrict__ vs, TagList L) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nv + ns) return;
+  u32 v0, v1;
+  bool split = i >= nv;
+  if (split) {
+    u32 e = i - nv;
+    if (gen_off[e] == gen_off[e + 1]) return;
+    v0 = vs[2 * e];
+    v1 = vs[2 * e + 1];
+  } else {
+    v0 = v1 = i;
+  }
+  u32 o = split ? nv + gen_off[i - nv] : i;
+  for (u32 t = 0; t < L.n; ++t) {
+    u32 w = L.ncomps[t];
+    double const* f = static_cast<double const*>(L.src[t]);
+    double* d = static_cast<double*>(L.dst[t]);
+    for (u32 c = 0; c < w; ++c) {
+      if (split) {
+        double sum = 0;
+        sum += f[(size_t) v0 * w + c];
+        sum += f[(size_t) v1 * w + c];
+        sum /= 2;
+        d[(size_t) o * w + c] = sum;
+      } else {
+        d[(size_t) o * w + c] = f[(size_t) i * w + c];
+      }
+    }
+  }
+}
+
 static void refine_verts(Mesh& m, Mesh& mo, u32 const* gen_off, u32 nsplit) {
   u32 nv = count(m, 0), ns = count(m, 1);
   set_ents(mo, 0, nv + nsplit, Buf<u32>());
-  u32 const* vs = ask_down(m, 1, 0);
+  TagList L;
+  std::vector<Tag*> src;
   for (auto& tp : m.tags[0]) {
     Tag& t = *tp;
     if (t.type != TagType::F64) continue;
-    u32 w = t.ncomps;
-    Buf<double> out((size_t) (nv + nsplit) * w);
-    d2d(out.get(), t.f.get(), (size_t) nv * w);
-    double const* f = t.f.get();
-    double* o = out.get();
-    parallel_for(ns, OHX_LAMBDA(u32 i) {
-      u32 g = gen_off[i];
-      if (g == gen_off[i + 1]) return;
-      double* dst = o + (size_t) (nv + g) * w;
-      u32 v0 = vs[2 * i], v1 = vs[2 * i + 1];
-      for (u32 c = 0; c < w; ++c) {
-        double s = 0;
-        s += f[(size_t) v0 * w + c];
-        s += f[(size_t) v1 * w + c];
-        s /= 2;
-        dst[c] = s;
-      }
-    });
-    add_tag_f64(mo, 0, t.name.c_str(), w, std::move(out), t.transfer);
+    Tag& nt = add_tag_f64(mo, 0, t.name.c_str(), t.ncomps, Buf<double>((size_t) (nv + nsplit) * t.ncomps),
+                          t.transfer);
+    tag_list_add(L, t, nt.f.get());
   }
+  if (L.n)
+    hipLaunchKernelGGL(refine_vertex_tags, dim3(nblocks(nv + ns)), dim3(BLOCK), 0, stream(), nv, ns, gen_off,
+                       ask_down(m, 1, 0), L);
+  check_launch("refine_vertex_tags");
 }
 
 static u32 prods_per_dom(u32 dom_dim, u32 src_dim, u32 prod_dim) {  // refine_topology.cpp:110-121
@@ -258,96 +265,169 @@ static u32 prods_per_dom(u32 dom_dim, u32 src_dim, u32 prod_dim) {  // refine_to
   return h_down_deg[src_dim][opp_dim];
 }
 
-// refine_ents (refine_common.cpp:64-129) with split_to_domain
-// (splits_to_domains.cpp:13-56) and refine_domain_entity
-// (refine_topology.cpp:26-107)
-static void refine_ents(Mesh& m, Mesh& mo, u32 const* gen_off) {
-  u32 dim = m.dim;
-  u32 nv = count(m, 0);
-  Buf<u32> off[4], dir[4], vert[4];
-  off[0] = filled(nv + 1, 0);
-  for (u32 d = 1; d <= dim; ++d) {
-    if (!has_dim(m, d)) continue;
-    u32 nd = count(m, d);
-    u32 const* sd = ask_down(m, d, 1);
-    u32 spd = h_down_deg[d][1];
-    Buf<u32> will(nd);
-    dir[d] = Buf<u32>(nd);
-    vert[d] = Buf<u32>(nd);
-    u32* wl = will.get();
-    u32* dr = dir[d].get();
-    u32* vt = vert[d].get();
-    parallel_for(nd, OHX_LAMBDA(u32 i) {
-      u32 w = 0, dd = 0, vv = 0;
-      for (u32 j = 0; j < spd; ++j) {
-        u32 s = sd[(size_t) i * spd + j];
-        if (gen_off[s] == gen_off[s + 1]) continue;
-        w = 1;
-        dd = j;
-        vv = nv + gen_off[s];
-        break;
-      }
-      wl[i] = w;
-      dr[i] = dd;
-      vt[i] = vv;
-    });
-    off[d] = exscan(wl, nd);
-  }
-  for (u32 pd = 0; pd <= dim; ++pd) {
-    if (!m.full && 0 < pd && pd < dim) continue;
-    // setup_refine (inherit.cpp:104-138)
-    Prods p;
-    u32 np = count(m, pd);
-    p.ndoms[0] = np;
-    p.own[0] = negate_offsets(off[pd].get(), np);
-    p.offs[0] = p.own[0].get();
-    u32 lo, hi;
-    if (m.full) {
-      lo = pd > 1 ? pd : 1;
-      hi = dim;
+// Arguments of the fused refine product kernel (one prod dimension).
+struct RefineArgs {
+  u32 dim, pd;
+  u32 nv;
+  u32 n[4];                // entity counts
+  u32 const* off[4];       // split offsets of domains of each dim (off[1] = edge split offsets)
+  u32 const* verts_of[4];  // down(d, 0)
+  u32 const* edges_of[4];  // down(d, 1), d >= 2
+  u32 const* soff;         // survivor offsets of dim pd (pd >= 1)
+  u32 ppd[4];              // products per splitting domain of dim d
+  u32 go[4];               // output base of the products of domains of dim d
+  u32 seg[5];              // index space: [survivors | doms of d = 1..3]
+  u32* out_verts;          // pd >= 1
+  u32 const* cd[4];        // class_dim / class_id per dim (may be null)
+  u32 const* ci[4];
+  u32* out_cd;
+  u32* out_ci;
+};
+
+// refine_ents (refine_common.cpp:64-129) for one product dimension, fused:
+// survivors keep their vertices and class (inherit.cpp:104-138); every
+// splitting domain (split_to_domain, splits_to_domains.cpp:13-56: along its
+// first split edge) emits its products (refine_domain_entity,
+// refine_topology.cpp:26-107: the base opposite each of the edge's
+// opposite-dim entities + the new vertex), classified like the domain.
+__global__ __launch_bounds__(BLOCK) void refine_products(RefineArgs a) {
+  u32 t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t >= a.seg[4]) return;
+  u32 pd = a.pd;
+  if (t < a.seg[0]) {
+    u32 i = t, o;
+    if (pd == 0) {
+      o = i;
     } else {
-      lo = hi = (pd == 0) ? 1 : dim;
+      u32 const* of = a.off[pd];
+      if (of[i] != of[i + 1]) return;
+      o = a.soff[i];
+      for (u32 k = 0; k <= pd; ++k) a.out_verts[(size_t) o * (pd + 1) + k] = a.verts_of[pd][(size_t) i * (pd + 1) + k];
     }
-    for (u32 d = lo; d <= hi; ++d) {
-      p.ndoms[d] = count(m, d);
-      p.own[d] = scale(off[d].get(), p.ndoms[d] + 1, prods_per_dom(d, 1, pd));
-      p.offs[d] = p.own[d].get();
-    }
-    p.finish();
-    if (pd) {
-      u32 nout = p.go[4];
-      u32 vpp = pd + 1;
-      Buf<u32> vo((size_t) nout * vpp);
-      expand_into_u32(vo.get(), ask_down(m, pd, 0), p.offs[0], np, vpp);
-      for (u32 d = 1; d <= dim; ++d) {
-        if (!(p.ndoms[d] && p.offs[d]) || !p.ngen[d]) continue;
-        u32 base_dim = pd - 1;
-        u32 opp_dim = d - 1 - base_dim;
-        if (1 < opp_dim) continue;
-        u32 ops = h_down_deg[1][opp_dim];
-        u32 vpd = d + 1, vpb = vpp - 1;
-        u32 const* vd = ask_down(m, d, 0);
-        u32 const* of = off[d].get();
-        u32 const* dr = dir[d].get();
-        u32 const* vt = vert[d].get();
-        u32* out = vo.get() + (size_t) p.go[d] * vpp;
-        parallel_for(count(m, d), OHX_LAMBDA(u32 i) {
-          u32 o = of[i];
-          if (o == of[i + 1]) return;
-          u32 const* os = canon(d, 1, opp_dim, dr[i]);
-          u32 const* bo = opp(d, opp_dim);
-          u32* q = out + (size_t) o * ops * vpp;
-          for (u32 j = 0; j < ops; ++j) {
-            u32 const* bv = canon(d, base_dim, 0, bo[os[j]]);
-            for (u32 k = 0; k < vpb; ++k) q[k] = vd[(size_t) i * vpd + bv[k]];
-            q[vpb] = vt[i];
-            q += vpp;
-          }
-        });
+    if (a.out_cd) a.out_cd[o] = a.cd[pd][i];
+    if (a.out_ci) a.out_ci[o] = a.ci[pd][i];
+    return;
+  }
+  u32 d = 1;
+  while (t >= a.seg[d]) ++d;
+  u32 i = t - a.seg[d - 1];
+  u32 const* of = a.off[d];
+  u32 k0 = of[i];
+  if (k0 == of[i + 1]) return;
+  u32 ppd = a.ppd[d];
+  u32 o = a.go[d] + k0 * ppd;
+  if (pd > 0) {
+    // the domain's first split edge: its local index and new vertex
+    u32 dir = 0, edge = i;
+    if (d >= 2) {
+      u32 spd = d == 2 ? 3u : 6u;
+      for (u32 j = 0; j < spd; ++j) {
+        u32 e = a.edges_of[d][(size_t) i * spd + j];
+        if (a.off[1][e] != a.off[1][e + 1]) {
+          dir = j;
+          edge = e;
+          break;
+        }
       }
-      set_ents(mo, pd, nout, std::move(vo));
     }
-    inherit_class(m, mo, pd, p);
+    u32 vert = a.nv + a.off[1][edge];
+    u32 opp_dim = d - pd;
+    u32 const* os = canon(d, 1, opp_dim, dir);
+    u32 const* bo = opp(d, opp_dim);
+    u32 const* vd = a.verts_of[d] + (size_t) i * (d + 1);
+    for (u32 j = 0; j < ppd; ++j) {
+      u32 const* bv = canon(d, pd - 1, 0, bo[os[j]]);
+      u32* q = a.out_verts + (size_t) (o + j) * (pd + 1);
+      for (u32 k = 0; k < pd; ++k) q[k] = vd[bv[k]];
+      q[pd] = vert;
+    }
+  }
+  for (u32 j = 0; j < ppd; ++j) {
+    if (a.out_cd) a.out_cd[o + j] = a.cd[d][i];
+    if (a.out_ci) a.out_ci[o + j] = a.ci[d][i];
+  }
+}
+
+static void refine_ents(Mesh& m, Mesh& mo, u32 const* gen_off) {
+  OHX_REQUIRE(m.full, "refine needs a full-topology mesh");
+  u32 dim = m.dim;
+  RefineArgs base{};
+  base.dim = dim;
+  base.nv = count(m, 0);
+  Buf<u32> off[4];
+  base.off[1] = gen_off;  // an edge splits iff it is in the independent set
+  for (u32 d = 0; d <= dim; ++d) {
+    base.n[d] = count(m, d);
+    base.verts_of[d] = d ? ask_down(m, d, 0) : nullptr;
+  }
+  for (u32 d = 2; d <= dim; ++d) {
+    u32 const* ed = ask_down(m, d, 1);
+    base.edges_of[d] = ed;
+    u32 spd = h_down_deg[d][1];
+    off[d] = Buf<u32>(base.n[d] + 1);
+    exscan_fn(base.n[d], OHX_LAMBDA(u32 i) {
+      for (u32 j = 0; j < spd; ++j) {
+        u32 e = ed[(size_t) i * spd + j];
+        if (gen_off[e] != gen_off[e + 1]) return 1u;
+      }
+      return 0u;
+    }, off[d].get());
+    base.off[d] = off[d].get();
+  }
+  u32 nsplit[4] = {0, 0, 0, 0};
+  {
+    u32 const* ptrs[3] = {base.off[1] + base.n[1], dim >= 2 ? base.off[2] + base.n[2] : nullptr,
+                          dim >= 3 ? base.off[3] + base.n[3] : nullptr};
+    u32 got[3];
+    read_many(ptrs, 3, got);
+    for (u32 d = 1; d <= dim; ++d) nsplit[d] = got[d - 1];
+  }
+  bool has_class = find_tag(m, 0, "class_dim") != nullptr;
+  bool has_id = find_tag(m, 0, "class_id") != nullptr;
+  if (has_class)
+    for (u32 d = 0; d <= dim; ++d) {
+      base.cd[d] = u32_tag(m, d, "class_dim");
+      base.ci[d] = has_id ? u32_tag(m, d, "class_id") : nullptr;
+    }
+  for (u32 pd = 0; pd <= dim; ++pd) {
+    RefineArgs a = base;
+    a.pd = pd;
+    u32 nsurv = base.n[pd] - (pd ? nsplit[pd] : 0);
+    Buf<u32> soff;
+    if (pd) soff = negate_offsets(base.off[pd], base.n[pd]);
+    a.soff = soff.get();
+    u32 total = nsurv;
+    a.seg[0] = base.n[pd];
+    for (u32 d = 1; d <= 3; ++d) {
+      bool live = d <= dim && d >= (pd > 1 ? pd : 1);
+      a.ppd[d] = live ? prods_per_dom(d, 1, pd) : 0;
+      a.go[d] = total;
+      total += a.ppd[d] * nsplit[d];
+      a.seg[d] = a.seg[d - 1] + ((live && a.ppd[d]) ? base.n[d] : 0);
+    }
+    a.seg[4] = a.seg[3];
+    Buf<u32> verts, cdo, cio;
+    if (pd) {
+      verts = Buf<u32>((size_t) total * (pd + 1));
+      a.out_verts = verts.get();
+    }
+    if (has_class) {
+      cdo = Buf<u32>(total);
+      a.out_cd = cdo.get();
+      if (has_id) {
+        cio = Buf<u32>(total);
+        a.out_ci = cio.get();
+      }
+    }
+    if (a.seg[4]) {
+      hipLaunchKernelGGL(refine_products, dim3(nblocks(a.seg[4])), dim3(BLOCK), 0, stream(), a);
+      check_launch("refine_products");
+    }
+    if (pd) set_ents(mo, pd, total, std::move(verts));
+    if (has_class) {
+      add_tag_u32(mo, pd, "class_dim", 1, std::move(cdo));
+      if (has_id) add_tag_u32(mo, pd, "class_id", 1, std::move(cio));
+    }
   }
 }
 
@@ -361,16 +441,13 @@ static void say(char const* verb, u32 n, u32 d) {
 static u32 refine_common(Mesh& m, Buf<u32>& cand, double qual_floor, bool require_better) {
   u32 ns = count(m, 1);
   Buf<double> q = refine_qualities(m, cand.get(), qual_floor, require_better);
-  u32 words[2];
-  to_host(words, scratch_words(), 2);
-  OHX_REQUIRE(!words[W_ERR], "refine: a child element has non-positive quality (refine_qualities.cpp:72)");
-  if (!words[W_ANY]) return 0;
+  if (!read_word(W_ANY)) return 0;
   Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
   cand.release();
   q.release();
-  Buf<u32> gen_off = exscan(indset.get(), ns);
+  Buf<u32> gen_off(ns + 1);
+  u32 total = exscan_total(indset.get(), ns, gen_off.get());
   indset.release();
-  u32 total = read_at(gen_off.get(), ns);
   Mesh mo(m.dim, m.full);
   refine_verts(m, mo, gen_off.get(), total);
   refine_ents(m, mo, gen_off.get());
@@ -437,8 +514,7 @@ static void check_exposed_sides(Mesh& m, u32 ed, u32* codes) {
   u32 const* uo = u.offsets.get();
   u32 const* ua = u.adj.get();
   u32 const* ud = u.dirs.get();
-  u32* err = scratch_words() + W_ERR;
-  fill(err, 1, 0);
+  u32* err = dev_error_word();
   parallel_for(count(m, 1), OHX_LAMBDA(u32 i) {
     u32 c = codes[i];
     if (c == DONT_COLLAPSE) return;
@@ -458,12 +534,11 @@ static void check_exposed_sides(Mesh& m, u32 ed, u32* codes) {
             break;
           }
         }
-        if (l == 2) *err = 1u;
+        if (l == 2) *err = ERR_EXPOSED_SIDE;
       }
     }
     codes[i] = c;
   });
-  OHX_REQUIRE(read_word(W_ERR) == 0, "check_for_exposed_side: edge vertex not in entity");
 }
 
 // coarsen_quality (coarsen_qualities.cpp:12-79)
@@ -522,6 +597,68 @@ __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const*
   u32 c = ((len / want) < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
   codes[i] = c;
   if (c) words[W_ANY] = 1u;
+}
+
+// tags_subset (subset.cpp:15-43): every tag of the surviving vertices,
+// one thread per old vertex, all tags in one launch
+__global__ __launch_bounds__(BLOCK) void subset_tags(u32 n, u32 const* __restrict__ soff, TagList L) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  u32 o = soff[i];
+  if (o == soff[i + 1]) return;
+  for (u32 t = 0; t < L.n; ++t) {
+    u32 w = L.ncomps[t];
+    if (L.f64[t]) {
+      double const* f = static_cast<double const*>(L.src[t]);
+      double* dd = static_cast<double*>(L.dst[t]);
+      for (u32 c = 0; c < w; ++c) dd[(size_t) o * w + c] = f[(size_t) i * w + c];
+    } else {
+      u32 const* f = static_cast<u32 const*>(L.src[t]);
+      u32* dd = static_cast<u32*>(L.dst[t]);
+      for (u32 c = 0; c < w; ++c) dd[(size_t) o * w + c] = f[(size_t) i * w + c];
+    }
+  }
+}
+
+struct CoarsenArgs {
+  u32 d, n, nsame;
+  u32 const* vent;
+  u32 const* esoff;  // survivor offsets
+  u32 const* egoff;  // generated offsets
+  u32 const* gv;     // target vertex of generating entities
+  u32 const* gd;     // local index of their collapsing vertex
+  u32 const* vsoff;  // vertex renumbering (survivor offsets of vertices)
+  u32* out;
+  u32 const* cd;
+  u32 const* ci;
+  u32* out_cd;
+  u32* out_ci;
+};
+
+// concat_verts_of_ents (inherit.cpp:234-255) + coarsen_elem
+// (coarsen_topology.cpp:9-44) + remap_conn (coarsen_common.cpp:29-33) +
+// setup_coarsen class inheritance (inherit.cpp:140-156), fused: survivors
+// first, then the entities rebuilt on the collapse target (the base
+// opposite the collapsing vertex, then the target), vertex ids renumbered.
+__global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= a.n) return;
+  u32 d = a.d, vpe = d + 1;
+  u32 const* v = a.vent + (size_t) i * vpe;
+  u32 o;
+  if (a.esoff[i] != a.esoff[i + 1]) {
+    o = a.esoff[i];
+    for (u32 j = 0; j < vpe; ++j) a.out[(size_t) o * vpe + j] = a.vsoff[v[j]];
+  } else if (a.egoff[i] != a.egoff[i + 1]) {
+    o = a.nsame + a.egoff[i];
+    u32 const* bv = canon(d, d - 1, 0, opp(d, 0)[a.gd[i]]);
+    for (u32 j = 0; j + 1 < vpe; ++j) a.out[(size_t) o * vpe + j] = a.vsoff[v[bv[j]]];
+    a.out[(size_t) o * vpe + vpe - 1] = a.vsoff[a.gv[i]];
+  } else {
+    return;
+  }
+  if (a.out_cd) a.out_cd[o] = a.cd[i];
+  if (a.out_ci) a.out_ci[o] = a.ci[i];
 }
 
 static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better) {
@@ -595,8 +732,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     double const* vqp = vq.get();
     u32 const* is = indset.get();
     u32* g = gvv.get();
-    u32* err = scratch_words() + W_ERR;
-    fill(err, 1, 0);
+    u32* err = dev_error_word();
     parallel_for(nv, OHX_LAMBDA(u32 i) {
       if (!is[i]) {
         g[i] = INVALID;
@@ -615,35 +751,41 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
           break;
         }
       }
-      if (gv == INVALID) *err = 1u;
+      if (gv == INVALID) *err = ERR_NO_DESTINATION;
       g[i] = gv;
     });
-    OHX_REQUIRE(read_word(W_ERR) == 0, "vertex_dest: no destination (collapses_to_verts.cpp:100)");
   }
   codes.release();
   cq.release();
   vq.release();
   // coarsen_interior (coarsen_common.cpp:191-219)
-  Buf<u32> goff = exscan(indset.get(), nv);
-  u32 total = read_at(goff.get(), nv);
+  Buf<u32> goff(nv + 1);
+  u32 total = exscan_total(indset.get(), nv, goff.get());
   indset.release();
   Buf<u32> soff = negate_offsets(goff.get(), nv);
-  u32 nv_out = read_at(soff.get(), nv);
+  u32 nv_out = nv - total;
   Mesh mo(dim, m.full);
   set_ents(mo, 0, nv_out, Buf<u32>());
-  for (auto& tp : m.tags[0]) {  // tags_subset (subset.cpp:15-43)
-    Tag& t = *tp;
-    if (t.type == TagType::F64) {
-      Buf<double> o((size_t) nv_out * t.ncomps);
-      expand_into_f64(o.get(), t.f.get(), soff.get(), nv, t.ncomps);
-      add_tag_f64(mo, 0, t.name.c_str(), t.ncomps, std::move(o), t.transfer);
-    } else {
-      Buf<u32> o((size_t) nv_out * t.ncomps);
-      expand_into_u32(o.get(), t.u.get(), soff.get(), nv, t.ncomps);
-      add_tag_u32(mo, 0, t.name.c_str(), t.ncomps, std::move(o), t.transfer);
+  {
+    // tags_subset (subset.cpp:15-43): every vertex tag, one launch
+    TagList L;
+    for (auto& tp : m.tags[0]) {
+      Tag& t = *tp;
+      void* dst;
+      if (t.type == TagType::F64)
+        dst = add_tag_f64(mo, 0, t.name.c_str(), t.ncomps, Buf<double>((size_t) nv_out * t.ncomps), t.transfer).f.get();
+      else
+        dst = add_tag_u32(mo, 0, t.name.c_str(), t.ncomps, Buf<u32>((size_t) nv_out * t.ncomps), t.transfer).u.get();
+      tag_list_add(L, t, dst);
+    }
+    if (L.n) {
+      hipLaunchKernelGGL(subset_tags, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, soff.get(), L);
+      check_launch("subset_tags");
     }
   }
   // coarsen_all_ents (coarsen_common.cpp:91-107), top dimension first
+  bool has_class = find_tag(m, 0, "class_dim") != nullptr;
+  bool has_id = find_tag(m, 0, "class_id") != nullptr;
   Buf<u32> fused[4];
   for (u32 dd = 0; dd < dim; ++dd) {
     u32 d = dim - dd;
@@ -653,7 +795,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32 nsd = count(m, d - 1);
     fused[d - 1] = filled(nsd, 0);
     u32 vpe = d + 1, spe = h_down_deg[d][d - 1];
-    Buf<u32> will(n), same(n), gv(n), gd(n);
+    Buf<u32> will(n), gv(n), gd(n);
     {
       // collapse_to_ent (collapses_to_ents.cpp:11-82)
       u32 const* go = goff.get();
@@ -661,7 +803,6 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       u32 const* fe = fused[d] ? fused[d].get() : nullptr;
       u32* fs = fused[d - 1].get();
       u32* wl = will.get();
-      u32* sm = same.get();
       u32* gvp = gv.get();
       u32* gdp = gd.get();
       parallel_for(n, OHX_LAMBDA(u32 i) {
@@ -672,65 +813,74 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
             col = v[j];
             dir = j;
           }
-        wl[i] = 0;
-        gvp[i] = 0;
-        gdp[i] = 0;
-        if (dir == INVALID) {
-          sm[i] = 1;
-          return;
+        u32 w = 0, gvv_ = 0, gd_ = 0;
+        if (dir != INVALID) {
+          u32 target = g[col];
+          u32 gdir = INVALID;
+          for (u32 j = 0; j < vpe; ++j)
+            if (v[j] == target) gdir = j;
+          if (gdir != INVALID) {
+            fs[sent[(size_t) i * spe + opp(d, 0)[gdir]]] = 1;
+          } else if (!(fe && fe[i])) {
+            w = 1;
+            gvv_ = target;
+            gd_ = dir;
+          }
         }
-        sm[i] = 0;
-        u32 target = g[col];
-        u32 gdir = INVALID;
-        for (u32 j = 0; j < vpe; ++j)
-          if (v[j] == target) gdir = j;
-        if (gdir != INVALID) {
-          fs[sent[(size_t) i * spe + opp(d, 0)[gdir]]] = 1;
-          return;
-        }
-        if (!(fe && fe[i])) {
-          wl[i] = 1;
-          gvp[i] = target;
-          gdp[i] = dir;
-        }
+        wl[i] = w;
+        gvp[i] = gvv_;
+        gdp[i] = gd_;
       });
     }
-    Buf<u32> egoff = exscan(will.get(), n);
-    Buf<u32> esoff = exscan(same.get(), n);
+    Buf<u32> egoff(n + 1), esoff(n + 1);
+    {
+      u32 const* wl = will.get();
+      u32 const* go = goff.get();
+      exscan_fn(n, OHX_LAMBDA(u32 i) { return wl[i]; }, egoff.get());
+      // entities with no collapsing vertex survive unchanged
+      exscan_fn(n, OHX_LAMBDA(u32 i) {
+        for (u32 j = 0; j < vpe; ++j) {
+          u32 v = vent[(size_t) i * vpe + j];
+          if (go[v] != go[v + 1]) return 0u;
+        }
+        return 1u;
+      }, esoff.get());
+    }
     will.release();
-    same.release();
-    u32 ngen = read_at(egoff.get(), n), nsame = read_at(esoff.get(), n);
+    u32 tots[2];
+    u32 const* pt[2] = {egoff.get() + n, esoff.get() + n};
+    read_many(pt, 2, tots);
+    u32 ngen = tots[0], nsame = tots[1];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
-    expand_into_u32(vout.get(), vent, esoff.get(), n, vpe);
-    {
-      // coarsen_elem (coarsen_topology.cpp:9-44): base opposite the
-      // collapsing vertex, then the target vertex
-      u32 const* eg = egoff.get();
-      u32 const* gvp = gv.get();
-      u32 const* gdp = gd.get();
-      u32* out = vout.get() + (size_t) nsame * vpe;
-      parallel_for(n, OHX_LAMBDA(u32 i) {
-        u32 o = eg[i];
-        if (o == eg[i + 1]) return;
-        u32 const* bv = canon(d, d - 1, 0, opp(d, 0)[gdp[i]]);
-        for (u32 j = 0; j + 1 < vpe; ++j) out[(size_t) o * vpe + j] = vent[(size_t) i * vpe + bv[j]];
-        out[(size_t) o * vpe + vpe - 1] = gvp[i];
-      });
+    Buf<u32> cdo, cio;
+    CoarsenArgs a{};
+    a.d = d;
+    a.n = n;
+    a.nsame = nsame;
+    a.vent = vent;
+    a.esoff = esoff.get();
+    a.egoff = egoff.get();
+    a.gv = gv.get();
+    a.gd = gd.get();
+    a.vsoff = soff.get();
+    a.out = vout.get();
+    if (has_class) {
+      cdo = Buf<u32>(nsame + ngen);
+      a.cd = u32_tag(m, d, "class_dim");
+      a.out_cd = cdo.get();
+      if (has_id) {
+        cio = Buf<u32>(nsame + ngen);
+        a.ci = u32_tag(m, d, "class_id");
+        a.out_ci = cio.get();
+      }
     }
-    {
-      // remap_conn (coarsen_common.cpp:29-33)
-      u32* vo = vout.get();
-      u32 const* s = soff.get();
-      parallel_for((nsame + ngen) * vpe, OHX_LAMBDA(u32 k) { vo[k] = s[vo[k]]; });
-    }
+    hipLaunchKernelGGL(coarsen_outputs, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), a);
+    check_launch("coarsen_outputs");
     set_ents(mo, d, nsame + ngen, std::move(vout));
-    Prods p;  // setup_coarsen (inherit.cpp:140-156)
-    p.ndoms[0] = n;
-    p.offs[0] = esoff.get();
-    p.ndoms[d] = n;
-    p.offs[d] = egoff.get();
-    p.finish();
-    inherit_class(m, mo, d, p);
+    if (has_class) {
+      add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
+      if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
+    }
   }
   say("collapsed", total, 1);
   overwrite(m, std::move(mo));
@@ -968,7 +1118,7 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
     u32 ne, u32 const* __restrict__ uo, u32 const* __restrict__ ua, u32 const* __restrict__ ud,
     u32 const* __restrict__ ev, u32 const* __restrict__ vt, double const* __restrict__ x,
     double const* __restrict__ eq, u32* __restrict__ cand, double* __restrict__ out, u32* __restrict__ rsz,
-    u32* __restrict__ words) {
+    u32* __restrict__ words, u32* __restrict__ err) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= ne || !cand[i]) return;
   double old_minq = 1;
@@ -983,7 +1133,7 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
     return;
   }
   if (rs < 3) {
-    words[W_ERR] = 1u;
+    *err = ERR_EDGE_RING;
     cand[i] = 0;
     return;
   }
@@ -1058,6 +1208,58 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_f64(
   }
 }
 
+__global__ __launch_bounds__(BLOCK) void copy_tags(u32 n, TagList L) {
+  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  for (u32 t = 0; t < L.n; ++t) {
+    u32 w = L.ncomps[t];
+    if (L.f64[t]) {
+      double const* f = static_cast<double const*>(L.src[t]);
+      double* dd = static_cast<double*>(L.dst[t]);
+      for (u32 c = 0; c < w; ++c) dd[(size_t) i * w + c] = f[(size_t) i * w + c];
+    } else {
+      u32 const* f = static_cast<u32 const*>(L.src[t]);
+      u32* dd = static_cast<u32*>(L.dst[t]);
+      for (u32 c = 0; c < w; ++c) dd[(size_t) i * w + c] = f[(size_t) i * w + c];
+    }
+  }
+}
+
+struct SwapKeepArgs {
+  u32 n, ne, vpe, nsame;
+  u32 const* soff;
+  u32 const* goff;
+  u32 const* vent;
+  u32* out;
+  u32 const* cd;
+  u32 const* ci;
+  u32 const* ecd;
+  u32 const* eci;
+  u32* out_cd;
+  u32* out_ci;
+};
+
+// concat_verts_of_ents for the survivors of a swap, and the class tags of
+// survivors (their own) and of products (their edge's, setup_swap,
+// inherit.cpp:158-166): threads [0, n) are entities, [n, n + ne) edges
+__global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
+  u32 t = blockIdx.x * BLOCK + threadIdx.x;
+  if (t < a.n) {
+    u32 o = a.soff[t];
+    if (o == a.soff[t + 1]) return;
+    for (u32 j = 0; j < a.vpe; ++j) a.out[(size_t) o * a.vpe + j] = a.vent[(size_t) t * a.vpe + j];
+    if (a.out_cd) a.out_cd[o] = a.cd[t];
+    if (a.out_ci) a.out_ci[o] = a.ci[t];
+    return;
+  }
+  u32 e = t - a.n;
+  if (e >= a.ne) return;
+  for (u32 k = a.goff[e]; k < a.goff[e + 1]; ++k) {
+    if (a.out_cd) a.out_cd[a.nsame + k] = a.ecd[e];
+    if (a.out_ci) a.out_ci[a.nsame + k] = a.eci[e];
+  }
+}
+
 // swap_common + swap_interior + swap_ents (swap.cpp:27-136)
 static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   u32 ne = count(m, 1);
@@ -1071,19 +1273,16 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   Buf<double> eq = mesh_qualities(m);
   Buf<double> q(ne);
   Buf<u32> rsz = filled(ne, 0);
-  zero_words(0, 2);
+  zero_words(0, 1);
   {
     ProfScope ps("swap_quality_f64", (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24);
     hipLaunchKernelGGL(swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, u.offsets.get(),
                        u.adj.get(), u.dirs.get(), ev, vt, coords(m), eq.get(), cand.get(), q.get(), rsz.get(),
-                       scratch_words());
+                       scratch_words(), dev_error_word());
     check_launch("swap_quality_f64");
   }
   eq.release();
-  u32 words[2];
-  to_host(words, scratch_words(), 2);
-  OHX_REQUIRE(!words[W_ERR], "find_edge_ring: ring does not close or has fewer than 3 tets");
-  if (!words[W_ANY]) return 0;
+  if (!read_word(W_ANY)) return 0;
   Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
   cand.release();
   q.release();
@@ -1091,57 +1290,84 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   u32 nv = count(m, 0);
   Mesh mo(m.dim, m.full);
   set_ents(mo, 0, nv, Buf<u32>());
-  for (auto& tp : m.tags[0]) {  // copy_tags
-    Tag& t = *tp;
-    if (t.type == TagType::F64)
-      add_tag_f64(mo, 0, t.name.c_str(), t.ncomps, copy_f64(t.f.get(), nv * t.ncomps), t.transfer);
-    else
-      add_tag_u32(mo, 0, t.name.c_str(), t.ncomps, copy(t.u.get(), nv * t.ncomps), t.transfer);
+  {
+    // copy_tags (tag.cpp): the vertices are unchanged, one launch for all tags
+    TagList L;
+    for (auto& tp : m.tags[0]) {
+      Tag& t = *tp;
+      void* dst;
+      if (t.type == TagType::F64)
+        dst = add_tag_f64(mo, 0, t.name.c_str(), t.ncomps, Buf<double>((size_t) nv * t.ncomps), t.transfer).f.get();
+      else
+        dst = add_tag_u32(mo, 0, t.name.c_str(), t.ncomps, Buf<u32>((size_t) nv * t.ncomps), t.transfer).u.get();
+      tag_list_add(L, t, dst);
+    }
+    if (L.n) {
+      hipLaunchKernelGGL(copy_tags, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, L);
+      check_launch("copy_tags");
+    }
   }
   u32 const* is = indset.get();
   u32 const* rs = rsz.get();
+  bool has_class = m.full && find_tag(m, 0, "class_dim") != nullptr;
+  bool has_id = has_class && find_tag(m, 0, "class_id") != nullptr;
   u32 dlo = m.full ? 1 : m.dim;
   for (u32 d = dlo; d <= m.dim; ++d) {
-    Buf<u32> cnt(ne);
-    {
-      u32* c = cnt.get();
-      u32 dd = d;
-      parallel_for(ne, OHX_LAMBDA(u32 i) {
-        c[i] = is[i] ? count_swap_ents(rs[i], dd, k_sw_nint[rs[i]], k_sw_sizes[rs[i]]) : 0u;
-      });
-    }
-    Buf<u32> goff = exscan(cnt.get(), ne);
-    cnt.release();
-    u32 ngen = read_at(goff.get(), ne);
+    u32 n = count(m, d);
     u32 vpe = d + 1;
-    Buf<u32> gen((size_t) ngen * vpe);
+    u32 lpe = h_down_deg[d][1];
+    u32 const* le = ask_down(m, d, 1);
+    // products per swapped edge (edge_topo_offset, swap_topology.cpp:16-44)
+    // and survivors: entities adjacent to no swapped edge (swap.cpp:41-45)
+    Buf<u32> goff(ne + 1), soff(n + 1);
+    exscan_fn(ne, OHX_LAMBDA(u32 i) {
+      return is[i] ? count_swap_ents(rs[i], d, k_sw_nint[rs[i]], k_sw_sizes[rs[i]]) : 0u;
+    }, goff.get());
+    exscan_fn(n, OHX_LAMBDA(u32 i) {
+      for (u32 j = 0; j < lpe; ++j)
+        if (is[le[(size_t) i * lpe + j]]) return 0u;
+      return 1u;
+    }, soff.get());
+    u32 tots[2];
+    u32 const* pt[2] = {goff.get() + ne, soff.get() + n};
+    read_many(pt, 2, tots);
+    u32 ngen = tots[0], nsame = tots[1];
+    Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     {
       ProfScope ps("swap_cavity_f64", (double) ne * 12 + (double) ngen * vpe * 4);
       hipLaunchKernelGGL(swap_cavity_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, d, is, goff.get(),
-                         u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m), gen.get());
+                         u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m),
+                         vout.get() + (size_t) nsame * vpe);
       check_launch("swap_cavity_f64");
     }
-    Buf<u32> old = mark_up(m, 1, d, is);
-    u32 n = count(m, d);
-    {
-      u32* o = old.get();
-      parallel_for(n, OHX_LAMBDA(u32 i) { o[i] = !o[i]; });
+    Buf<u32> cdo, cio;
+    SwapKeepArgs a{};
+    a.n = n;
+    a.ne = ne;
+    a.vpe = vpe;
+    a.nsame = nsame;
+    a.soff = soff.get();
+    a.goff = goff.get();
+    a.vent = ask_down(m, d, 0);
+    a.out = vout.get();
+    if (has_class) {
+      cdo = Buf<u32>(nsame + ngen);
+      a.cd = u32_tag(m, d, "class_dim");
+      a.ecd = u32_tag(m, 1, "class_dim");
+      a.out_cd = cdo.get();
+      if (has_id) {
+        cio = Buf<u32>(nsame + ngen);
+        a.ci = u32_tag(m, d, "class_id");
+        a.eci = u32_tag(m, 1, "class_id");
+        a.out_ci = cio.get();
+      }
     }
-    Buf<u32> soff = exscan(old.get(), n);
-    old.release();
-    u32 nsame = read_at(soff.get(), n);
-    Buf<u32> vout((size_t) (nsame + ngen) * vpe);
-    expand_into_u32(vout.get(), ask_down(m, d, 0), soff.get(), n, vpe);
-    d2d(vout.get() + (size_t) nsame * vpe, gen.get(), (size_t) ngen * vpe);
+    hipLaunchKernelGGL(swap_keep, dim3(nblocks(n + ne)), dim3(BLOCK), 0, stream(), a);
+    check_launch("swap_keep");
     set_ents(mo, d, nsame + ngen, std::move(vout));
-    if (m.full) {
-      Prods p;  // setup_swap (inherit.cpp:158-166)
-      p.ndoms[0] = n;
-      p.offs[0] = soff.get();
-      p.ndoms[1] = ne;
-      p.offs[1] = goff.get();
-      p.finish();
-      inherit_class(m, mo, d, p);
+    if (has_class) {
+      add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
+      if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
     }
   }
   say("swapped", total, 1);
@@ -1164,14 +1390,16 @@ struct Ctx {
   u32 ops = 0, max_ops = 0;
   AdaptStats st;
 };
-void adapt_summary(Mesh& m) {  // adapt.cpp:22-34
+// adapt_summary (adapt.cpp:22-34) only prints: it reads the mesh and changes
+// nothing, so it runs only when the per-pass lines are printed
+void adapt_summary(Mesh& m) {
+  if (!g_verbose) return;
   u32 nel = count(m, m.dim);
   double mq = mesh_min_quality(m);
   u32 ne = count(m, 1);
   Buf<double> s = measure_edges(m);
   double lo = min_f64(s.get(), ne, 1), hi = max_f64(s.get(), ne, 1);
-  if (g_verbose)
-    std::printf("%10u elements, min quality %.0f%%, metric range %.2f - %.2f\n", nel, mq * 100.0, lo, hi);
+  std::printf("%10u elements, min quality %.0f%%, metric range %.2f - %.2f\n", nel, mq * 100.0, lo, hi);
 }
 bool incr_op(Mesh& m, Ctx& c) {  // adapt.cpp:36-45
   if (c.ops > c.max_ops) return false;

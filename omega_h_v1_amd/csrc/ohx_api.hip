@@ -179,6 +179,7 @@ unsigned ohx_element_qualities_f64(unsigned dim, unsigned n, unsigned const* ev,
     StreamScope sc((hipStream_t) s);
     Buf<double> q = element_qualities(dim, n, ev, x);
     d2d(out, q.get(), n);
+    sync();
     return 0u;
   });
 }
@@ -201,6 +202,7 @@ unsigned ohx_measure_edges_f64(unsigned ne, unsigned const* ev, double const* x,
       double want = (h[a] + h[b]) / 2;
       out[i] = len / want;
     });
+    sync();
     return 0u;
   });
 }
@@ -209,6 +211,7 @@ unsigned ohx_exscan_u32(unsigned const* a, unsigned n, unsigned* out, void* s) {
   return guard(OHX_ERROR, [&]() -> unsigned {
     StreamScope sc((hipStream_t) s);
     exscan(a, n, out);
+    sync();
     return 0u;
   });
 }
@@ -224,6 +227,7 @@ unsigned ohx_up_from_down(unsigned hd, unsigned ld, unsigned nh, unsigned nl, un
     d2d(offs, u.offsets.get(), (size_t) nl + 1);
     d2d(adj, u.adj.get(), nuses);
     d2d(dirs, u.dirs.get(), nuses);
+    sync();
     return 0u;
   });
 }
@@ -242,6 +246,7 @@ unsigned ohx_find_indset(unsigned n, unsigned const* offs, unsigned const* adj, 
     m.star[0][1] = std::move(g);
     Buf<u32> st = find_indset(m, 0, filter, good);
     d2d(state, st.get(), n);
+    sync();
     return 0u;
   });
 }
@@ -253,6 +258,7 @@ unsigned ohx_fp64_selftest(unsigned n, double const* a, double const* b, double*
       so[i] = sqrt(a[i]);
       dq[i] = a[i] / b[i];
     });
+    sync();
     return 0u;
   });
 }

@@ -46,7 +46,8 @@ void fail(char const* what, char const* file, int line);
 // host reads back (scan totals, reduction results, convergence flags).
 struct Runtime {
   hipStream_t stream = nullptr;
-  void* pinned = nullptr;  // 4 KiB of hipHostMalloc'd memory
+  void* pinned = nullptr;      // 4 KiB of mapped pinned host memory ...
+  void* pinned_dev = nullptr;  // ... and its device address
   int device = 0;
 };
 Runtime& rt();
@@ -55,15 +56,44 @@ extern hipStream_t g_stream_override;
 inline hipStream_t stream() { return g_stream_override ? g_stream_override : rt().stream; }
 struct StreamScope {
   hipStream_t prev;
-  explicit StreamScope(hipStream_t s) : prev(g_stream_override) { if (s) g_stream_override = s; }
-  // temporaries of the call go back to the arena, which reuses blocks in
-  // library-stream order: finish the caller's stream before that
+  bool own;
+  // s == nullptr: the call runs on the library stream and completes before
+  // returning (the caller has no handle to wait on); otherwise it runs on
+  // the caller's stream, which is drained at the end because the call's
+  // temporaries go back to the arena, which reuses blocks in library-stream
+  // order
+  explicit StreamScope(hipStream_t s) : prev(g_stream_override), own(s == nullptr) {
+    if (s) g_stream_override = s;
+  }
   ~StreamScope() {
-    if (g_stream_override && g_stream_override != prev) (void) hipStreamSynchronize(g_stream_override);
+    (void) hipStreamSynchronize(stream());
     g_stream_override = prev;
   }
 };
 void sync();
+
+// Host-visible slots (8 bytes each) in mapped pinned memory: kernels write
+// the few scalars the host needs (scan totals, reductions, error codes)
+// straight into them, so a read costs one stream synchronisation and no
+// copy.  Slot 0 is the device error word, checked by every sync().
+enum Slot : unsigned { SLOT_ERROR = 0, SLOT_SCAN = 1, SLOT_REDUCE = 2, SLOT_GATHER = 8, SLOT_STAGE = 32 };
+template <class T>
+inline T* host_slot(unsigned k) { return reinterpret_cast<T*>(static_cast<char*>(rt().pinned) + 8 * k); }
+template <class T>
+inline T* dev_slot(unsigned k) { return reinterpret_cast<T*>(static_cast<char*>(rt().pinned_dev) + 8 * k); }
+inline unsigned* dev_error_word() { return dev_slot<unsigned>(SLOT_ERROR); }
+// device error codes (set by kernels, reported by the next sync())
+enum DevError : unsigned {
+  ERR_NONE = 0,
+  ERR_REFLECT_NOT_FOUND = 1,
+  ERR_DUAL_NONMANIFOLD = 2,
+  ERR_HINGE_SIDES = 3,
+  ERR_REFINE_QUALITY = 4,
+  ERR_EXPOSED_SIDE = 5,
+  ERR_NO_DESTINATION = 6,
+  ERR_EDGE_RING = 7,
+  ERR_SCAN_SPIN = 8,
+};
 
 // ------------------------------------------------------------------ arena
 // Caching allocator over hipMalloc: power-of-two size classes, free lists
@@ -136,11 +166,13 @@ std::vector<T> host_copy(T const* src, size_t n) {
 // read one device scalar (one D2H round trip)
 template <class T>
 T read_at(T const* p, size_t i) {
-  T* h = static_cast<T*>(rt().pinned);
+  T* h = host_slot<T>(SLOT_STAGE);
   OHX_CHECK(hipMemcpyAsync(h, p + i, sizeof(T), hipMemcpyDeviceToHost, stream()));
   sync();
   return *h;
 }
+// read several device scalars with one launch and one synchronisation
+void read_many(unsigned const* const* ptrs, unsigned n, unsigned* out);
 template <class T>
 void d2d(T* dst, T const* src, size_t n) {
   if (!n) return;

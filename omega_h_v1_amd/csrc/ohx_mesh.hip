@@ -212,13 +212,15 @@ __global__ __launch_bounds__(BLOCK) void upfd_count(u32 nuses, u32 const* __rest
   if (i < nuses) atomicAdd(&counts[in[i]], 1u);
 }
 
+// slots are taken by counting the bucket back down (the bucket is sorted
+// afterwards, so the slot order does not matter)
 __global__ __launch_bounds__(BLOCK) void upfd_fill(u32 nuses, u32 const* __restrict__ in,
-                                                   u32 const* __restrict__ offs, u32* __restrict__ cursor,
+                                                   u32 const* __restrict__ offs, u32* __restrict__ counts,
                                                    u32* __restrict__ out) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nuses) return;
   u32 d = in[i];
-  u32 j = atomicAdd(&cursor[d], 1u);
+  u32 j = atomicSub(&counts[d], 1u) - 1u;
   out[offs[d] + j] = i;
 }
 
@@ -255,7 +257,6 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
     hipLaunchKernelGGL(upfd_count, dim3(nblocks(nuses)), dim3(BLOCK), 0, stream(), nuses, lows_of_highs,
                        counts.get());
   u.offsets = exscan(counts.get(), nlows);
-  fill(counts.get(), nlows, 0);
   u.adj = Buf<u32>(nuses);
   u.dirs = Buf<u32>(nuses);
   if (nuses)
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(BLOCK) void reflect_down_kernel(u32 nh, u32 const* 
         }
       }
     }
-    if (found == INVALID) *err = 1u;
+    if (found == INVALID) *err = ERR_REFLECT_NOT_FOUND;
     out[(size_t) i * LPH + j] = found;
   }
 }
@@ -322,8 +323,7 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   u32 lph = h_down_deg[hd][ld];
   Buf<u32> out((size_t) nh * lph);
   if (!nh) return out;
-  u32* err = scratch_words();
-  fill(err, 1, 0);
+  u32* err = dev_error_word();
   // compulsory bytes: high vertices in, low ids out, lows-of-vertex CSR and
   // low vertices read once
   ProfScope ps("reflect_down", (double) nh * ((hd + 1) * 4 + lph * 4) +
@@ -342,7 +342,6 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   else
     fail("reflect_down: unsupported dimensions", __FILE__, __LINE__);
   check_launch("reflect_down");
-  OHX_REQUIRE(read_at(err, 0) == 0, "reflect_down: a sub-entity was not found (non-conforming mesh)");
   return out;
 }
 
@@ -379,8 +378,8 @@ static Graph star_general(Mesh& m, u32 ld, u32 hd) {
     dg[i] = size;
   });
   Graph g;
-  g.offsets = exscan(dg, nl);
-  u32 nadj = read_at(g.offsets.get(), nl);
+  g.offsets = Buf<u32>(nl + 1);
+  u32 nadj = exscan_total(dg, nl, g.offsets.get());
   g.adj = Buf<u32>(nadj);
   u32* ga = g.adj.get();
   u32 const* go = g.offsets.get();
@@ -459,8 +458,8 @@ static Graph star_edge_tet(Mesh& m) {
   u32* dg = deg.get();
   parallel_for(ne, OHX_LAMBDA(u32 i) { dg[i] = 2 * (to[i + 1] - to[i]) + (ro[i + 1] - ro[i]); });
   Graph g;
-  g.offsets = exscan(dg, ne);
-  g.adj = Buf<u32>(read_at(g.offsets.get(), ne));
+  g.offsets = Buf<u32>(ne + 1);
+  g.adj = Buf<u32>(exscan_total(dg, ne, g.offsets.get()));
   u32* ga = g.adj.get();
   u32 const* go = g.offsets.get();
   parallel_for(ne, OHX_LAMBDA(u32 i) {
@@ -497,8 +496,7 @@ static Buf<u32> dual_from_verts(Mesh& m) {
   u32* o = out.get();
   u32 const* uo = u.offsets.get();
   u32 const* ua = u.adj.get();
-  u32* err = scratch_words();
-  fill(err, 1, 0);
+  u32* err = dev_error_word();
   parallel_for(ne, OHX_LAMBDA(u32 i) {
     for (u32 j = 0; j < spe; ++j) {
       u32 const* sc = canon(dim, sd, 0, j);
@@ -525,11 +523,10 @@ static Buf<u32> dual_from_verts(Mesh& m) {
           ++nfound;
         }
       }
-      if (nfound > 1) *err = 1u;
+      if (nfound > 1) *err = ERR_DUAL_NONMANIFOLD;
       o[(size_t) i * spe + j] = found;
     }
   });
-  OHX_REQUIRE(read_at(err, 0) == 0, "dual_from_verts: non-manifold side");
   return out;
 }
 
@@ -569,8 +566,8 @@ void bridge_graph(u32 n, u32 const* offs, u32 const* adj, u32* p_nedges, Buf<u32
       if (i < adj[j]) ++c;
     dg[i] = c;
   });
-  Buf<u32> bo = exscan(dg, n);
-  u32 ne = read_at(bo.get(), n);
+  Buf<u32> bo(n + 1);
+  u32 ne = exscan_total(dg, n, bo.get());
   *ev = Buf<u32>((size_t) ne * 2);
   if (dirs) *dirs = Buf<u32>(ne);
   u32* e2 = ev->get();
@@ -853,8 +850,7 @@ static void derive_class_dim(Mesh& m, double crease_angle) {
     u32 const* h = bh.get();
     u32 const* ho = hs.offsets.get();
     u32 const* ha = hs.adj.get();
-    u32* err = scratch_words();
-    fill(err, 1, 0);
+    u32* err = dev_error_word();
     parallel_for(nh, OHX_LAMBDA(u32 i) {
       if (!h[i]) {
         d[i] = 2.0;  // not a boundary hinge: no angle
@@ -864,12 +860,11 @@ static void derive_class_dim(Mesh& m, double crease_angle) {
       for (u32 j = ho[i]; j < ho[i + 1]; ++j)
         if (b[ha[j]]) {
           if (nadj < 2) adj[nadj] = ha[j];
-          else *err = 1u;
+          else *err = ERR_HINGE_SIDES;
           ++nadj;
         }
       d[i] = dot(ld3(n + 3 * (size_t) adj[0]), ld3(n + 3 * (size_t) adj[1]));
     });
-    OHX_REQUIRE(read_at(err, 0) == 0, "derive_model: more than two boundary sides at a hinge");
   }
   std::vector<double> hd = host_copy(dots.get(), nh);
   std::vector<u32> crease(nh);
@@ -939,8 +934,8 @@ static void host_components(u32 n, std::vector<u32> const& offs, std::vector<u32
 static void equal_order_class_id(Mesh& m, u32 d) {
   u32 n = count(m, d);
   Buf<u32> eq = mark_class(m, d, d, INVALID);
-  Buf<u32> eqo = exscan(eq.get(), n);
-  u32 neq = read_at(eqo.get(), n);
+  Buf<u32> eqo(n + 1);
+  u32 neq = exscan_total(eq.get(), n, eqo.get());
   Buf<u32> br = mark_class(m, d - 1, d, INVALID);
   u32 bpe = h_down_deg[d][d - 1];
   u32 const* be = ask_down(m, d, d - 1);
@@ -958,8 +953,8 @@ static void equal_order_class_id(Mesh& m, u32 d) {
       dg[e[i]] = k;
     });
   }
-  Buf<u32> offs = exscan(deg.get(), neq);
-  u32 nadj = read_at(offs.get(), neq);
+  Buf<u32> offs(neq + 1);
+  u32 nadj = exscan_total(deg.get(), neq, offs.get());
   Buf<u32> adj(nadj);
   {
     u32* a = adj.get();

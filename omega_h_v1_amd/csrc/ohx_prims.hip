@@ -2,94 +2,11 @@
 #include <cfloat>
 
 #include "ohx_prims.hpp"
+#include "ohx_scan.hpp"
 
 namespace ohx {
 
 namespace {
-constexpr u32 SCAN_T = 256;
-constexpr u32 SCAN_I = 16;
-constexpr u32 SCAN_TILE = SCAN_T * SCAN_I;  // 4096 items per block
-
-__device__ inline u32 wave_incl_scan(u32 x) {
-  u32 lane = threadIdx.x & 63u;
-#pragma unroll
-  for (u32 d = 1; d < 64; d <<= 1) {
-    u32 y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
-  return x;
-}
-
-// exclusive scan of one value per thread over the 256-thread block
-__device__ inline u32 block_excl_scan(u32 x, u32* lds_w, u32* total) {
-  u32 incl = wave_incl_scan(x);
-  u32 wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-  if (lane == 63u) lds_w[wave] = incl;
-  __syncthreads();
-  u32 off = 0, tot = 0;
-#pragma unroll
-  for (u32 w = 0; w < SCAN_T / 64; ++w) {
-    u32 v = lds_w[w];
-    if (w < wave) off += v;
-    tot += v;
-  }
-  *total = tot;
-  __syncthreads();
-  return off + incl - x;
-}
-
-__global__ __launch_bounds__(SCAN_T) void scan_tile_sums(u32 const* a, u32 n, u32* sums) {
-  __shared__ u32 lds_w[SCAN_T / 64];
-  u32 base = blockIdx.x * SCAN_TILE;
-  u32 s = 0;
-#pragma unroll
-  for (u32 k = 0; k < SCAN_I; ++k) {
-    u32 idx = base + k * SCAN_T + threadIdx.x;
-    if (idx < n) s += a[idx];
-  }
-  u32 tot;
-  (void) block_excl_scan(s, lds_w, &tot);
-  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-
-// scans the tile of block b, adds offs[b] (if given), writes out[i+1]
-__global__ __launch_bounds__(SCAN_T) void scan_tile_apply(u32 const* a, u32 n, u32 const* offs,
-                                                          u32* out) {
-  __shared__ u32 tile[SCAN_TILE + SCAN_TILE / 32];
-  __shared__ u32 lds_w[SCAN_T / 64];
-  u32 base = blockIdx.x * SCAN_TILE;
-  auto pad = [](u32 i) { return i + (i >> 5); };
-#pragma unroll
-  for (u32 k = 0; k < SCAN_I; ++k) {
-    u32 li = k * SCAN_T + threadIdx.x;
-    u32 idx = base + li;
-    tile[pad(li)] = (idx < n) ? a[idx] : 0u;
-  }
-  __syncthreads();
-  u32 mine[SCAN_I];
-  u32 s = 0;
-#pragma unroll
-  for (u32 j = 0; j < SCAN_I; ++j) {
-    mine[j] = tile[pad(threadIdx.x * SCAN_I + j)];
-    s += mine[j];
-  }
-  u32 tot;
-  u32 run = block_excl_scan(s, lds_w, &tot) + (offs ? offs[blockIdx.x] : 0u);
-#pragma unroll
-  for (u32 j = 0; j < SCAN_I; ++j) {
-    run += mine[j];
-    tile[pad(threadIdx.x * SCAN_I + j)] = run;
-  }
-  __syncthreads();
-#pragma unroll
-  for (u32 k = 0; k < SCAN_I; ++k) {
-    u32 li = k * SCAN_T + threadIdx.x;
-    u32 idx = base + li;
-    if (idx < n) out[idx + 1] = tile[pad(li)];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
-}
-
 template <class T>
 struct MinOp {
   __device__ static T apply(T a, T b) { return b < a ? b : a; }
@@ -148,34 +65,57 @@ T reduce(T const* a, u32 n, u32 begin, T init) {
   if (n <= begin) return init;
   u32 nb = nblocks(n - begin, 256);
   if (nb > RED_BLOCKS) nb = RED_BLOCKS;
-  Buf<T> part(nb + 1);
+  Buf<T> part(nb);
   hipLaunchKernelGGL((reduce_partial<T, Op>), dim3(nb), dim3(256), 0, stream(), a, begin, n, init,
                      part.get());
   hipLaunchKernelGGL((reduce_final<T, Op>), dim3(1), dim3(256), 0, stream(), part.get(), nb, init,
-                     part.get() + nb);
+                     dev_slot<T>(SLOT_REDUCE));
   check_launch("reduce");
-  return read_at(part.get(), nb);
+  sync();
+  return *host_slot<T>(SLOT_REDUCE);
 }
 }  // namespace
 
+namespace {
+struct ScanState {
+  u64* tiles = nullptr;  // process-lifetime tile words (never cleared)
+  u32 cap = 0;
+  u64* ticket = nullptr;
+  u32 epoch = 0;
+};
+ScanState& scan_state() {
+  static ScanState st;
+  return st;
+}
+}  // namespace
+
+ScanTiles scan_tiles_for(u32 nt) {
+  ScanState& st = scan_state();
+  if (!st.ticket) {
+    st.ticket = static_cast<u64*>(dev_alloc(sizeof(u64)));
+    OHX_CHECK(hipMemsetAsync(st.ticket, 0xff, sizeof(u64), stream()));
+  }
+  if (nt > st.cap) {
+    // a larger tile array; the old one is released once the stream drains
+    if (st.tiles) {
+      sync();
+      dev_free(st.tiles);
+    }
+    st.cap = nt < 1024 ? 1024 : nt;
+    st.tiles = static_cast<u64*>(dev_alloc(sizeof(u64) * st.cap));
+    OHX_CHECK(hipMemsetAsync(st.tiles, 0xff, sizeof(u64) * st.cap, stream()));
+  }
+  // epochs stay below 2^30 - 1, so fresh all-ones words never match one
+  st.epoch = st.epoch + 1 >= 0x3fffffffu ? 1u : st.epoch + 1;
+  return ScanTiles{st.tiles, st.ticket, st.epoch};
+}
+
 void exscan(u32 const* a, u32 n, u32* out) {
-  if (n == 0) {
-    fill(out, 1, 0);
-    return;
-  }
-  u32 nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-  if (nb == 1) {
-    hipLaunchKernelGGL(scan_tile_apply, dim3(1), dim3(SCAN_T), 0, stream(), a, n, nullptr, out);
-    check_launch("scan_tile_apply");
-    return;
-  }
-  Buf<u32> sums(nb);
-  Buf<u32> offs(nb + 1);
-  hipLaunchKernelGGL(scan_tile_sums, dim3(nb), dim3(SCAN_T), 0, stream(), a, n, sums.get());
-  check_launch("scan_tile_sums");
-  exscan(sums.get(), nb, offs.get());
-  hipLaunchKernelGGL(scan_tile_apply, dim3(nb), dim3(SCAN_T), 0, stream(), a, n, offs.get(), out);
-  check_launch("scan_tile_apply");
+  exscan_fn(n, OHX_LAMBDA(u32 i) { return a[i]; }, out);
+}
+
+u32 exscan_total(u32 const* a, u32 n, u32* out) {
+  return exscan_fn_total(n, OHX_LAMBDA(u32 i) { return a[i]; }, out);
 }
 
 Buf<u32> exscan(u32 const* a, u32 n) {
@@ -230,10 +170,9 @@ Buf<double> copy_f64(double const* a, u32 n) {
 }
 
 Buf<u32> negate_offsets(u32 const* offs, u32 n) {
-  Buf<u32> neg(n);
-  u32* g = neg.get();
-  parallel_for(n, OHX_LAMBDA(u32 i) { g[i] = (offs[i + 1] == offs[i]) ? 1u : 0u; });
-  return exscan(g, n);
+  Buf<u32> o(n + 1);
+  exscan_fn(n, OHX_LAMBDA(u32 i) { return (offs[i + 1] == offs[i]) ? 1u : 0u; }, o.get());
+  return o;
 }
 
 Buf<u32> scale(u32 const* a, u32 n, u32 s) {

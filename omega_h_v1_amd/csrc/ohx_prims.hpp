@@ -10,6 +10,8 @@ namespace ohx {
 // out[0] = 0, out[i+1] = a[0] + ... + a[i]; out has n+1 entries
 void exscan(u32 const* a, u32 n, u32* out);
 Buf<u32> exscan(u32 const* a, u32 n);
+// as exscan, and returns the total out[n] to the host (one synchronisation)
+u32 exscan_total(u32 const* a, u32 n, u32* out);
 // reductions returning a host scalar
 u32 max_u32(u32 const* a, u32 n);
 u32 sum_u32(u32 const* a, u32 n);

@@ -3,6 +3,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <cstring>
 
 #include "ohx_common.hpp"
 
@@ -29,16 +30,62 @@ Runtime& rt() {
       throw Error("ohx: no HIP device visible (this library has no CPU path)");
     OHX_CHECK(hipGetDevice(&r.device));
     OHX_CHECK(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
-    OHX_CHECK(hipHostMalloc(&r.pinned, 4096, hipHostMallocDefault));
+    OHX_CHECK(hipHostMalloc(&r.pinned, 4096, hipHostMallocMapped));
+    std::memset(r.pinned, 0, 4096);
+    OHX_CHECK(hipHostGetDevicePointer(&r.pinned_dev, r.pinned, 0));
   }
   return r;
 }
 
-void sync() { OHX_CHECK(hipStreamSynchronize(stream())); }
+static char const* dev_error_text(unsigned e) {
+  switch (e) {
+    case ERR_REFLECT_NOT_FOUND: return "reflect_down: a sub-entity was not found (non-conforming mesh)";
+    case ERR_DUAL_NONMANIFOLD: return "dual_from_verts: non-manifold side";
+    case ERR_HINGE_SIDES: return "derive_model: more than two boundary sides at a hinge";
+    case ERR_REFINE_QUALITY: return "refine: a child element has non-positive quality (refine_qualities.cpp:72)";
+    case ERR_EXPOSED_SIDE: return "check_for_exposed_side: edge vertex not in entity";
+    case ERR_NO_DESTINATION: return "vertex_dest: no collapse destination (collapses_to_verts.cpp:100)";
+    case ERR_EDGE_RING: return "find_edge_ring: ring does not close or has fewer than 3 tets";
+    case ERR_SCAN_SPIN: return "scan: look-back did not complete (spin limit)";
+  }
+  return "unknown device error";
+}
+
+void sync() {
+  OHX_CHECK(hipStreamSynchronize(stream()));
+  volatile unsigned* e = host_slot<unsigned>(SLOT_ERROR);
+  if (*e) {
+    unsigned code = *e;
+    *e = 0;
+    fail(dev_error_text(code), __FILE__, __LINE__);
+  }
+}
 
 void check_launch(char const* name) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) fail(name, __FILE__, __LINE__);
+}
+
+namespace {
+struct Ptrs16 {
+  unsigned const* p[16];
+};
+__global__ void gather_scalars(Ptrs16 q, unsigned cnt, unsigned* o) {
+  unsigned i = threadIdx.x;
+  if (i < cnt) o[2 * i] = q.p[i] ? *q.p[i] : 0u;
+}
+}  // namespace
+
+void read_many(unsigned const* const* ptrs, unsigned n, unsigned* out) {
+  OHX_REQUIRE(n <= 16, "read_many: at most 16 scalars");
+  if (!n) return;
+  Ptrs16 pk;
+  for (unsigned i = 0; i < 16; ++i) pk.p[i] = i < n ? ptrs[i] : nullptr;
+  hipLaunchKernelGGL(gather_scalars, dim3(1), dim3(64), 0, stream(), pk, n, dev_slot<unsigned>(SLOT_GATHER));
+  check_launch("read_many");
+  sync();
+  unsigned* h = host_slot<unsigned>(SLOT_GATHER);
+  for (unsigned i = 0; i < n; ++i) out[i] = h[2 * i];
 }
 
 // ------------------------------------------------------------------ arena
