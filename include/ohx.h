@@ -73,6 +73,17 @@ unsigned ohx_find_indset(unsigned n, unsigned const* offsets, unsigned const* ad
 unsigned ohx_fp64_selftest(unsigned n, double const* a, double const* b,
     double* sqrt_out, double* div_out, void* stream) OSH_PUBLIC;
 
+/* Anisotropic extension (no counterpart in the reference, whose
+ * size_from_hessian.cpp:12-40 builds only a scalar size): FP64 cyclic Jacobi
+ * eigendecomposition of symmetric 3x3 tensors (6 components m00 m01 m02 m11
+ * m12 m22 each): lam (3 per tensor), V (3x3 row-major per tensor, columns
+ * are eigenvectors) ... */
+unsigned ohx_eigen3_f64(unsigned n, double const* h6, double* lam, double* V9,
+    void* stream) OSH_PUBLIC;
+/* ... and the metric M = V diag(clamp(|lam|/eps, 1/hmax^2, 1/hmin^2)) V^T */
+unsigned ohx_metric_from_hessian_f64(unsigned n, double const* h6, double eps,
+    double hmin, double hmax, double* m6, void* stream) OSH_PUBLIC;
+
 /* ----------------------------------------------------------- mesh seam */
 typedef struct ohx_mesh ohx_mesh;
 
@@ -108,6 +119,11 @@ void ohx_mesh_add_field(ohx_mesh* m, unsigned dim, char const* name,
 void ohx_mesh_free_field(ohx_mesh* m, unsigned dim, char const* name) OSH_PUBLIC;
 /* uniform field on every vertex, set on the device */
 void ohx_mesh_set_uniform_field(ohx_mesh* m, char const* name, double value) OSH_PUBLIC;
+
+/* a vertex "metric" field (6 components) from a vertex Hessian field; a
+ * mesh with a "metric" field adapts in that metric instead of "adapt_size" */
+void ohx_mesh_metric_from_hessian(ohx_mesh* m, char const* hessian, double eps,
+    double hmin, double hmax, char const* out_name) OSH_PUBLIC;
 
 double ohx_mesh_min_quality(ohx_mesh* m) OSH_PUBLIC;
 void ohx_mesh_qualities(ohx_mesh* m, double* out) OSH_PUBLIC;

@@ -107,8 +107,8 @@ template <u32 DIM>
 __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     u32 ns, u32 const* __restrict__ vs, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ ve, double const* __restrict__ x,
-    double const* __restrict__ eq, double qual_floor, u32* __restrict__ cand, double* __restrict__ out,
-    u32* __restrict__ words, u32* __restrict__ err) {
+    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor, u32* __restrict__ cand,
+    double* __restrict__ out, u32* __restrict__ words, u32* __restrict__ err) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= ns || !cand[i]) return;
   constexpr u32 VPE = DIM + 1;
@@ -121,6 +121,18 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     s = 0; s += a.y; s += b.y; s /= 2; split.y = s;
     s = 0; s += a.z; s += b.z; s /= 2; split.z = s;
   }
+  // the new vertex's metric is the average its field would get on refinement
+  Sym msplit;
+  if (mt) {
+    Sym ma = ld6(mt + 6 * (size_t) vs[2 * i]), mb = ld6(mt + 6 * (size_t) vs[2 * i + 1]);
+    for (int k = 0; k < 6; ++k) {
+      double s = 0;
+      s += ma.m[k];
+      s += mb.m[k];
+      s /= 2;
+      msplit.m[k] = s;
+    }
+  }
   double minq = 1, old_minq = 1;
   u32 const* bo = opp(DIM, 0);
   for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
@@ -130,10 +142,16 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     for (u32 k = 0; k < 2; ++k) {
       u32 const* bv = canon(DIM, DIM - 1, 0, bo[evs[k]]);
       V3 p[VPE];
+      Sym pm[VPE];
 #pragma unroll
-      for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) ve[(size_t) e * VPE + bv[l]]);
+      for (u32 l = 0; l + 1 < VPE; ++l) {
+        u32 v = ve[(size_t) e * VPE + bv[l]];
+        p[l] = ld3(x + 3 * (size_t) v);
+        if (mt) pm[l] = ld6(mt + 6 * (size_t) v);
+      }
       p[VPE - 1] = split;
-      double q = entity_quality(DIM, p);
+      pm[VPE - 1] = msplit;
+      double q = mt ? metric_quality<DIM>(p, pm, false) : entity_quality(DIM, p);
       if (!(q > 0)) *err = ERR_REFINE_QUALITY;
       if (q < minq) minq = q;
     }
@@ -150,13 +168,11 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
 __global__ __launch_bounds__(BLOCK) void refine_candidate_f64(u32 n, u32 const* __restrict__ ev,
                                                               double const* __restrict__ x,
                                                               double const* __restrict__ h,
+                                                              double const* __restrict__ mt,
                                                               u32* __restrict__ cand, u32* __restrict__ words) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  u32 a = ev[2 * i], b = ev[2 * i + 1];
-  double len = sqrt(sqdist(ld3(x + 3 * (size_t) b), ld3(x + 3 * (size_t) a)));
-  double want = (h[a] + h[b]) / 2;
-  u32 c = (len / want) > 1.0 ? 1u : 0u;
+  u32 c = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt) > 1.0 ? 1u : 0u;
   cand[i] = c;
   if (c) words[W_ANY] = 1u;
 }
@@ -174,12 +190,12 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   ProfScope ps("refine_quality_f64", (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24);
   if (dim == 3)
     hipLaunchKernelGGL(refine_quality_f64<3>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
-                       u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), eq.get(), qual_floor,
-                       cand, out.get(), scratch_words(), dev_error_word());
+                       u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(),
+                       qual_floor, cand, out.get(), scratch_words(), dev_error_word());
   else
     hipLaunchKernelGGL(refine_quality_f64<2>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
-                       u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), eq.get(), qual_floor,
-                       cand, out.get(), scratch_words(), dev_error_word());
+                       u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(),
+                       qual_floor, cand, out.get(), scratch_words(), dev_error_word());
   check_launch("refine_quality_f64");
   return out;
 }
@@ -458,14 +474,14 @@ static u32 refine_common(Mesh& m, Buf<u32>& cand, double qual_floor, bool requir
 
 u32 refine_by_size(Mesh& m, double qual_floor) {
   u32 ne = count(m, 1);
-  Tag* st = find_tag(m, 0, "adapt_size");
-  OHX_REQUIRE(st && st->type == TagType::F64, "mesh has no adapt_size field");
+  double const* h = size_of(m);
+  double const* mt = metric_of(m);
   Buf<u32> cand(ne);
   zero_words(0, 1);
   if (ne) {
     ProfScope ps("refine_candidate_f64", (double) ne * 12 + (double) count(m, 0) * 32);
     hipLaunchKernelGGL(refine_candidate_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
-                       ask_down(m, 1, 0), coords(m), st->f.get(), cand.get(), scratch_words());
+                       ask_down(m, 1, 0), coords(m), h, mt, cand.get(), scratch_words());
     check_launch("refine_candidate_f64");
   }
   if (!read_word(W_ANY)) return 0;
@@ -546,8 +562,8 @@ template <u32 DIM>
 __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
     u32 ne, u32 const* __restrict__ ev, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ vel, double const* __restrict__ x,
-    double const* __restrict__ eq, double qual_floor, u32* __restrict__ codes, double* __restrict__ out,
-    u32* __restrict__ words) {
+    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor, u32* __restrict__ codes,
+    double* __restrict__ out, u32* __restrict__ words) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= ne) return;
   u32 c = codes[i];
@@ -559,7 +575,9 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
     double minq = 1, old_minq = 1;
     u32 col = ev[2 * i + j], gen = ev[2 * i + (1 - j)];
     V3 p[VPE];
+    Sym pm[VPE];
     p[VPE - 1] = ld3(x + 3 * (size_t) gen);
+    if (mt) pm[VPE - 1] = ld6(mt + 6 * (size_t) gen);
     for (u32 k = uo[col]; k < uo[col + 1]; ++k) {
       u32 e = ua[k];
       u32 const* v = vel + (size_t) e * VPE;
@@ -571,8 +589,11 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
       if (adj) continue;
       u32 const* bv = canon(DIM, DIM - 1, 0, bo[ud[k]]);
 #pragma unroll
-      for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
-      double q = element_quality(DIM, p);
+      for (u32 l = 0; l + 1 < VPE; ++l) {
+        p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
+        if (mt) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
+      }
+      double q = mt ? metric_quality<DIM>(p, pm, true) : element_quality(DIM, p);
       if (q < minq) minq = q;
     }
     if ((minq < qual_floor) || (eq && (minq <= old_minq)))
@@ -587,14 +608,13 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
 // coarsen_size_code (coarsen.cpp:13-21), fused with measure_edge
 __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const* __restrict__ ev,
                                                                double const* __restrict__ x,
-                                                               double const* __restrict__ h, double floor_,
+                                                               double const* __restrict__ h,
+                                                               double const* __restrict__ mt, double floor_,
                                                                u32* __restrict__ codes, u32* __restrict__ words) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  u32 a = ev[2 * i], b = ev[2 * i + 1];
-  double len = sqrt(sqdist(ld3(x + 3 * (size_t) b), ld3(x + 3 * (size_t) a)));
-  double want = (h[a] + h[b]) / 2;
-  u32 c = ((len / want) < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
+  double sz = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
+  u32 c = (sz < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
   codes[i] = c;
   if (c) words[W_ANY] = 1u;
 }
@@ -671,19 +691,19 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   u32 const* ev = ask_down(m, 1, 0);
   u32 const* vel = ask_down(m, dim, 0);
   Up const& ue = ask_up(m, 0, dim);
-  if (require_better) eq = element_qualities(dim, count(m, dim), vel, coords(m));
+  if (require_better) eq = element_qualities(dim, count(m, dim), vel, coords(m), count(m, 0), metric_of(m));
   Buf<double> cq((size_t) 2 * ne);
   zero_words(0, 1);
   {
     ProfScope ps("coarsen_quality_f64", (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28);
     if (dim == 3)
       hipLaunchKernelGGL(coarsen_quality_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, ev,
-                         ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), eq.get(), qual_floor,
-                         codes.get(), cq.get(), scratch_words());
+                         ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), metric_of(m), eq.get(),
+                         qual_floor, codes.get(), cq.get(), scratch_words());
     else
       hipLaunchKernelGGL(coarsen_quality_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, ev,
-                         ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), eq.get(), qual_floor,
-                         codes.get(), cq.get(), scratch_words());
+                         ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), metric_of(m), eq.get(),
+                         qual_floor, codes.get(), cq.get(), scratch_words());
     check_launch("coarsen_quality_f64");
   }
   eq.release();
@@ -889,14 +909,14 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
 
 u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
   u32 ne = count(m, 1);
-  Tag* st = find_tag(m, 0, "adapt_size");
-  OHX_REQUIRE(st && st->type == TagType::F64, "mesh has no adapt_size field");
+  double const* h = size_of(m);
+  double const* mt = metric_of(m);
   Buf<u32> codes(ne);
   zero_words(0, 1);
   if (ne) {
     ProfScope ps("coarsen_size_code_f64", (double) ne * 12 + (double) count(m, 0) * 32);
     hipLaunchKernelGGL(coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
-                       ask_down(m, 1, 0), coords(m), st->f.get(), size_floor, codes.get(), scratch_words());
+                       ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words());
     check_launch("coarsen_size_code_f64");
   }
   if (!read_word(W_ANY)) return 0;
@@ -1076,8 +1096,10 @@ __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 cons
   return rs;
 }
 
-// choose_edge_swap (edge_swap.cpp:230-280)
-__device__ inline u32 choose_edge_swap(u32 rs, V3 const* ex, V3 const* rx, double* quality) {
+// choose_edge_swap (edge_swap.cpp:230-280); with vertex metrics (em, rm
+// non-null) each tet is measured in the average metric of its corners
+__device__ inline u32 choose_edge_swap(u32 rs, V3 const* ex, V3 const* rx, double* quality,
+                                       Sym const* em = nullptr, Sym const* rm = nullptr) {
   u32 tpm = k_sw_sizes[rs], nm = k_sw_counts[rs];
   unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs];
   unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
@@ -1090,12 +1112,24 @@ __device__ inline u32 choose_edge_swap(u32 rs, V3 const* ex, V3 const* rx, doubl
     for (u32 j = 0; j < tpm; ++j) {
       u32 tri = mesh[j];
       if (!((cached >> tri) & 1ull)) {
-        V3 t0 = rx[tris[tri][0]], t1 = rx[tris[tri][1]], t2 = rx[tris[tri][2]];
-        double tmin = 1;
-        double q = tet_quality(t0, t1, t2, ex[1]);
-        if (q < tmin) tmin = q;
-        q = tet_quality(t1, t0, t2, ex[0]);
-        if (q < tmin) tmin = q;
+        u32 a0 = tris[tri][0], a1 = tris[tri][1], a2 = tris[tri][2];
+        V3 t0 = rx[a0], t1 = rx[a1], t2 = rx[a2];
+        double tmin = 1, q;
+        if (em) {
+          V3 pa[4] = {t0, t1, t2, ex[1]};
+          Sym ma[4] = {rm[a0], rm[a1], rm[a2], em[1]};
+          q = metric_quality<3>(pa, ma, true);
+          if (q < tmin) tmin = q;
+          V3 pb[4] = {t1, t0, t2, ex[0]};
+          Sym mb[4] = {rm[a1], rm[a0], rm[a2], em[0]};
+          q = metric_quality<3>(pb, mb, true);
+          if (q < tmin) tmin = q;
+        } else {
+          q = tet_quality(t0, t1, t2, ex[1]);
+          if (q < tmin) tmin = q;
+          q = tet_quality(t1, t0, t2, ex[0]);
+          if (q < tmin) tmin = q;
+        }
         cached |= (1ull << tri);
         cache[tri] = tmin;
       }
@@ -1117,8 +1151,8 @@ __device__ inline u32 choose_edge_swap(u32 rs, V3 const* ex, V3 const* rx, doubl
 __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
     u32 ne, u32 const* __restrict__ uo, u32 const* __restrict__ ua, u32 const* __restrict__ ud,
     u32 const* __restrict__ ev, u32 const* __restrict__ vt, double const* __restrict__ x,
-    double const* __restrict__ eq, u32* __restrict__ cand, double* __restrict__ out, u32* __restrict__ rsz,
-    u32* __restrict__ words, u32* __restrict__ err) {
+    double const* __restrict__ mt, double const* __restrict__ eq, u32* __restrict__ cand,
+    double* __restrict__ out, u32* __restrict__ rsz, u32* __restrict__ words, u32* __restrict__ err) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= ne || !cand[i]) return;
   double old_minq = 1;
@@ -1141,7 +1175,14 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
   V3 rx[MAX_EDGE_SWAP];
   for (u32 j = 0; j < rs; ++j) rx[j] = ld3(x + 3 * (size_t) ring_v[j]);
   double sq;
-  choose_edge_swap(rs, ex, rx, &sq);
+  if (mt) {
+    Sym em[2] = {ld6(mt + 6 * (size_t) edge_v[0]), ld6(mt + 6 * (size_t) edge_v[1])};
+    Sym rm[MAX_EDGE_SWAP];
+    for (u32 j = 0; j < rs; ++j) rm[j] = ld6(mt + 6 * (size_t) ring_v[j]);
+    choose_edge_swap(rs, ex, rx, &sq, em, rm);
+  } else {
+    choose_edge_swap(rs, ex, rx, &sq);
+  }
   if (sq > old_minq) {
     out[i] = sq;
     rsz[i] = rs;
@@ -1161,7 +1202,8 @@ static inline __host__ __device__ u32 count_swap_ents(u32 rs, u32 d, u32 nint, u
 __global__ __launch_bounds__(BLOCK) void swap_cavity_f64(
     u32 ne, u32 d, u32 const* __restrict__ indset, u32 const* __restrict__ goff, u32 const* __restrict__ uo,
     u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
-    u32 const* __restrict__ vt, double const* __restrict__ x, u32* __restrict__ out) {
+    u32 const* __restrict__ vt, double const* __restrict__ x, double const* __restrict__ mt,
+    u32* __restrict__ out) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= ne || !indset[i]) return;
   u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
@@ -1171,7 +1213,15 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_f64(
   V3 rx[MAX_EDGE_SWAP];
   for (u32 j = 0; j < rs; ++j) rx[j] = ld3(x + 3 * (size_t) ring_v[j]);
   double sq;
-  u32 code = choose_edge_swap(rs, ex, rx, &sq);
+  u32 code;
+  if (mt) {
+    Sym em[2] = {ld6(mt + 6 * (size_t) edge_v[0]), ld6(mt + 6 * (size_t) edge_v[1])};
+    Sym rm[MAX_EDGE_SWAP];
+    for (u32 j = 0; j < rs; ++j) rm[j] = ld6(mt + 6 * (size_t) ring_v[j]);
+    code = choose_edge_swap(rs, ex, rx, &sq, em, rm);
+  } else {
+    code = choose_edge_swap(rs, ex, rx, &sq);
+  }
   u32 tpm = k_sw_sizes[rs];
   unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs] + code * tpm;
   unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
@@ -1277,8 +1327,8 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   {
     ProfScope ps("swap_quality_f64", (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24);
     hipLaunchKernelGGL(swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, u.offsets.get(),
-                       u.adj.get(), u.dirs.get(), ev, vt, coords(m), eq.get(), cand.get(), q.get(), rsz.get(),
-                       scratch_words(), dev_error_word());
+                       u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m), eq.get(), cand.get(), q.get(),
+                       rsz.get(), scratch_words(), dev_error_word());
     check_launch("swap_quality_f64");
   }
   eq.release();
@@ -1336,7 +1386,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
     {
       ProfScope ps("swap_cavity_f64", (double) ne * 12 + (double) ngen * vpe * 4);
       hipLaunchKernelGGL(swap_cavity_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, d, is, goff.get(),
-                         u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m),
+                         u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m),
                          vout.get() + (size_t) nsame * vpe);
       check_launch("swap_cavity_f64");
     }
@@ -1481,7 +1531,7 @@ u32 warp_to_limit(Mesh& m, double qual_floor) {
   };
   axpy(f);
   while (true) {
-    double mq = min_element_quality(dim, ne, ev, xp, nv);
+    double mq = min_element_quality(dim, ne, ev, xp, nv, metric_of(m));
     if (!(mq < qual_floor)) break;
     if (g_verbose) std::printf("quality %f exceeds limit\n", mq);
     hit = true;

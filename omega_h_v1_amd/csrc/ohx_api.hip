@@ -263,6 +263,30 @@ unsigned ohx_fp64_selftest(unsigned n, double const* a, double const* b, double*
   });
 }
 
+unsigned ohx_eigen3_f64(unsigned n, double const* h6, double* lam, double* V9, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    StreamScope sc((hipStream_t) s);
+    parallel_for(n, OHX_LAMBDA(u32 i) {
+      double V[3][3];
+      jacobi_eigen3(h6 + 6 * (size_t) i, lam + 3 * (size_t) i, V);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) V9[9 * (size_t) i + 3 * a + b] = V[a][b];
+    });
+    sync();
+    return 0u;
+  });
+}
+
+unsigned ohx_metric_from_hessian_f64(unsigned n, double const* h6, double eps, double hmin, double hmax,
+                                     double* m6, void* s) {
+  return guard(OHX_ERROR, [&]() -> unsigned {
+    StreamScope sc((hipStream_t) s);
+    parallel_for(n, OHX_LAMBDA(u32 i) { metric_from_hessian(h6 + 6 * (size_t) i, eps, hmin, hmax, m6 + 6 * (size_t) i); });
+    sync();
+    return 0u;
+  });
+}
+
 // ------------------------------------------------------------ mesh seam
 ohx_mesh* ohx_mesh_new(unsigned dim, unsigned nv, double const* x, unsigned ne, unsigned const* conn) {
   return guard((ohx_mesh*) nullptr, [&]() -> ohx_mesh* {
@@ -384,6 +408,21 @@ void ohx_mesh_set_uniform_field(ohx_mesh* m, char const* name, double v) {
   guard_void([&]() {
     if (find_tag(m->m, 0, name)) remove_tag(m->m, 0, name);
     add_tag_f64(m->m, 0, name, 1, filled_f64(count(m->m, 0), v));
+  });
+}
+
+void ohx_mesh_metric_from_hessian(ohx_mesh* m, char const* hname, double eps, double hmin, double hmax,
+                                  char const* out_name) {
+  guard_void([&]() {
+    Tag* t = find_tag(m->m, 0, hname);
+    OHX_REQUIRE(t && t->type == TagType::F64 && t->ncomps == 6, "hessian field must have 6 FP64 components");
+    u32 nv = count(m->m, 0);
+    Buf<double> out((size_t) 6 * nv);
+    double const* h = t->f.get();
+    double* o = out.get();
+    parallel_for(nv, OHX_LAMBDA(u32 i) { metric_from_hessian(h + 6 * (size_t) i, eps, hmin, hmax, o + 6 * (size_t) i); });
+    if (find_tag(m->m, 0, out_name)) remove_tag(m->m, 0, out_name);
+    add_tag_f64(m->m, 0, out_name, 6, std::move(out));
   });
 }
 

@@ -100,3 +100,185 @@ __host__ __device__ inline double entity_quality(u32 dim, V3 const* x) {
 }
 
 }  // namespace ohx
+
+namespace ohx {
+
+// ------------------------------------------------------------------------
+// Anisotropic metric extension (no counterpart in the reference, which has
+// only the scalar "adapt_size").  A vertex field "metric" holds a symmetric
+// positive definite 3x3 tensor as (m00, m01, m02, m11, m12, m22).  An
+// element's metric is the average of its vertices' (summed in vertex order,
+// then divided, like average_element_field); its quality is the reference's
+// quality formula evaluated on the edge vectors mapped by the Cholesky
+// factor U (U^T U = M), so it is the v1 measure in metric space; an edge's
+// metric length is sqrt(e^T M e) with M the average of its two vertices'.
+// FP64 throughout, fixed operation order (the oracle repeats it exactly).
+struct Sym {
+  double m[6];
+};
+
+__host__ __device__ inline Sym ld6(double const* p) {
+  Sym s;
+  for (int k = 0; k < 6; ++k) s.m[k] = p[k];
+  return s;
+}
+
+template <int N>
+__host__ __device__ inline Sym sym_average(Sym const* v) {
+  Sym s;
+  for (int k = 0; k < 6; ++k) {
+    double a = 0;
+    for (int i = 0; i < N; ++i) a += v[i].m[k];
+    a /= N;
+    s.m[k] = a;
+  }
+  return s;
+}
+
+// upper Cholesky factor: u00 u01 u02 / u11 u12 / u22
+struct Chol {
+  double u00, u01, u02, u11, u12, u22;
+};
+
+__host__ __device__ inline Chol cholesky(Sym const& a) {
+  Chol c;
+  c.u00 = sqrt(a.m[0]);
+  c.u01 = a.m[1] / c.u00;
+  c.u02 = a.m[2] / c.u00;
+  c.u11 = sqrt(a.m[3] - c.u01 * c.u01);
+  c.u12 = (a.m[4] - c.u01 * c.u02) / c.u11;
+  c.u22 = sqrt((a.m[5] - c.u02 * c.u02) - c.u12 * c.u12);
+  return c;
+}
+
+__host__ __device__ inline V3 chol_map(Chol const& c, V3 e) {
+  return V3{(c.u00 * e.x + c.u01 * e.y) + c.u02 * e.z, c.u11 * e.y + c.u12 * e.z, c.u22 * e.z};
+}
+
+// e^T M e for the average M of two vertex metrics
+__host__ __device__ inline double metric_len2(Sym const& a, Sym const& b, V3 e) {
+  double m[6];
+  for (int k = 0; k < 6; ++k) {
+    double s = 0;
+    s += a.m[k];
+    s += b.m[k];
+    s /= 2;
+    m[k] = s;
+  }
+  double s = 0;
+  s += e.x * ((m[0] * e.x + m[1] * e.y) + m[2] * e.z);
+  s += e.y * ((m[1] * e.x + m[3] * e.y) + m[4] * e.z);
+  s += e.z * ((m[2] * e.x + m[4] * e.y) + m[5] * e.z);
+  return s;
+}
+
+// quality of a simplex with per-vertex metrics (dim 2 or 3); signed_z picks
+// the signed z-area triangle form (element quality) over the unsigned one
+template <int DIM>
+__host__ __device__ inline double metric_quality(V3 const* x, Sym const* mv, bool signed_z) {
+  Sym me = sym_average<DIM + 1>(mv);
+  Chol c = cholesky(me);
+  V3 o{0.0, 0.0, 0.0};
+  V3 p1 = chol_map(c, sub(x[1], x[0]));
+  V3 p2 = chol_map(c, sub(x[2], x[0]));
+  if (DIM == 3) {
+    V3 p3 = chol_map(c, sub(x[3], x[0]));
+    return tet_quality(o, p1, p2, p3);
+  }
+  return tri_quality(o, p1, p2, signed_z);
+}
+
+}  // namespace ohx
+
+namespace ohx {
+
+// edge size for the adapt decisions: v1's |x1 - x0| / ((h0 + h1) / 2)
+// (size.cpp:57-69), or the metric length sqrt(e^T M e) when mt != null
+__host__ __device__ inline double edge_size(u32 a, u32 b, double const* x, double const* h, double const* mt) {
+  V3 xa = ld3(x + 3 * (size_t) a), xb = ld3(x + 3 * (size_t) b);
+  if (mt) return sqrt(metric_len2(ld6(mt + 6 * (size_t) a), ld6(mt + 6 * (size_t) b), sub(xb, xa)));
+  double len = sqrt(sqdist(xb, xa));
+  double want = (h[a] + h[b]) / 2;
+  return len / want;
+}
+
+}  // namespace ohx
+
+namespace ohx {
+
+// ------------------------------------------------------------------------
+// Metric from a Hessian (extension of size_from_hessian.cpp:12-40, which in
+// the reference builds only a scalar size): symmetric eigendecomposition
+// H = V diag(l) V^T by cyclic Jacobi rotations in FP64, eigenvalues mapped
+// to l' = clamp(|l| / eps, 1/hmax^2, 1/hmin^2), M = V diag(l') V^T.
+// Fixed operation order; the oracle repeats it exactly.
+__host__ __device__ inline void jacobi_eigen3(double const* h6, double* lam, double (*V)[3]) {
+  double A[3][3] = {{h6[0], h6[1], h6[2]}, {h6[1], h6[3], h6[4]}, {h6[2], h6[4], h6[5]}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+  int const P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    double off = 0;
+    off += A[0][1] * A[0][1];
+    off += A[0][2] * A[0][2];
+    off += A[1][2] * A[1][2];
+    double dia = 0;
+    dia += A[0][0] * A[0][0];
+    dia += A[1][1] * A[1][1];
+    dia += A[2][2] * A[2][2];
+    if (off <= 1e-40 * dia || off == 0.0) break;
+    for (int r = 0; r < 3; ++r) {
+      int p = P[r], q = Q[r];
+      double apq = A[p][q];
+      if (apq == 0.0) continue;
+      double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+      double at = theta < 0 ? -theta : theta;
+      double t = 1.0 / (at + sqrt(theta * theta + 1.0));
+      if (theta < 0) t = -t;
+      double c = 1.0 / sqrt(t * t + 1.0);
+      double s = t * c;
+      double app = A[p][p], aqq = A[q][q];
+      A[p][p] = app - t * apq;
+      A[q][q] = aqq + t * apq;
+      A[p][q] = 0.0;
+      A[q][p] = 0.0;
+      int k = 3 - p - q;
+      double akp = A[k][p], akq = A[k][q];
+      A[k][p] = c * akp - s * akq;
+      A[p][k] = A[k][p];
+      A[k][q] = s * akp + c * akq;
+      A[q][k] = A[k][q];
+      for (int i = 0; i < 3; ++i) {
+        double vip = V[i][p], viq = V[i][q];
+        V[i][p] = c * vip - s * viq;
+        V[i][q] = s * vip + c * viq;
+      }
+    }
+  }
+  lam[0] = A[0][0];
+  lam[1] = A[1][1];
+  lam[2] = A[2][2];
+}
+
+__host__ __device__ inline void metric_from_hessian(double const* h6, double eps, double hmin, double hmax,
+                                                    double* m6) {
+  double lam[3], V[3][3];
+  jacobi_eigen3(h6, lam, V);
+  double lo = 1.0 / (hmax * hmax), hi = 1.0 / (hmin * hmin);
+  double l[3];
+  for (int k = 0; k < 3; ++k) {
+    double a = lam[k] < 0 ? -lam[k] : lam[k];
+    a = a / eps;
+    if (a < lo) a = lo;
+    if (a > hi) a = hi;
+    l[k] = a;
+  }
+  int const I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+  for (int c = 0; c < 6; ++c) {
+    double s = 0;
+    for (int k = 0; k < 3; ++k) s += (V[I[c]][k] * l[k]) * V[J[c]][k];
+    m6[c] = s;
+  }
+}
+
+}  // namespace ohx

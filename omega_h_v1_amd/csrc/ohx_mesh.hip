@@ -85,6 +85,20 @@ double const* coords(Mesh& m) {
   return t->f.get();
 }
 
+double const* metric_of(Mesh& m) {
+  Tag* t = find_tag(m, 0, "metric");
+  if (!t) return nullptr;
+  OHX_REQUIRE(t->type == TagType::F64 && t->ncomps == 6, "metric field must have 6 FP64 components");
+  return t->f.get();
+}
+
+double const* size_of(Mesh& m) {
+  if (metric_of(m)) return nullptr;
+  Tag* st = find_tag(m, 0, "adapt_size");
+  OHX_REQUIRE(st && st->type == TagType::F64, "mesh has neither an adapt_size nor a metric field");
+  return st->f.get();
+}
+
 u32 const* u32_tag(Mesh& m, u32 d, char const* name) {
   Tag* t = find_tag(m, d, name);
   OHX_REQUIRE(t && t->type == TagType::U32, "missing integer tag");
@@ -94,16 +108,32 @@ u32 const* u32_tag(Mesh& m, u32 d, char const* name) {
 // ---------------------------------------------------------------- quality
 // One thread per element: gather the vertex coordinates, evaluate the
 // reference's quality formula in FP64 (quality.cpp:13-45).
+// element quality, isotropic (v1) or in the vertex metric when mt != null
+template <u32 DIM>
+__device__ inline double elem_q(u32 i, u32 const* __restrict__ ev, double const* __restrict__ x,
+                                double const* __restrict__ mt) {
+  V3 p[DIM + 1];
+  u32 v[DIM + 1];
+#pragma unroll
+  for (u32 j = 0; j <= DIM; ++j) {
+    v[j] = ev[(size_t) i * (DIM + 1) + j];
+    p[j] = ld3(x + 3 * (size_t) v[j]);
+  }
+  if (!mt) return element_quality(DIM, p);
+  Sym m[DIM + 1];
+#pragma unroll
+  for (u32 j = 0; j <= DIM; ++j) m[j] = ld6(mt + 6 * (size_t) v[j]);
+  return metric_quality<DIM>(p, m, true);
+}
+
 template <u32 DIM>
 __global__ __launch_bounds__(BLOCK) void elem_quality_f64(u32 n, u32 const* __restrict__ ev,
                                                           double const* __restrict__ x,
+                                                          double const* __restrict__ mt,
                                                           double* __restrict__ out) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  V3 p[DIM + 1];
-#pragma unroll
-  for (u32 j = 0; j <= DIM; ++j) p[j] = ld3(x + 3 * (size_t) ev[(size_t) i * (DIM + 1) + j]);
-  out[i] = element_quality(DIM, p);
+  out[i] = elem_q<DIM>(i, ev, x, mt);
 }
 
 // Fused quality + minimum over elements begin..n-1 (no per-element store).
@@ -111,14 +141,12 @@ template <u32 DIM>
 __global__ __launch_bounds__(BLOCK) void elem_min_quality_f64(u32 n, u32 begin,
                                                               u32 const* __restrict__ ev,
                                                               double const* __restrict__ x,
+                                                              double const* __restrict__ mt,
                                                               double* __restrict__ part) {
   __shared__ double lds[BLOCK / 64];
   double v = 1.7976931348623157e308;
   for (u32 i = begin + blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-    V3 p[DIM + 1];
-#pragma unroll
-    for (u32 j = 0; j <= DIM; ++j) p[j] = ld3(x + 3 * (size_t) ev[(size_t) i * (DIM + 1) + j]);
-    double q = element_quality(DIM, p);
+    double q = elem_q<DIM>(i, ev, x, mt);
     if (q < v) v = q;
   }
 #pragma unroll
@@ -136,66 +164,64 @@ __global__ __launch_bounds__(BLOCK) void elem_min_quality_f64(u32 n, u32 begin,
   }
 }
 
-Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x, u32 nverts) {
+Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x, u32 nverts, double const* mt) {
   Buf<double> q(n);
   if (!n) return q;
   // compulsory bytes: connectivity, each vertex's coordinates once, output
   ProfScope ps("elem_quality_f64", (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) n * 8);
   if (dim == 3)
-    hipLaunchKernelGGL(elem_quality_f64<3>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, q.get());
+    hipLaunchKernelGGL(elem_quality_f64<3>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, mt, q.get());
   else
-    hipLaunchKernelGGL(elem_quality_f64<2>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, q.get());
+    hipLaunchKernelGGL(elem_quality_f64<2>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, mt, q.get());
   check_launch("elem_quality_f64");
   return q;
 }
 
 Buf<double> mesh_qualities(Mesh& m) {
-  return element_qualities(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m), count(m, 0));
+  return element_qualities(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m), count(m, 0), metric_of(m));
 }
 
 // min over elements 1..n-1: the serial build's doubles_min starts at 1
 // (doubles.cpp:111-118), and mesh_min_quality goes through it
-double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x, u32 nverts) {
+double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x, u32 nverts, double const* mt) {
   if (n <= 1) return 1.7976931348623157e308;
   u32 nb = std::min<u32>(nblocks(n - 1), 1024);
   Buf<double> part(nb);
   {
     ProfScope ps("elem_min_quality_f64", (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) nb * 8);
     if (dim == 3)
-      hipLaunchKernelGGL(elem_min_quality_f64<3>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, part.get());
+      hipLaunchKernelGGL(elem_min_quality_f64<3>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, mt, part.get());
     else
-      hipLaunchKernelGGL(elem_min_quality_f64<2>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, part.get());
+      hipLaunchKernelGGL(elem_min_quality_f64<2>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, mt, part.get());
     check_launch("elem_min_quality_f64");
   }
   return min_f64(part.get(), nb, 0);
 }
 
 double mesh_min_quality(Mesh& m) {
-  return min_element_quality(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m), count(m, 0));
+  return min_element_quality(m.dim, count(m, m.dim), ask_down(m, m.dim, 0), coords(m), count(m, 0), metric_of(m));
 }
 
 // measure_edge (size.cpp:57-88): |x1 - x0| / ((h0 + h1) / 2)
 __global__ __launch_bounds__(BLOCK) void measure_edge_f64(u32 n, u32 const* __restrict__ ev,
                                                           double const* __restrict__ x,
                                                           double const* __restrict__ h,
+                                                          double const* __restrict__ mt,
                                                           double* __restrict__ out) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  u32 a = ev[2 * i], b = ev[2 * i + 1];
-  double len = sqrt(sqdist(ld3(x + 3 * (size_t) b), ld3(x + 3 * (size_t) a)));
-  double want = (h[a] + h[b]) / 2;
-  out[i] = len / want;
+  out[i] = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
 }
 
 Buf<double> measure_edges(Mesh& m) {
   u32 ne = count(m, 1);
-  Tag* st = find_tag(m, 0, "adapt_size");
-  OHX_REQUIRE(st && st->type == TagType::F64, "mesh has no adapt_size field");
+  double const* h = size_of(m);
+  double const* mt = metric_of(m);
   Buf<double> out(ne);
   if (ne) {
-    ProfScope ps("measure_edge_f64", (double) ne * 16 + (double) count(m, 0) * 32);
+    ProfScope ps("measure_edge_f64", (double) ne * 16 + (double) count(m, 0) * (mt ? 72 : 32));
     hipLaunchKernelGGL(measure_edge_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
-                       ask_down(m, 1, 0), coords(m), st->f.get(), out.get());
+                       ask_down(m, 1, 0), coords(m), h, mt, out.get());
     check_launch("measure_edge_f64");
   }
   return out;
@@ -762,14 +788,12 @@ void unmark_boundary(Mesh& m, u32 ent_dim, u32* marked) {
 // mesh_mark_slivers (mark.cpp:202-232): quality fused with the threshold
 template <u32 DIM>
 __global__ __launch_bounds__(BLOCK) void mark_sliver_f64(u32 n, u32 const* __restrict__ ev,
-                                                         double const* __restrict__ x, double good,
+                                                         double const* __restrict__ x,
+                                                         double const* __restrict__ mt, double good,
                                                          u32* __restrict__ out) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
-  V3 p[DIM + 1];
-#pragma unroll
-  for (u32 j = 0; j <= DIM; ++j) p[j] = ld3(x + 3 * (size_t) ev[(size_t) i * (DIM + 1) + j]);
-  out[i] = (element_quality(DIM, p) < good) ? 1u : 0u;
+  out[i] = (elem_q<DIM>(i, ev, x, mt) < good) ? 1u : 0u;
 }
 
 Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers) {
@@ -779,10 +803,10 @@ Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers) {
     ProfScope ps("mark_sliver_f64", (double) ne * (m.dim + 1) * 4 + (double) count(m, 0) * 24 + (double) ne * 4);
     if (m.dim == 3)
       hipLaunchKernelGGL(mark_sliver_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
-                         ask_down(m, 3, 0), coords(m), good_qual, s.get());
+                         ask_down(m, 3, 0), coords(m), metric_of(m), good_qual, s.get());
     else
       hipLaunchKernelGGL(mark_sliver_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
-                         ask_down(m, 2, 0), coords(m), good_qual, s.get());
+                         ask_down(m, 2, 0), coords(m), metric_of(m), good_qual, s.get());
     check_launch("mark_sliver_f64");
   }
   mark_dual_layers(m, s, nlayers);

@@ -64,13 +64,20 @@ Tag& add_tag_u32(Mesh& m, u32 d, char const* name, u32 ncomps, Buf<u32>&& data, 
 Tag& add_tag_f64(Mesh& m, u32 d, char const* name, u32 ncomps, Buf<double>&& data, int transfer = 0);
 void remove_tag(Mesh& m, u32 d, char const* name);
 double const* coords(Mesh& m);
+// the anisotropic vertex metric (6 components), or null for the isotropic
+// "adapt_size" field of the reference
+double const* metric_of(Mesh& m);
+// "adapt_size", or null when the mesh carries a metric
+double const* size_of(Mesh& m);
 u32 const* u32_tag(Mesh& m, u32 d, char const* name);
 
 // kernels on a mesh
 // nverts only feeds the traffic accounting of the kernel timers
-Buf<double> element_qualities(u32 dim, u32 n, u32 const* verts_of_elems, double const* x, u32 nverts = 0);
+Buf<double> element_qualities(u32 dim, u32 n, u32 const* verts_of_elems, double const* x, u32 nverts = 0,
+                              double const* metric = nullptr);
 Buf<double> mesh_qualities(Mesh& m);
-double min_element_quality(u32 dim, u32 n, u32 const* verts_of_elems, double const* x, u32 nverts = 0);
+double min_element_quality(u32 dim, u32 n, u32 const* verts_of_elems, double const* x, u32 nverts = 0,
+                           double const* metric = nullptr);
 double mesh_min_quality(Mesh& m);
 Buf<double> measure_edges(Mesh& m);
 

@@ -40,3 +40,34 @@ def uniform_size(nverts, h):
 
 
 PI_REF = 3.14159265359  # algebra.hpp:8
+
+
+def tanh_front_hessian(coords, a=50.0, s0=1.5, dim=3):
+    """Hessian of f = tanh(a (x + y [+ z] - s0)): f''(s) times the all-ones
+    matrix over the active coordinates, as 6 components per vertex."""
+    s = coords[:, 0] + coords[:, 1] + (coords[:, 2] if dim == 3 else 0.0) - s0
+    th = np.tanh(a * s)
+    f2 = -2.0 * a * a * th * (1.0 - th * th)
+    h = np.zeros((len(coords), 6))
+    if dim == 3:
+        h[:] = f2[:, None]  # all second partials are f''
+    else:
+        h[:, 0] = f2  # xx
+        h[:, 1] = f2  # xy
+        h[:, 3] = f2  # yy
+    return h
+
+
+def rotating_metric(coords, h_short, aspect, dim=3):
+    """Metric whose short axis (size h_short) lies in the xy plane at angle
+    2 pi z, long axes aspect * h_short (6 components per vertex)."""
+    ang = 2.0 * np.pi * coords[:, 2] if dim == 3 else 2.0 * np.pi * coords[:, 0]
+    c, s = np.cos(ang), np.sin(ang)
+    l1 = 1.0 / (h_short * h_short)
+    l2 = 1.0 / ((aspect * h_short) ** 2)
+    m = np.zeros((len(coords), 6))
+    m[:, 0] = l1 * c * c + l2 * s * s
+    m[:, 1] = (l1 - l2) * c * s
+    m[:, 3] = l1 * s * s + l2 * c * c
+    m[:, 5] = l2
+    return m

@@ -544,20 +544,108 @@ static double element_quality(u32 dim, double const (*x)[3])
   return dim == 3 ? tet_quality(x) : tri_quality_gen(x, 1);
 }
 
+/* ---------------------------------------------- anisotropic extension
+ * Not in the reference (which has only the scalar "adapt_size").  A vertex
+ * field "metric" holds an SPD 3x3 tensor (m00, m01, m02, m11, m12, m22); an
+ * element's metric is its vertices' average (summed in vertex order, then
+ * divided); its quality is the reference's formula on the edge vectors
+ * mapped by the upper Cholesky factor U (U^T U = M); an edge's metric length
+ * is sqrt(e^T M e) with M the average of its two vertices'.  Same operation
+ * order as the device code. */
+static double const* metric_of(orc_mesh* m)
+{
+  otag* t = find_tag(m, 0, "metric");
+  if (!t)
+    return 0;
+  if (t->type != T_F64 || t->ncomps != 6)
+    die("metric field must have 6 FP64 components");
+  return (double const*) t->data;
+}
+
+static void sym_average(double const (*mv)[6], u32 n, double* out)
+{
+  for (u32 k = 0; k < 6; ++k) {
+    double a = 0;
+    for (u32 i = 0; i < n; ++i)
+      a += mv[i][k];
+    a /= n;
+    out[k] = a;
+  }
+}
+
+static void chol_map(double const* mm, double const* e, double* y)
+{
+  double u00 = sqrt(mm[0]);
+  double u01 = mm[1] / u00;
+  double u02 = mm[2] / u00;
+  double u11 = sqrt(mm[3] - u01 * u01);
+  double u12 = (mm[4] - u01 * u02) / u11;
+  double u22 = sqrt((mm[5] - u02 * u02) - u12 * u12);
+  y[0] = (u00 * e[0] + u01 * e[1]) + u02 * e[2];
+  y[1] = u11 * e[1] + u12 * e[2];
+  y[2] = u22 * e[2];
+}
+
+static double metric_quality(u32 dim, double const (*x)[3], double const (*mv)[6], int signed_z)
+{
+  double me[6];
+  sym_average(mv, dim + 1, me);
+  double p[4][3] = {{0, 0, 0}};
+  for (u32 i = 1; i <= dim; ++i) {
+    double e[3];
+    sub3(x[i], x[0], e);
+    chol_map(me, e, p[i]);
+  }
+  if (dim == 3)
+    return tet_quality((double const (*)[3]) p);
+  return tri_quality_gen((double const (*)[3]) p, signed_z);
+}
+
+static double metric_len2(double const* a, double const* b, double const* e)
+{
+  double mm[6];
+  for (u32 k = 0; k < 6; ++k) {
+    double s = 0;
+    s += a[k];
+    s += b[k];
+    s /= 2;
+    mm[k] = s;
+  }
+  double s = 0;
+  s += e[0] * ((mm[0] * e[0] + mm[1] * e[1]) + mm[2] * e[2]);
+  s += e[1] * ((mm[1] * e[0] + mm[3] * e[1]) + mm[4] * e[2]);
+  s += e[2] * ((mm[2] * e[0] + mm[4] * e[1]) + mm[5] * e[2]);
+  return s;
+}
+
+/* quality of an element given its corner coordinates and, if mt, corner
+ * metrics */
+static double corner_quality(u32 dim, double const (*x)[3], double const (*mv)[6],
+    double const* mt, int signed_z)
+{
+  if (mt)
+    return metric_quality(dim, x, mv, signed_z);
+  return signed_z ? element_quality(dim, x) : entity_quality(dim, x);
+}
+
 double orc_tet_quality(double const* x12) { return tet_quality((double const (*)[3]) x12); }
 double orc_triangle_quality(double const* x9) { return tri_quality_gen((double const (*)[3]) x9, 0); }
 double orc_triangle_xy_quality(double const* x9) { return tri_quality_gen((double const (*)[3]) x9, 1); }
 
 /* element_qualities (quality.cpp:13-45) */
-static double* element_qualities(u32 dim, u32 n, u32 const* ev, double const* x)
+static double* element_qualities(u32 dim, u32 n, u32 const* ev, double const* x,
+    double const* mt)
 {
   double* q = NEW(double, n);
   u32 vpe = dim + 1;
   for (u32 i = 0; i < n; ++i) {
-    double ex[4][3];
-    for (u32 j = 0; j < vpe; ++j)
+    double ex[4][3], em[4][6];
+    for (u32 j = 0; j < vpe; ++j) {
       memcpy(ex[j], x + 3 * ev[i * vpe + j], sizeof(double) * 3);
-    q[i] = element_quality(dim, (double const (*)[3]) ex);
+      if (mt)
+        memcpy(em[j], mt + 6 * ev[i * vpe + j], sizeof(double) * 6);
+    }
+    q[i] = corner_quality(dim, (double const (*)[3]) ex, (double const (*)[6]) em, mt, 1);
   }
   return q;
 }
@@ -565,7 +653,7 @@ static double* element_qualities(u32 dim, u32 n, u32 const* ev, double const* x)
 static double* mesh_qualities(orc_mesh* m)
 {
   return element_qualities(m->dim, mesh_count(m, m->dim), ask_down(m, m->dim, 0),
-      coords_of(m));
+      coords_of(m), metric_of(m));
 }
 
 static double mesh_min_quality(orc_mesh* m) /* quality.cpp:67-73 */
@@ -583,13 +671,23 @@ static double* measure_edges(orc_mesh* m)
   u32 ne = mesh_count(m, 1);
   u32 const* ev = ask_down(m, 1, 0);
   double const* x = coords_of(m);
-  otag* st = find_tag(m, 0, "adapt_size");
-  if (!st)
-    die("no adapt_size");
-  double const* h = (double const*) st->data;
+  double const* mt = metric_of(m);
+  double const* h = 0;
+  if (!mt) {
+    otag* st = find_tag(m, 0, "adapt_size");
+    if (!st)
+      die("no adapt_size and no metric");
+    h = (double const*) st->data;
+  }
   double* out = NEW(double, ne);
   for (u32 i = 0; i < ne; ++i) {
     u32 a = ev[i * 2 + 0], b = ev[i * 2 + 1];
+    if (mt) {
+      double e[3];
+      sub3(x + 3 * b, x + 3 * a, e);
+      out[i] = sqrt(metric_len2(mt + 6 * a, mt + 6 * b, e));
+      continue;
+    }
     double len = sqrt(vsqdist(x + 3 * b, x + 3 * a));
     double want = (h[a] + h[b]) / 2;
     out[i] = len / want;
@@ -1526,6 +1624,7 @@ static double* refine_qualities(orc_mesh* m, u32 src_dim, u32* cands,
   oup const* u = ask_up(m, src_dim, dim);
   double const* x = coords_of(m);
   double* eq = require_better ? mesh_qualities(m) : 0;
+  double const* mt = metric_of(m);
   u32 base_dim = dim - 1;
   u32 vps = down_deg[src_dim][0], vpe = dim + 1;
   u32 const* bo = opp(dim, 0);
@@ -1540,6 +1639,14 @@ static double* refine_qualities(orc_mesh* m, u32 src_dim, u32* cands,
         split[c] += x[vs[i * vps + k] * 3 + c];
     for (u32 c = 0; c < 3; ++c)
       split[c] /= vps;
+    double msplit[6] = {0, 0, 0, 0, 0, 0};
+    if (mt) {
+      for (u32 k = 0; k < vps; ++k)
+        for (u32 c = 0; c < 6; ++c)
+          msplit[c] += mt[vs[i * vps + k] * 6 + c];
+      for (u32 c = 0; c < 6; ++c)
+        msplit[c] /= vps;
+    }
     double minq = 1, old_minq = 1;
     for (u32 j = u->offsets[i]; j < u->offsets[i + 1]; ++j) {
       u32 dir = u->dirs[j], e = u->adj[j];
@@ -1549,11 +1656,15 @@ static double* refine_qualities(orc_mesh* m, u32 src_dim, u32* cands,
       for (u32 k = 0; k < vps; ++k) {
         u32 base = bo[evs[k]];
         u32 const* bv = CANON(dim, base_dim, 0, base);
-        double ex[4][3];
-        for (u32 l = 0; l + 1 < vpe; ++l)
+        double ex[4][3], em[4][6];
+        for (u32 l = 0; l + 1 < vpe; ++l) {
           memcpy(ex[l], x + 3 * ve[e * vpe + bv[l]], sizeof(double) * 3);
+          if (mt)
+            memcpy(em[l], mt + 6 * ve[e * vpe + bv[l]], sizeof(double) * 6);
+        }
         memcpy(ex[vpe - 1], split, sizeof(double) * 3);
-        double q = entity_quality(dim, (double const (*)[3]) ex);
+        memcpy(em[vpe - 1], msplit, sizeof(double) * 6);
+        double q = corner_quality(dim, (double const (*)[3]) ex, (double const (*)[6]) em, mt, 0);
         if (!(q > 0))
           die("refine_quality: non-positive child quality");
         if (q < minq)
@@ -1873,7 +1984,8 @@ static double* coarsen_qualities(orc_mesh* m, u32* codes, double qual_floor,
   u32 const* vel = ask_down(m, dim, 0);
   oup const* u = ask_up(m, 0, dim);
   double const* x = coords_of(m);
-  double* eq = require_better ? element_qualities(dim, mesh_count(m, dim), vel, x) : 0;
+  double const* mt = metric_of(m);
+  double* eq = require_better ? element_qualities(dim, mesh_count(m, dim), vel, x, mt) : 0;
   u32 vpe = dim + 1;
   u32 const* bo = opp(dim, 0);
   double* out = NEW(double, ne * 2);
@@ -1887,8 +1999,10 @@ static double* coarsen_qualities(orc_mesh* m, u32* codes, double qual_floor,
         continue;
       double minq = 1, old_minq = 1;
       u32 col = ev[i * 2 + j], gen = ev[i * 2 + (1 - j)];
-      double ex[4][3];
+      double ex[4][3], em[4][6];
       memcpy(ex[vpe - 1], x + 3 * gen, sizeof(double) * 3);
+      if (mt)
+        memcpy(em[vpe - 1], mt + 6 * gen, sizeof(double) * 6);
       for (u32 k = u->offsets[col]; k < u->offsets[col + 1]; ++k) {
         u32 e = u->adj[k];
         u32 const* ve = vel + e * vpe;
@@ -1901,9 +2015,12 @@ static double* coarsen_qualities(orc_mesh* m, u32* codes, double qual_floor,
         if (adj)
           continue;
         u32 const* bv = CANON(dim, dim - 1, 0, bo[u->dirs[k]]);
-        for (u32 l = 0; l + 1 < vpe; ++l)
+        for (u32 l = 0; l + 1 < vpe; ++l) {
           memcpy(ex[l], x + 3 * ve[bv[l]], sizeof(double) * 3);
-        double q = element_quality(dim, (double const (*)[3]) ex);
+          if (mt)
+            memcpy(em[l], mt + 6 * ve[bv[l]], sizeof(double) * 6);
+        }
+        double q = corner_quality(dim, (double const (*)[3]) ex, (double const (*)[6]) em, mt, 1);
         if (q < minq)
           minq = q;
       }
@@ -2290,7 +2407,7 @@ void orc_swap_int_edges(unsigned ring_size, unsigned code, unsigned* out)
  * ring; each triangle makes two tets with the edge endpoints; worst tet
  * quality per triangle cached; strict '>' keeps the first best code */
 static u32 choose_edge_swap(u32 rs, double const (*ex)[3], double const (*rx)[3],
-    double* quality)
+    double* quality, double const (*em)[6], double const (*rm)[6])
 {
   u32 tpm = swap_mesh_sizes[rs], nm = swap_mesh_counts[rs];
   u32 const* mesh = swap_meshes[rs];
@@ -2305,19 +2422,30 @@ static u32 choose_edge_swap(u32 rs, double const (*ex)[3], double const (*rx)[3]
     for (u32 j = 0; j < tpm; ++j) {
       u32 tri = mesh[j];
       if (!cached[tri]) {
-        double tx[4][3];
-        for (u32 k = 0; k < 3; ++k)
+        double tx[4][3], tm[4][6];
+        for (u32 k = 0; k < 3; ++k) {
           memcpy(tx[k], rx[tris[tri][k]], sizeof(double) * 3);
+          if (em)
+            memcpy(tm[k], rm[tris[tri][k]], sizeof(double) * 6);
+        }
         double tmin = 1;
         for (u32 k = 0; k < 2; ++k) {
           memcpy(tx[3], ex[1 - k], sizeof(double) * 3);
-          double q = tet_quality((double const (*)[3]) tx);
+          if (em)
+            memcpy(tm[3], em[1 - k], sizeof(double) * 6);
+          double q = em ? metric_quality(3, (double const (*)[3]) tx, (double const (*)[6]) tm, 1)
+                        : tet_quality((double const (*)[3]) tx);
           if (q < tmin)
             tmin = q;
-          double t[3];
+          double t[3], tt[6];
           memcpy(t, tx[0], sizeof(t));
           memcpy(tx[0], tx[1], sizeof(t));
           memcpy(tx[1], t, sizeof(t));
+          if (em) {
+            memcpy(tt, tm[0], sizeof(tt));
+            memcpy(tm[0], tm[1], sizeof(tt));
+            memcpy(tm[1], tt, sizeof(tt));
+          }
         }
         cached[tri] = 1;
         cache[tri] = tmin;
@@ -2342,7 +2470,7 @@ unsigned orc_choose_edge_swap(unsigned ring_size, double const* edge_x6,
     double const* ring_x, double* quality)
 {
   return choose_edge_swap(ring_size, (double const (*)[3]) edge_x6,
-      (double const (*)[3]) ring_x, quality);
+      (double const (*)[3]) ring_x, quality, 0, 0);
 }
 
 /* find_edge_ring (edge_ring.cpp:12-74): the ring's vertices ordered by
@@ -2395,6 +2523,7 @@ static void swap_qualities(orc_mesh* m, u32* cands, double** p_q, u32** p_rs)
   u32 ne = mesh_count(m, 1);
   oup const* u = ask_up(m, 1, 3);
   double const* x = coords_of(m);
+  double const* mt = metric_of(m);
   double* eq = mesh_qualities(m);
   double* q = NEW(double, ne);
   u32* rsz = u32_filled(ne, 0);
@@ -2417,8 +2546,16 @@ static void swap_qualities(orc_mesh* m, u32* cands, double** p_q, u32** p_rs)
     memcpy(ex[1], x + 3 * edge_v[1], sizeof(double) * 3);
     for (u32 j = 0; j < rs; ++j)
       memcpy(rx[j], x + 3 * ring_v[j], sizeof(double) * 3);
+    double emt[2][6], rmt[MAX_EDGE_SWAP][6];
+    if (mt) {
+      memcpy(emt[0], mt + 6 * edge_v[0], sizeof(double) * 6);
+      memcpy(emt[1], mt + 6 * edge_v[1], sizeof(double) * 6);
+      for (u32 j = 0; j < rs; ++j)
+        memcpy(rmt[j], mt + 6 * ring_v[j], sizeof(double) * 6);
+    }
     double sq;
-    choose_edge_swap(rs, (double const (*)[3]) ex, (double const (*)[3]) rx, &sq);
+    choose_edge_swap(rs, (double const (*)[3]) ex, (double const (*)[3]) rx, &sq,
+        mt ? (double const (*)[6]) emt : 0, mt ? (double const (*)[6]) rmt : 0);
     if (sq > old_minq) {
       q[i] = sq;
       rsz[i] = rs;
@@ -2497,6 +2634,7 @@ static void swap_ents(orc_mesh* m, orc_mesh* mo, u32 d, u32 const* indset,
   u32 vpe = down_deg[d][0];
   u32* gen = NEW(u32, (size_t) ngen * vpe);
   double const* x = coords_of(m);
+  double const* mt = metric_of(m);
   for (u32 i = 0; i < ne; ++i) {
     if (!indset[i])
       continue;
@@ -2509,8 +2647,16 @@ static void swap_ents(orc_mesh* m, orc_mesh* mo, u32 d, u32 const* indset,
     memcpy(ex[1], x + 3 * edge_v[1], sizeof(double) * 3);
     for (u32 j = 0; j < rs; ++j)
       memcpy(rx[j], x + 3 * ring_v[j], sizeof(double) * 3);
+    double emt[2][6], rmt[MAX_EDGE_SWAP][6];
+    if (mt) {
+      memcpy(emt[0], mt + 6 * edge_v[0], sizeof(double) * 6);
+      memcpy(emt[1], mt + 6 * edge_v[1], sizeof(double) * 6);
+      for (u32 j = 0; j < rs; ++j)
+        memcpy(rmt[j], mt + 6 * ring_v[j], sizeof(double) * 6);
+    }
     double sq;
-    u32 code = choose_edge_swap(rs, (double const (*)[3]) ex, (double const (*)[3]) rx, &sq);
+    u32 code = choose_edge_swap(rs, (double const (*)[3]) ex, (double const (*)[3]) rx, &sq,
+        mt ? (double const (*)[6]) emt : 0, mt ? (double const (*)[6]) rmt : 0);
     get_swap_ents(rs, code, d, edge_v, ring_v, gen + (size_t) goff[i] * vpe);
   }
   u32* old = mark_up(m, 1, d, indset);
@@ -2739,7 +2885,7 @@ unsigned orc_warp_to_limit(orc_mesh* m, double qual_floor)
   for (u32 i = 0; i < 3 * nv; ++i)
     xo[i] = f * w[i] + x[i];
   while (1) {
-    double* q = element_qualities(dim, ne, ev, xo);
+    double* q = element_qualities(dim, ne, ev, xo, metric_of(m));
     double mq = orc_doubles_min(q, ne);
     free(q);
     if (!(mq < qual_floor))
@@ -2861,4 +3007,114 @@ void orc_measure_edges(orc_mesh* m, double* out)
   double* s = measure_edges(m);
   memcpy(out, s, sizeof(double) * mesh_count(m, 1));
   free(s);
+}
+
+/* metric from a Hessian: cyclic Jacobi eigendecomposition in FP64 and
+ * eigenvalue clamping; the extension of size_from_hessian.cpp:12-40 that the
+ * device code implements, in the same operation order */
+static void jacobi_eigen3(double const* h6, double* lam, double V[3][3])
+{
+  double A[3][3] = {{h6[0], h6[1], h6[2]}, {h6[1], h6[3], h6[4]}, {h6[2], h6[4], h6[5]}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      V[i][j] = (i == j) ? 1.0 : 0.0;
+  int const P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    double off = 0;
+    off += A[0][1] * A[0][1];
+    off += A[0][2] * A[0][2];
+    off += A[1][2] * A[1][2];
+    double dia = 0;
+    dia += A[0][0] * A[0][0];
+    dia += A[1][1] * A[1][1];
+    dia += A[2][2] * A[2][2];
+    if (off <= 1e-40 * dia || off == 0.0)
+      break;
+    for (int r = 0; r < 3; ++r) {
+      int p = P[r], q = Q[r];
+      double apq = A[p][q];
+      if (apq == 0.0)
+        continue;
+      double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+      double at = theta < 0 ? -theta : theta;
+      double t = 1.0 / (at + sqrt(theta * theta + 1.0));
+      if (theta < 0)
+        t = -t;
+      double c = 1.0 / sqrt(t * t + 1.0);
+      double s = t * c;
+      double app = A[p][p], aqq = A[q][q];
+      A[p][p] = app - t * apq;
+      A[q][q] = aqq + t * apq;
+      A[p][q] = 0.0;
+      A[q][p] = 0.0;
+      int k = 3 - p - q;
+      double akp = A[k][p], akq = A[k][q];
+      A[k][p] = c * akp - s * akq;
+      A[p][k] = A[k][p];
+      A[k][q] = s * akp + c * akq;
+      A[q][k] = A[k][q];
+      for (int i = 0; i < 3; ++i) {
+        double vip = V[i][p], viq = V[i][q];
+        V[i][p] = c * vip - s * viq;
+        V[i][q] = s * vip + c * viq;
+      }
+    }
+  }
+  lam[0] = A[0][0];
+  lam[1] = A[1][1];
+  lam[2] = A[2][2];
+}
+
+void orc_eigen3(double const* h6, double* lam, double* V9)
+{
+  double V[3][3];
+  jacobi_eigen3(h6, lam, V);
+  memcpy(V9, V, sizeof(V));
+}
+
+void orc_metric_from_hessian(orc_mesh* m, char const* hname, double eps, double hmin,
+    double hmax, char const* out_name)
+{
+  otag* t = find_tag(m, 0, hname);
+  if (!t || t->type != T_F64 || t->ncomps != 6)
+    die("hessian field must have 6 FP64 components");
+  u32 nv = mesh_count(m, 0);
+  double const* h = (double const*) t->data;
+  double* out = NEW(double, 6 * nv);
+  double lo = 1.0 / (hmax * hmax), hi = 1.0 / (hmin * hmin);
+  int const I[6] = {0, 0, 0, 1, 1, 2}, J[6] = {0, 1, 2, 1, 2, 2};
+  for (u32 v = 0; v < nv; ++v) {
+    double lam[3], V[3][3], l[3];
+    jacobi_eigen3(h + 6 * v, lam, V);
+    for (int k = 0; k < 3; ++k) {
+      double a = lam[k] < 0 ? -lam[k] : lam[k];
+      a = a / eps;
+      if (a < lo)
+        a = lo;
+      if (a > hi)
+        a = hi;
+      l[k] = a;
+    }
+    for (int c = 0; c < 6; ++c) {
+      double s = 0;
+      for (int k = 0; k < 3; ++k)
+        s += (V[I[c]][k] * l[k]) * V[J[c]][k];
+      out[6 * v + c] = s;
+    }
+  }
+  if (find_tag(m, 0, out_name))
+    tag_remove(&m->tags[0], out_name);
+  tag_add(&m->tags[0], T_F64, out_name, 6, out);
+}
+
+double orc_metric_quality(unsigned dim, double const* x, double const* m6, int signed_z)
+{
+  return metric_quality(dim, (double const (*)[3]) x, (double const (*)[6]) m6, signed_z);
+}
+
+double orc_metric_length(double const* a6, double const* b6, double const* x0, double const* x1)
+{
+  double e[3];
+  sub3(x1, x0, e);
+  return sqrt(metric_len2(a6, b6, e));
 }

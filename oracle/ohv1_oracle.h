@@ -78,6 +78,14 @@ void orc_swap_int_edges(unsigned ring_size, unsigned code, unsigned* out);
 void orc_invert_map(unsigned nin, unsigned const* in, unsigned nout,
     unsigned* out, unsigned* offsets);
 
+/* anisotropic extension: metric from a vertex Hessian field (6 comps) */
+void orc_eigen3(double const* h6, double* lam, double* V9);
+double orc_metric_quality(unsigned dim, double const* x, double const* m6, int signed_z);
+double orc_metric_length(double const* a6, double const* b6, double const* x0,
+    double const* x1);
+void orc_metric_from_hessian(orc_mesh* m, char const* hessian, double eps,
+    double hmin, double hmax, char const* out_name);
+
 /* verbosity of the per-pass prints (reference prints them from rank 0) */
 void orc_set_verbose(int v);
 

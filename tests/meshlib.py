@@ -57,6 +57,7 @@ _SIG = {
     "swap_slivers": (_u, [_p, _d, _u]),
     "adapt_timed": (C.c_int, [_p, _d, _d, _u, _u, _p]),
     "warp_to_limit": (_u, [_p, _d]),
+    "metric_from_hessian": (None, [_p, _s, _d, _d, _d, _s]),
 }
 _DOWN_DEG = [[1, 0, 0, 0], [2, 1, 0, 0], [3, 3, 1, 0], [4, 6, 4, 1]]
 # per-prefix symbol names that differ from the common name
@@ -271,6 +272,9 @@ class Mesh:
         stats = np.zeros(4, dtype=np.float64)
         r = self.call("adapt_timed", size_floor, good_qual, nlayers, max_ops, _ptr(stats))
         return int(r), stats
+
+    def metric_from_hessian(self, hessian, eps, hmin, hmax, out="metric"):
+        self.call("metric_from_hessian", hessian.encode(), eps, hmin, hmax, out.encode())
 
     def warp_to_limit(self, qual_floor):
         return int(self.call("warp_to_limit", qual_floor))

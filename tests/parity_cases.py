@@ -72,11 +72,40 @@ def c2_steps(lib, n=256, h=1.0 / 200.0, warp_floor=0.2, on_device=False):
     yield "adapt", m
 
 
+def aniso_steps(lib, dim, n, cycles=3, kind="tanh", a=10.0, eps=0.8, hmin=0.04, hmax=0.3,
+                h_short=0.05, aspect=4.0, on_device=False):
+    """Anisotropic extension: per cycle a fresh metric (from the analytic
+    Hessian of a tanh front through the FP64 eigendecomposition, or an
+    analytic rotating metric), then one mesh_adapt in that metric."""
+    m = make_mesh(lib, dim, n, on_device)
+    yield "setup", m
+    for cyc in range(cycles):
+        x = m.coords().reshape(-1, 3)
+        if m.tag(0, "metric", f64=True) is not None:
+            m.free_field(0, "metric")
+        if kind == "tanh":
+            m.add_field(0, "hessian", W.tanh_front_hessian(x, a=a, s0=1.5 if dim == 3 else 1.0, dim=dim).ravel())
+            m.metric_from_hessian("hessian", eps, hmin, hmax)
+            m.free_field(0, "hessian")
+        else:
+            m.add_field(0, "metric", W.rotating_metric(x, h_short, aspect, dim=dim).ravel())
+        yield "metric%d" % cyc, m
+        m.last = m.adapt(1.0 / 3.0, 0.3, 4, 50)
+        yield "adapt%d" % cyc, m
+
+
 CASES = {
     "c1": lambda lib, dev=False: c1_steps(lib, dev),
     "warp2d_sine_n16": lambda lib, dev=False: warp_steps(lib, 2, 16, 1 / 12.0, "sine", on_device=dev),
     "warp2d_rot_n32": lambda lib, dev=False: warp_steps(lib, 2, 32, 1 / 40.0, "rot", on_device=dev),
     "warp3d_rot_n4": lambda lib, dev=False: warp_steps(lib, 3, 4, 0.15, "rot", on_device=dev),
+}
+
+ANISO_CASES = {
+    "aniso3d_tanh_n6": lambda lib, dev=False: aniso_steps(lib, 3, 6, on_device=dev),
+    "aniso2d_tanh_n16": lambda lib, dev=False: aniso_steps(lib, 2, 16, eps=0.3, hmin=0.02, hmax=0.2, on_device=dev),
+    "aniso3d_rot_n6": lambda lib, dev=False: aniso_steps(lib, 3, 6, kind="rot", h_short=0.06, aspect=4.0,
+                                                         on_device=dev),
 }
 
 
