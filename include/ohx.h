@@ -156,11 +156,13 @@ osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;
 
 /* ------------------------------------------------------ instrumentation */
 void ohx_set_verbose(int on) OSH_PUBLIC;
-/* HIP-event timing of named kernels on the stream they run on */
+/* per-kernel timing: while enabled, each timed launch carries a start and a
+ * stop event in its own dispatch (hipExtLaunchKernel), so the interval is
+ * the kernel's execution alone */
 void ohx_profile_enable(int on) OSH_PUBLIC;
 void ohx_profile_reset(void) OSH_PUBLIC;
-/* total milliseconds, launches and work items (elements, edges...) of a
- * timed kernel; returns 0 if unseen */
+/* total milliseconds, launches and algorithmic HBM bytes (each input read
+ * once, each output written once) of a timed kernel; returns 0 if unseen */
 int ohx_profile_get(char const* name, double* ms, unsigned long long* launches,
     double* units) OSH_PUBLIC;
 /* newline-separated names of the timed kernels; returns the full length */

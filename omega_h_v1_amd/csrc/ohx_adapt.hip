@@ -73,8 +73,8 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
     parallel_for(n, OHX_LAMBDA(u32 i) { s[i] = filter[i] ? UNKNOWN : NOT_IN_SET; });
   }
   if (!n) return a;
-  // per round: state in and out, goodness and the star CSR
-  ProfScope ps("indset", (double) n * 16 + (double) g.adj.size() * 4);
+  // compulsory bytes per round: state in and out, goodness, the star CSR
+  double round_bytes = (double) n * 16 + (double) (n + 1) * 4 + (double) g.adj.size() * 4;
   // rounds are launched in batches; a round after convergence copies the
   // state unchanged, so running past the converged round is harmless
   constexpr u32 BATCH = 4, MAX_ROUNDS = 100;
@@ -82,11 +82,10 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
   for (u32 done = 0; done < MAX_ROUNDS; done += BATCH) {
     fill(flags, BATCH, 0);
     for (u32 r = 0; r < BATCH; ++r) {
-      hipLaunchKernelGGL(indset_round, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, g.offsets.get(),
+      launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
                          g.adj.get(), good, a.get(), b.get(), flags + r);
       std::swap(a, b);
     }
-    check_launch("indset_round");
     u32 h[BATCH];
     to_host(h, flags, BATCH);
     if (!h[BATCH - 1]) return a;
@@ -187,16 +186,15 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   if (require_better) eq = mesh_qualities(m);
   Buf<double> out(ns);
   zero_words(0, 1);
-  ProfScope ps("refine_quality_f64", (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24);
+  double prof_bytes = (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24;
   if (dim == 3)
-    hipLaunchKernelGGL(refine_quality_f64<3>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
+    launch_timed("refine_quality_f64", prof_bytes, refine_quality_f64<3>, dim3(nblocks(ns)), dim3(BLOCK), ns, vs,
                        u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(),
                        qual_floor, cand, out.get(), scratch_words(), dev_error_word());
   else
-    hipLaunchKernelGGL(refine_quality_f64<2>, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ns, vs,
+    launch_timed("refine_quality_f64", prof_bytes, refine_quality_f64<2>, dim3(nblocks(ns)), dim3(BLOCK), ns, vs,
                        u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(),
                        qual_floor, cand, out.get(), scratch_words(), dev_error_word());
-  check_launch("refine_quality_f64");
   return out;
 }
 
@@ -479,10 +477,9 @@ u32 refine_by_size(Mesh& m, double qual_floor) {
   Buf<u32> cand(ne);
   zero_words(0, 1);
   if (ne) {
-    ProfScope ps("refine_candidate_f64", (double) ne * 12 + (double) count(m, 0) * 32);
-    hipLaunchKernelGGL(refine_candidate_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+    double prof_bytes = (double) ne * 12 + (double) count(m, 0) * 32;
+    launch_timed("refine_candidate_f64", prof_bytes, refine_candidate_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
                        ask_down(m, 1, 0), coords(m), h, mt, cand.get(), scratch_words());
-    check_launch("refine_candidate_f64");
   }
   if (!read_word(W_ANY)) return 0;
   return refine_common(m, cand, qual_floor, false);
@@ -695,16 +692,15 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   Buf<double> cq((size_t) 2 * ne);
   zero_words(0, 1);
   {
-    ProfScope ps("coarsen_quality_f64", (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28);
+    double prof_bytes = (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28;
     if (dim == 3)
-      hipLaunchKernelGGL(coarsen_quality_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, ev,
+      launch_timed("coarsen_quality_f64", prof_bytes, coarsen_quality_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), ne, ev,
                          ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), metric_of(m), eq.get(),
                          qual_floor, codes.get(), cq.get(), scratch_words());
     else
-      hipLaunchKernelGGL(coarsen_quality_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, ev,
+      launch_timed("coarsen_quality_f64", prof_bytes, coarsen_quality_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), ne, ev,
                          ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), metric_of(m), eq.get(),
                          qual_floor, codes.get(), cq.get(), scratch_words());
-    check_launch("coarsen_quality_f64");
   }
   eq.release();
   if (!read_word(W_ANY)) return 0;
@@ -914,10 +910,9 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
   Buf<u32> codes(ne);
   zero_words(0, 1);
   if (ne) {
-    ProfScope ps("coarsen_size_code_f64", (double) ne * 12 + (double) count(m, 0) * 32);
-    hipLaunchKernelGGL(coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+    double prof_bytes = (double) ne * 12 + (double) count(m, 0) * 32;
+    launch_timed("coarsen_size_code_f64", prof_bytes, coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
                        ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words());
-    check_launch("coarsen_size_code_f64");
   }
   if (!read_word(W_ANY)) return 0;
   return coarsen_common(m, codes, qual_floor, false);
@@ -1325,11 +1320,10 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   Buf<u32> rsz = filled(ne, 0);
   zero_words(0, 1);
   {
-    ProfScope ps("swap_quality_f64", (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24);
-    hipLaunchKernelGGL(swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, u.offsets.get(),
+    double prof_bytes = (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24;
+    launch_timed("swap_quality_f64", prof_bytes, swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), ne, u.offsets.get(),
                        u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m), eq.get(), cand.get(), q.get(),
                        rsz.get(), scratch_words(), dev_error_word());
-    check_launch("swap_quality_f64");
   }
   eq.release();
   if (!read_word(W_ANY)) return 0;
@@ -1384,11 +1378,10 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
     u32 ngen = tots[0], nsame = tots[1];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     {
-      ProfScope ps("swap_cavity_f64", (double) ne * 12 + (double) ngen * vpe * 4);
-      hipLaunchKernelGGL(swap_cavity_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, d, is, goff.get(),
+      double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
+      launch_timed("swap_cavity_f64", prof_bytes, swap_cavity_f64, dim3(nblocks(ne)), dim3(BLOCK), ne, d, is, goff.get(),
                          u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m),
                          vout.get() + (size_t) nsame * vpe);
-      check_launch("swap_cavity_f64");
     }
     Buf<u32> cdo, cio;
     SwapKeepArgs a{};

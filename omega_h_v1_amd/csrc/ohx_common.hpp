@@ -4,6 +4,7 @@
 // array lives in HBM; the host keeps only counts and control flow.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -71,6 +72,8 @@ struct StreamScope {
   }
 };
 void sync();
+// after a failed scan: return the scan's tile ticket to zero (stream drained)
+void scan_recover();
 
 // Host-visible slots (8 bytes each) in mapped pinned memory: kernels write
 // the few scalars the host needs (scan totals, reductions, error codes)
@@ -199,15 +202,29 @@ void parallel_for(u32 n, F f) {
 #define OHX_LAMBDA [=] __device__
 
 // ---------------------------------------------------------------- timing
-// Per-kernel-name HIP event timers, enabled by ohx_profile_enable().
-// units: the work items of the launch (elements, edges...), summed per name
-void prof_begin(char const* name);
-void prof_end(char const* name, double units);
-struct ProfScope {
-  char const* name;
-  double units;
-  explicit ProfScope(char const* n, double u = 0) : name(n), units(u) { prof_begin(n); }
-  ~ProfScope() { prof_end(name, units); }
+// Per-kernel-name timers, enabled by ohx_profile_enable().  A timed launch
+// carries its start and stop events in its own dispatch
+// (hipExtLaunchKernel), so the interval is the kernel's execution alone, as
+// a kernel trace measures it; bytes: the launch's algorithmic (compulsory)
+// HBM bytes, summed per name.
+struct ProfEvents {
+  hipEvent_t a = nullptr, b = nullptr;
 };
+bool prof_on();
+ProfEvents prof_take();
+void prof_record(char const* name, double bytes, ProfEvents e);
+
+template <class K, class... A>
+void launch_timed(char const* name, double bytes, K kernel, dim3 grid, dim3 block, A... args) {
+  if (prof_on()) {
+    ProfEvents e = prof_take();
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, stream(), e.a, e.b, 0, args...);
+    check_launch(name);
+    prof_record(name, bytes, e);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, stream(), args...);
+    check_launch(name);
+  }
+}
 
 }  // namespace ohx

@@ -168,12 +168,11 @@ Buf<double> element_qualities(u32 dim, u32 n, u32 const* ev, double const* x, u3
   Buf<double> q(n);
   if (!n) return q;
   // compulsory bytes: connectivity, each vertex's coordinates once, output
-  ProfScope ps("elem_quality_f64", (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) n * 8);
+  double prof_bytes = (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) n * 8;
   if (dim == 3)
-    hipLaunchKernelGGL(elem_quality_f64<3>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, mt, q.get());
+    launch_timed("elem_quality_f64", prof_bytes, elem_quality_f64<3>, dim3(nblocks(n)), dim3(BLOCK), n, ev, x, mt, q.get());
   else
-    hipLaunchKernelGGL(elem_quality_f64<2>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, ev, x, mt, q.get());
-  check_launch("elem_quality_f64");
+    launch_timed("elem_quality_f64", prof_bytes, elem_quality_f64<2>, dim3(nblocks(n)), dim3(BLOCK), n, ev, x, mt, q.get());
   return q;
 }
 
@@ -188,12 +187,11 @@ double min_element_quality(u32 dim, u32 n, u32 const* ev, double const* x, u32 n
   u32 nb = std::min<u32>(nblocks(n - 1), 1024);
   Buf<double> part(nb);
   {
-    ProfScope ps("elem_min_quality_f64", (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) nb * 8);
+    double prof_bytes = (double) n * (dim + 1) * 4 + (double) nverts * 24 + (double) nb * 8;
     if (dim == 3)
-      hipLaunchKernelGGL(elem_min_quality_f64<3>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, mt, part.get());
+      launch_timed("elem_min_quality_f64", prof_bytes, elem_min_quality_f64<3>, dim3(nb), dim3(BLOCK), n, 1u, ev, x, mt, part.get());
     else
-      hipLaunchKernelGGL(elem_min_quality_f64<2>, dim3(nb), dim3(BLOCK), 0, stream(), n, 1u, ev, x, mt, part.get());
-    check_launch("elem_min_quality_f64");
+      launch_timed("elem_min_quality_f64", prof_bytes, elem_min_quality_f64<2>, dim3(nb), dim3(BLOCK), n, 1u, ev, x, mt, part.get());
   }
   return min_f64(part.get(), nb, 0);
 }
@@ -219,10 +217,9 @@ Buf<double> measure_edges(Mesh& m) {
   double const* mt = metric_of(m);
   Buf<double> out(ne);
   if (ne) {
-    ProfScope ps("measure_edge_f64", (double) ne * 16 + (double) count(m, 0) * (mt ? 72 : 32));
-    hipLaunchKernelGGL(measure_edge_f64, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+    double prof_bytes = (double) ne * 16 + (double) count(m, 0) * (mt ? 72 : 32);
+    launch_timed("measure_edge_f64", prof_bytes, measure_edge_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
                        ask_down(m, 1, 0), coords(m), h, mt, out.get());
-    check_launch("measure_edge_f64");
   }
   return out;
 }
@@ -274,24 +271,27 @@ __global__ __launch_bounds__(BLOCK) void upfd_sort_split(u32 nlows, u32 deg, u32
 }
 
 void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_highs, Up& u) {
-  // compulsory bytes: the down map in, offsets and (high, direction) out
-  ProfScope ps("up_from_down", (double) nhighs * h_down_deg[hd][ld] * 12 + (double) (nlows + 1) * 4);
   u32 deg = h_down_deg[hd][ld];
   u32 nuses = nhighs * deg;
   Buf<u32> counts = filled(nlows, 0);
-  if (nuses)
-    hipLaunchKernelGGL(upfd_count, dim3(nblocks(nuses)), dim3(BLOCK), 0, stream(), nuses, lows_of_highs,
-                       counts.get());
+  if (nuses) {
+    // compulsory bytes: the down map in, each counter read and written
+    launch_timed("upfd_count", (double) nuses * 4 + (double) nlows * 8, upfd_count, dim3(nblocks(nuses)),
+                 dim3(BLOCK), nuses, lows_of_highs, counts.get());
+  }
   u.offsets = exscan(counts.get(), nlows);
   u.adj = Buf<u32>(nuses);
   u.dirs = Buf<u32>(nuses);
-  if (nuses)
-    hipLaunchKernelGGL(upfd_fill, dim3(nblocks(nuses)), dim3(BLOCK), 0, stream(), nuses, lows_of_highs,
-                       u.offsets.get(), counts.get(), u.adj.get());
-  if (nlows)
-    hipLaunchKernelGGL(upfd_sort_split, dim3(nblocks(nlows)), dim3(BLOCK), 0, stream(), nlows, deg,
-                       u.offsets.get(), u.adj.get(), u.dirs.get());
-  check_launch("up_from_down");
+  if (nuses) {
+    // down map and uses out, offsets and counters
+    launch_timed("upfd_fill", (double) nuses * 8 + (double) nlows * 12, upfd_fill, dim3(nblocks(nuses)),
+                 dim3(BLOCK), nuses, lows_of_highs, u.offsets.get(), counts.get(), u.adj.get());
+  }
+  if (nlows) {
+    // offsets, uses in, (high, direction) out
+    launch_timed("upfd_sort_split", (double) (nlows + 1) * 4 + (double) nuses * 12, upfd_sort_split,
+                 dim3(nblocks(nlows)), dim3(BLOCK), nlows, deg, u.offsets.get(), u.adj.get(), u.dirs.get());
+  }
 }
 
 // ---------------------------------------------------------- reflect_down
@@ -352,22 +352,21 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   u32* err = dev_error_word();
   // compulsory bytes: high vertices in, low ids out, lows-of-vertex CSR and
   // low vertices read once
-  ProfScope ps("reflect_down", (double) nh * ((hd + 1) * 4 + lph * 4) +
+  double prof_bytes = (double) nh * ((hd + 1) * 4 + lph * 4) +
                                    (double) (count(m, 0) + 1) * 4 + (double) read_nuses(lv, count(m, 0)) * 8 +
-                                   (double) count(m, ld) * (ld + 1) * 4);
+                                   (double) count(m, ld) * (ld + 1) * 4;
   dim3 g(nblocks(nh)), b(BLOCK);
   if (hd == 3 && ld == 1)
-    hipLaunchKernelGGL((reflect_down_kernel<3, 1>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
+    launch_timed("reflect_down", prof_bytes, reflect_down_kernel<3, 1>, g, b, nh, vh, vl, lv.offsets.get(),
                        lv.adj.get(), lv.dirs.get(), out.get(), err);
   else if (hd == 3 && ld == 2)
-    hipLaunchKernelGGL((reflect_down_kernel<3, 2>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
+    launch_timed("reflect_down", prof_bytes, reflect_down_kernel<3, 2>, g, b, nh, vh, vl, lv.offsets.get(),
                        lv.adj.get(), lv.dirs.get(), out.get(), err);
   else if (hd == 2 && ld == 1)
-    hipLaunchKernelGGL((reflect_down_kernel<2, 1>), g, b, 0, stream(), nh, vh, vl, lv.offsets.get(),
+    launch_timed("reflect_down", prof_bytes, reflect_down_kernel<2, 1>, g, b, nh, vh, vl, lv.offsets.get(),
                        lv.adj.get(), lv.dirs.get(), out.get(), err);
   else
     fail("reflect_down: unsupported dimensions", __FILE__, __LINE__);
-  check_launch("reflect_down");
   return out;
 }
 
@@ -800,14 +799,13 @@ Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers) {
   u32 ne = count(m, m.dim);
   Buf<u32> s(ne);
   if (ne) {
-    ProfScope ps("mark_sliver_f64", (double) ne * (m.dim + 1) * 4 + (double) count(m, 0) * 24 + (double) ne * 4);
+    double prof_bytes = (double) ne * (m.dim + 1) * 4 + (double) count(m, 0) * 24 + (double) ne * 4;
     if (m.dim == 3)
-      hipLaunchKernelGGL(mark_sliver_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+      launch_timed("mark_sliver_f64", prof_bytes, mark_sliver_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), ne,
                          ask_down(m, 3, 0), coords(m), metric_of(m), good_qual, s.get());
     else
-      hipLaunchKernelGGL(mark_sliver_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne,
+      launch_timed("mark_sliver_f64", prof_bytes, mark_sliver_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), ne,
                          ask_down(m, 2, 0), coords(m), metric_of(m), good_qual, s.get());
-    check_launch("mark_sliver_f64");
   }
   mark_dual_layers(m, s, nlayers);
   return s;

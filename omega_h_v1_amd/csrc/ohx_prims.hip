@@ -93,7 +93,7 @@ ScanTiles scan_tiles_for(u32 nt) {
   ScanState& st = scan_state();
   if (!st.ticket) {
     st.ticket = static_cast<u64*>(dev_alloc(sizeof(u64)));
-    OHX_CHECK(hipMemsetAsync(st.ticket, 0xff, sizeof(u64), stream()));
+    OHX_CHECK(hipMemsetAsync(st.ticket, 0, sizeof(u64), stream()));
   }
   if (nt > st.cap) {
     // a larger tile array; the old one is released once the stream drains
@@ -108,6 +108,11 @@ ScanTiles scan_tiles_for(u32 nt) {
   // epochs stay below 2^30 - 1, so fresh all-ones words never match one
   st.epoch = st.epoch + 1 >= 0x3fffffffu ? 1u : st.epoch + 1;
   return ScanTiles{st.tiles, st.ticket, st.epoch};
+}
+
+void scan_recover() {
+  ScanState& st = scan_state();
+  if (st.ticket) OHX_CHECK(hipMemset(st.ticket, 0, sizeof(u64)));
 }
 
 void exscan(u32 const* a, u32 n, u32* out) {

@@ -57,6 +57,7 @@ void sync() {
   if (*e) {
     unsigned code = *e;
     *e = 0;
+    if (code == ERR_SCAN_SPIN) scan_recover();
     fail(dev_error_text(code), __FILE__, __LINE__);
   }
 }
@@ -192,23 +193,17 @@ hipEvent_t take_event() {
 }
 }  // namespace
 
-void prof_begin(char const* name) {
-  Prof& p = prof();
-  if (!p.on) return;
-  hipEvent_t e = take_event();
-  OHX_CHECK(hipEventRecord(e, stream()));
-  p.open[name] = e;
+bool prof_on() { return prof().on; }
+
+ProfEvents prof_take() {
+  ProfEvents e;
+  e.a = take_event();
+  e.b = take_event();
+  return e;
 }
 
-void prof_end(char const* name, double units) {
-  Prof& p = prof();
-  if (!p.on) return;
-  auto it = p.open.find(name);
-  if (it == p.open.end()) return;
-  hipEvent_t e = take_event();
-  OHX_CHECK(hipEventRecord(e, stream()));
-  p.pending.push_back({name, it->second, e, units});
-  p.open.erase(it);
+void prof_record(char const* name, double bytes, ProfEvents e) {
+  prof().pending.push_back({name, e.a, e.b, bytes});
 }
 
 void prof_enable(bool on) {

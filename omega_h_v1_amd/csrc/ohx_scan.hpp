@@ -3,7 +3,8 @@
 // exscan_fn(n, f, out, host_total): out[0] = 0, out[i+1] = f(0) + ... + f(i),
 // where f is a __device__ functor; optionally the total goes to a host slot.
 // Small inputs (one tile) use one block; larger ones a decoupled look-back
-// scan (tile tickets, 64-bit (epoch, flag, value) tile words, bounded spin).
+// scan (fetch-add tile tickets, 64-bit (epoch, flag, value) tile words, a
+// wave-wide look-back, bounded spin).
 #pragma once
 
 #include "ohx_common.hpp"
@@ -68,21 +69,11 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
   if (threadIdx.x == 0) {
     u32 t = 0;
     if (ntiles > 1) {
-      // per-epoch ticket: the first block of a scan restarts the count
-      u64 old = __hip_atomic_load(ticket, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      while (true) {
-        u64 nxt;
-        if ((u32) (old >> 32) != epoch) {
-          nxt = ((u64) epoch << 32) | 1u;
-          t = 0;
-        } else {
-          nxt = old + 1;
-          t = (u32) old;
-        }
-        if (__hip_atomic_compare_exchange_strong(ticket, &old, nxt, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-          break;
-      }
+      // tile ticket in dispatch order (a block only ever waits on tiles that
+      // earlier-started blocks own); the holder of the last ticket returns
+      // the counter to zero for the next scan on the stream
+      t = (u32) __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == ntiles - 1) __hip_atomic_store(ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     s_tile = t;
   }
@@ -109,36 +100,53 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
   }
   u32 agg;
   u32 run = block_excl_scan(s, lds_w, &agg);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
+    // wave 0 publishes this tile and looks back over up to 64 predecessor
+    // tiles per step: lane k reads tile j - k; the closest inclusive prefix
+    // ends the walk once every tile before it in the window is published
+    u32 lane = threadIdx.x;
     u32 prefix = 0;
     if (ntiles > 1) {
-      if (t == 0) {
-        __hip_atomic_store(&tiles[0], pack(epoch, 1, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(&tiles[t], pack(epoch, 0, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        u32 j = t - 1;
+      if (lane == 0)
+        __hip_atomic_store(&tiles[t], pack(epoch, t == 0 ? 1u : 0u, agg), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      if (t > 0) {
+        int j = (int) t - 1;
         u32 spins = 0;
         while (true) {
-          u64 v = __hip_atomic_load(&tiles[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          int idx = j - (int) lane;
+          u64 v = idx >= 0 ? __hip_atomic_load(&tiles[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : pack(epoch, 1, 0);
           u32 hi = (u32) (v >> 32);
-          if ((hi >> 1) != epoch) {
+          bool ready = (hi >> 1) == epoch;
+          bool incl = ready && (hi & 1u);
+          u64 pmask = __ballot(incl);
+          u64 nrmask = __ballot(!ready);
+          u32 first = pmask ? (u32) (__ffsll((long long) pmask) - 1) : 64u;
+          u64 need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+          if (nrmask & need) {
             if (++spins > (1u << 22)) {
-              *err = ERR_SCAN_SPIN;
+              if (lane == 0) *err = ERR_SCAN_SPIN;
               break;
             }
             __builtin_amdgcn_s_sleep(1);
             continue;
           }
-          prefix += (u32) v;
-          if (hi & 1u) break;
-          --j;
+          u32 val = lane <= first ? (u32) v : 0u;
+#pragma unroll
+          for (u32 d = 32; d >= 1; d >>= 1) val += __shfl_xor(val, d, 64);
+          prefix += val;
+          if (first < 64) break;
+          j -= 64;
         }
-        __hip_atomic_store(&tiles[t], pack(epoch, 1, prefix + agg), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0)
+          __hip_atomic_store(&tiles[t], pack(epoch, 1, prefix + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    s_prefix = prefix;
-    if (t == ntiles - 1 && host_total) *host_total = prefix + agg;
+    if (lane == 0) {
+      s_prefix = prefix;
+      if (t == ntiles - 1 && host_total) *host_total = prefix + agg;
+    }
   }
   __syncthreads();
   run += s_prefix;
@@ -163,9 +171,11 @@ template <class F>
 void exscan_fn(u32 n, F f, u32* out, u32* host_total_dev = nullptr) {
   u32 nt = n ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
   ScanTiles st = scan_tiles_for(nt);
-  hipLaunchKernelGGL(scan_detail::scan_fn_kernel<F>, dim3(nt), dim3(SCAN_T), 0, stream(), n, f, nt, out,
+  // compulsory bytes: one 4-byte input per item (a lower bound for functors
+  // that read more) and the n + 1 offsets out
+  double prof_bytes = (double) n * 4 + (double) (n + 1) * 4;
+  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
                      st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
-  check_launch("scan_fn_kernel");
 }
 
 // as exscan_fn and returns the total (one synchronisation)
