@@ -74,6 +74,11 @@ def profile_get(dll, name):
     return ms.value, n.value, units.value
 
 
+def _set_profile_api(dll):
+    dll.ohx_profile_only.argtypes = [C.c_char_p]
+    dll.ohx_profile_only.restype = None
+
+
 def profile_names(dll):
     f = dll.ohx_profile_names
     f.restype = C.c_uint
@@ -171,6 +176,7 @@ def main():
     if lib is None:
         sys.exit("bench.py: product library not built (run __graft_entry__.build())")
     dll = lib.dll
+    _set_profile_api(dll)
     base = prepare(lib, True)
     t_in = base.count(2)
 
@@ -182,7 +188,27 @@ def main():
     for _ in range(args.warmup):
         step()
     dll.ohx_synchronize()
+
+    # one untimed survey step with every kernel timed: the per-kernel table
+    # and the dominant kernel (most total time)
     dll.ohx_profile_reset()
+    dll.ohx_profile_only(None)
+    dll.ohx_profile_enable(1)
+    step()
+    dll.ohx_synchronize()
+    dll.ohx_profile_enable(0)
+    survey = {name: profile_get(dll, name) for name in profile_names(dll)}
+    host_sync = survey.pop("host_sync", None)
+    dom = max(survey, key=lambda k: survey[k][0]) if survey else None
+    per_kernel = {
+        name: {"ms": round(ms, 3), "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
+               "GB/s": round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+        for name, (ms, n, nb) in sorted(survey.items(), key=lambda kv: -kv[1][0])}
+
+    # the timed region; the dominant kernel's launches carry start/stop
+    # events in their own dispatches, which give its average duration
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(dom.encode() if dom else None)
     dll.ohx_profile_enable(1)
     elems = 0.0
     passes = 0
@@ -197,29 +223,18 @@ def main():
     dll.ohx_synchronize()
     t1 = time.perf_counter()
     dll.ohx_profile_enable(0)
+    dll.ohx_profile_only(None)
     wall = t1 - t0
     value = elems / wall
 
-    # every profiled scope brackets exactly one kernel launch with HIP events
-    # on the library stream; "bytes" is the launch's algorithmic HBM traffic
-    kernels = {}
-    for name in profile_names(dll):
-        ms, n, nbytes = profile_get(dll, name)
-        kernels[name] = {"ms": ms, "launches": n, "bytes": nbytes}
-    dom = max(kernels, key=lambda k: kernels[k]["ms"]) if kernels else None
     roofline = None
     if dom:
-        k = kernels[dom]
-        achieved = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+        ms, n, nbytes = profile_get(dll, dom)
+        achieved = nbytes / (ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get(dom),
-                    "avg_launch_us": round(1e3 * k["ms"] / max(k["launches"], 1), 3),
-                    "algorithmic_bytes_per_launch": round(k["bytes"] / max(k["launches"], 1))}
-    per_kernel = {
-        name: {"ms": round(v["ms"], 3), "launches": v["launches"],
-               "avg_us": round(1e3 * v["ms"] / max(v["launches"], 1), 2),
-               "GB/s": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
-        for name, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms"])}
+                    "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
+                    "algorithmic_bytes_per_launch": round(nbytes / max(n, 1))}
 
     prod_digest = digest(last)
     cpu = None if args.no_cpu else cpu_baseline(prod_digest, passes // args.steps)
@@ -245,6 +260,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": per_kernel,
+        "kernels_note": "one untimed step with every kernel timed by dispatch-carried events",
+        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
         "peak_device_bytes": int(peak_bytes(dll)),
     }
     print(json.dumps(out))

@@ -156,10 +156,17 @@ osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;
 
 /* ------------------------------------------------------ instrumentation */
 void ohx_set_verbose(int on) OSH_PUBLIC;
+/* round limit of the independent set: 100 as the reference (indset.cpp:33,
+ * the default; 0 restores it); a higher limit is an extension for meshes
+ * whose equal-goodness candidates form long chains (each round settles at
+ * least the lowest-numbered candidate of a tie chain) */
+void ohx_set_indset_max_rounds(unsigned n) OSH_PUBLIC;
 /* per-kernel timing: while enabled, each timed launch carries a start and a
  * stop event in its own dispatch (hipExtLaunchKernel), so the interval is
  * the kernel's execution alone */
 void ohx_profile_enable(int on) OSH_PUBLIC;
+/* restrict the timing to the kernel of this name (NULL: every kernel) */
+void ohx_profile_only(char const* name) OSH_PUBLIC;
 void ohx_profile_reset(void) OSH_PUBLIC;
 /* total milliseconds, launches and algorithmic HBM bytes (each input read
  * once, each output written once) of a timed kernel; returns 0 if unseen */

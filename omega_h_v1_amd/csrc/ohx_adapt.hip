@@ -13,6 +13,8 @@
 namespace ohx {
 
 int g_verbose = 0;
+// indset.cpp:33 stops after 100 rounds; raising it is an extension
+u32 g_indset_max_rounds = 100;
 
 static double now_s() {
   using namespace std::chrono;
@@ -77,7 +79,8 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
   double round_bytes = (double) n * 16 + (double) (n + 1) * 4 + (double) g.adj.size() * 4;
   // rounds are launched in batches; a round after convergence copies the
   // state unchanged, so running past the converged round is harmless
-  constexpr u32 BATCH = 4, MAX_ROUNDS = 100;
+  constexpr u32 BATCH = 4;
+  u32 const MAX_ROUNDS = g_indset_max_rounds;
   u32* flags = scratch_words() + W_FLAGS;
   for (u32 done = 0; done < MAX_ROUNDS; done += BATCH) {
     fill(flags, BATCH, 0);
@@ -90,7 +93,7 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
     to_host(h, flags, BATCH);
     if (!h[BATCH - 1]) return a;
   }
-  fail("independent set did not converge in 100 rounds (indset.cpp:107-116)", __FILE__, __LINE__);
+  fail("independent set did not converge within the round limit (indset.cpp:107-116)", __FILE__, __LINE__);
   return a;
 }
 
@@ -452,7 +455,21 @@ static void say(char const* verb, u32 n, u32 d) {
 }
 
 // refine_common (refine_common.cpp:131-190) for edges
+// Every adjacency CSR uses 32-bit offsets, as the reference's.  The largest
+// one a pass builds is the edge star (3D: 2 uses(1,2) + uses(1,3) = 6F + 6T;
+// 2D: 2 uses(1,2) = 6T), and a refine pass can at most double the element
+// count; refuse a mesh that could overflow 32 bits instead of wrapping.
+static void check_index_space(Mesh& m) {
+  size_t t = count(m, m.dim);
+  // faces not derived yet: at most 4 per tet
+  size_t f = m.dim == 3 ? (has_dim(m, 2) ? (size_t) m.counts[2] : 4 * t) : 0;
+  size_t star = m.dim == 3 ? 6 * f + 6 * t : 6 * t;
+  if (2 * star >= ((size_t) 1 << 32))
+    fail("mesh too large for 32-bit adjacency offsets", __FILE__, __LINE__);
+}
+
 static u32 refine_common(Mesh& m, Buf<u32>& cand, double qual_floor, bool require_better) {
+  check_index_space(m);
   u32 ns = count(m, 1);
   Buf<double> q = refine_qualities(m, cand.get(), qual_floor, require_better);
   if (!read_word(W_ANY)) return 0;
@@ -679,6 +696,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
 }
 
 static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better) {
+  check_index_space(m);
   u32 dim = m.dim;
   u32 ne = count(m, 1);
   check_basic_class(m, codes.get());
@@ -1307,6 +1325,7 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
 
 // swap_common + swap_interior + swap_ents (swap.cpp:27-136)
 static u32 swap_common(Mesh& m, Buf<u32>& cand) {
+  check_index_space(m);
   u32 ne = count(m, 1);
   if (!max_u32(cand.get(), ne)) return 0;
   unmark_boundary(m, 1, cand.get());

@@ -29,5 +29,6 @@ u32 swap_nint_edges(u32 ring_size);
 void swap_int_edges_host(u32 ring_size, u32 code, u32* out);
 
 extern int g_verbose;
+extern u32 g_indset_max_rounds;
 
 }  // namespace ohx

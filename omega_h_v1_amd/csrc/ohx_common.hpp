@@ -211,12 +211,16 @@ struct ProfEvents {
   hipEvent_t a = nullptr, b = nullptr;
 };
 bool prof_on();
+bool prof_on_for(char const* name);
+void prof_only(char const* name);
+// a host-side interval (e.g. time spent waiting in a synchronisation)
+void prof_host(char const* name, double ms);
 ProfEvents prof_take();
 void prof_record(char const* name, double bytes, ProfEvents e);
 
 template <class K, class... A>
 void launch_timed(char const* name, double bytes, K kernel, dim3 grid, dim3 block, A... args) {
-  if (prof_on()) {
+  if (prof_on_for(name)) {
     ProfEvents e = prof_take();
     hipExtLaunchKernelGGL(kernel, grid, block, 0, stream(), e.a, e.b, 0, args...);
     check_launch(name);

@@ -272,6 +272,7 @@ __global__ __launch_bounds__(BLOCK) void upfd_sort_split(u32 nlows, u32 deg, u32
 
 void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_highs, Up& u) {
   u32 deg = h_down_deg[hd][ld];
+  if ((size_t) nhighs * deg >= ((size_t) 1 << 32)) fail("up_from_down: more than 2^32 uses", __FILE__, __LINE__);
   u32 nuses = nhighs * deg;
   Buf<u32> counts = filled(nlows, 0);
   if (nuses) {
@@ -433,7 +434,7 @@ static Graph star_vertex_edge(Mesh& m) {
   Up const& u = ask_up(m, 0, 1);
   Graph g;
   g.offsets = copy(u.offsets.get(), nv + 1);
-  u32 nadj = read_at(u.offsets.get(), nv);
+  u32 nadj = u.adj.size();
   g.adj = Buf<u32>(nadj);
   u32* ga = g.adj.get();
   u32 const* ua = u.adj.get();
@@ -452,7 +453,7 @@ static Graph star_edge_tri(Mesh& m) {
   u32* go = g.offsets.get();
   u32 const* uo = u.offsets.get();
   parallel_for(ne + 1, OHX_LAMBDA(u32 i) { go[i] = uo[i] * 2; });
-  u32 nuse = read_at(uo, ne);
+  u32 nuse = u.adj.size();
   g.adj = Buf<u32>((size_t) nuse * 2);
   u32* ga = g.adj.get();
   u32 const* ua = u.adj.get();
@@ -484,7 +485,10 @@ static Graph star_edge_tet(Mesh& m) {
   parallel_for(ne, OHX_LAMBDA(u32 i) { dg[i] = 2 * (to[i + 1] - to[i]) + (ro[i + 1] - ro[i]); });
   Graph g;
   g.offsets = Buf<u32>(ne + 1);
-  g.adj = Buf<u32>(exscan_total(dg, ne, g.offsets.get()));
+  size_t nadj = (size_t) 2 * ut.adj.size() + ur.adj.size();
+  if (nadj >= ((size_t) 1 << 32)) fail("edge star: more than 2^32 entries", __FILE__, __LINE__);
+  exscan(dg, ne, g.offsets.get());
+  g.adj = Buf<u32>(nadj);
   u32* ga = g.adj.get();
   u32 const* go = g.offsets.get();
   parallel_for(ne, OHX_LAMBDA(u32 i) {

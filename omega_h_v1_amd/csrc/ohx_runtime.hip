@@ -7,6 +7,8 @@
 
 #include "ohx_common.hpp"
 
+#include <chrono>
+
 namespace ohx {
 
 static std::string g_last_error;
@@ -52,7 +54,15 @@ static char const* dev_error_text(unsigned e) {
 }
 
 void sync() {
-  OHX_CHECK(hipStreamSynchronize(stream()));
+  if (prof_on()) {
+    // host-side count and wall time of the stream synchronisations
+    auto t0 = std::chrono::steady_clock::now();
+    OHX_CHECK(hipStreamSynchronize(stream()));
+    auto t1 = std::chrono::steady_clock::now();
+    prof_host("host_sync", std::chrono::duration<double, std::milli>(t1 - t0).count());
+  } else {
+    OHX_CHECK(hipStreamSynchronize(stream()));
+  }
   volatile unsigned* e = host_slot<unsigned>(SLOT_ERROR);
   if (*e) {
     unsigned code = *e;
@@ -162,6 +172,7 @@ void dev_reset_high_water() { arena().high = arena().in_use; }
 namespace {
 struct Prof {
   bool on = false;
+  std::string only;  // when set, only the kernel of this name is timed
   std::vector<hipEvent_t> pool;
   struct Pending {
     std::string name;
@@ -194,6 +205,19 @@ hipEvent_t take_event() {
 }  // namespace
 
 bool prof_on() { return prof().on; }
+
+bool prof_on_for(char const* name) {
+  Prof& p = prof();
+  return p.on && (p.only.empty() || p.only == name);
+}
+
+void prof_only(char const* name) { prof().only = name ? name : ""; }
+
+void prof_host(char const* name, double ms) {
+  auto& t = prof().totals[name];
+  t.ms += ms;
+  t.launches += 1;
+}
 
 ProfEvents prof_take() {
   ProfEvents e;

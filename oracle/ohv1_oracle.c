@@ -28,6 +28,14 @@ typedef unsigned u32;
 #define REF_PI 3.14159265359 /* algebra.hpp:8 */
 
 static int g_verbose = 0;
+static unsigned indset_max_rounds = 100;
+/* round limit of the independent set (indset.cpp:33: 100); raising it is an
+ * extension for meshes whose ties form long chains */
+void orc_set_indset_max_rounds(unsigned n)
+{
+  indset_max_rounds = n ? n : 100;
+}
+
 void orc_set_verbose(int v) { g_verbose = v; }
 
 static void die(char const* what)
@@ -1463,7 +1471,7 @@ static u32* find_indset(orc_mesh* m, u32 ent_dim, u32 const* filter,
   for (u32 i = 0; i < n; ++i)
     state[i] = filter[i] ? UNKNOWN : NOT_IN_SET;
   u32* old = NEW(u32, n);
-  for (u32 it = 0; it < 100; ++it) {
+  for (u32 it = 0; it < indset_max_rounds; ++it) {
     memcpy(old, state, sizeof(u32) * n);
     for (u32 i = 0; i < n; ++i) {
       if (old[i] != UNKNOWN)
@@ -1502,7 +1510,7 @@ static u32* find_indset(orc_mesh* m, u32 ent_dim, u32 const* filter,
       return state;
     }
   }
-  die("independent set did not converge in 100 rounds");
+  die("independent set did not converge in the round limit");
   return 0;
 }
 

@@ -86,6 +86,9 @@ double orc_metric_length(double const* a6, double const* b6, double const* x0,
 void orc_metric_from_hessian(orc_mesh* m, char const* hessian, double eps,
     double hmin, double hmax, char const* out_name);
 
+/* round limit of the independent set (default 100, as indset.cpp:33) */
+void orc_set_indset_max_rounds(unsigned n);
+
 /* verbosity of the per-pass prints (reference prints them from rank 0) */
 void orc_set_verbose(int v);
 
