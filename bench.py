@@ -14,6 +14,21 @@ that changed nothing, the per-pass summaries), which makes the figure a
 lower bound of the per-pass definition.
 
 Prints one JSON line (see the keys at the end of main()).
+
+Other workloads (--workload), each printing one JSON line of its own:
+  c3  3D Kuhn box 64^3 x 6 = 1,572,864 tets; per cycle the metric comes from
+      the analytic Hessian of f = tanh(50 (x + y + z - 1.5)) through the FP64
+      eigendecomposition (eps 0.01, h in [0.002, 0.1]), then one mesh_adapt
+      (at most 50 operations, the reference default); 8 cycles.  A cycle that
+      reaches the operation limit ends there and the next cycle starts from
+      the mesh as it is (the reference would abort).  --precision picks the
+      FP64 path or the FP32-mixed one.
+  c5  3D Kuhn box 160^3 x 6 = 24,576,000 tets, rotating anisotropic metric
+      (short axis 1e-3 in the xy plane at angle 2 pi z, 20:1), the first 8
+      passes of mesh_adapt: per-kernel GB/s at the real shape and the peak
+      device memory.
+  --calibrate-copy runs plain device-to-device copies of known size, for
+  scaling the FETCH_SIZE / WRITE_SIZE counters of a profiler run.
 """
 import argparse
 import ctypes as C
@@ -160,23 +175,143 @@ def cpu_baseline(product_digest, product_passes):
     return out
 
 
+
+COPY_BYTES = 1 << 28  # 256 MiB
+
+
+def calibrate_copy():
+    """Five device-to-device copies of 256 MiB (one copy kernel each): the
+    profiler's FETCH_SIZE / WRITE_SIZE for them against the known size give
+    the counters' scale."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": COPY_BYTES, "copies": 5}))
+
+
+def _profile_table(dll):
+    tab = {name: profile_get(dll, name) for name in profile_names(dll)}
+    host_sync = tab.pop("host_sync", None)
+    return tab, host_sync
+
+
+def _roofline(tab):
+    if not tab:
+        return None, {}
+    dom = max(tab, key=lambda k: tab[k][0])
+    ms, n, nbytes = tab[dom]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get(dom),
+            "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
+            "algorithmic_bytes_per_launch": round(nbytes / max(n, 1))}
+    per = {name: {"ms": round(ms, 3), "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
+                  "GB/s": round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+           for name, (ms, n, nb) in sorted(tab.items(), key=lambda kv: -kv[1][0])}
+    return roof, per
+
+
+def run_large(args, lib):
+    """The 3D anisotropic workloads (c3, c5); every kernel timed by
+    dispatch-carried events over the whole run."""
+    dll = lib.dll
+    dll.ohx_set_precision.argtypes = [C.c_int]
+    dll.ohx_set_precision(1 if args.precision == "mixed" else 0)
+    dll.ohx_device_bytes_high_water.restype = C.c_ulonglong
+    dll.ohx_set_indset_max_rounds(2000)
+    if args.workload == "c3":
+        n, cycles, max_ops = 64, 8, 50
+    else:
+        n, cycles, max_ops = 160, 1, 8
+    t0 = time.perf_counter()
+    m = ML.Mesh.box_on_device(lib, 3, n)
+    m.derive_model_full()
+    dll.ohx_synchronize()
+    t_setup = time.perf_counter() - t0
+    t_in0 = m.count(3)
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(None)
+    dll.ohx_profile_enable(1)
+    elems = passes = pass_time = adapt_time = field_time = 0.0
+    limited = 0
+    for _ in range(cycles):
+        tf = time.perf_counter()
+        x = m.coords().reshape(-1, 3)
+        if m.tag(0, "metric", f64=True) is not None:
+            m.free_field(0, "metric")
+        if args.workload == "c3":
+            m.add_field(0, "hessian", W.tanh_front_hessian(x, a=50.0, s0=1.5, dim=3).ravel())
+            m.metric_from_hessian("hessian", 0.01, 0.002, 0.1)
+            m.free_field(0, "hessian")
+        else:
+            m.add_field(0, "metric", W.rotating_metric(x, 1e-3, 20.0, dim=3).ravel())
+        field_time += time.perf_counter() - tf
+        r, st = m.adapt_timed(1.0 / 3.0, 0.3, 4, max_ops)
+        limited += int(r < 0)
+        passes += st[0]
+        elems += st[1]
+        pass_time += st[2]
+        adapt_time += st[3]
+    dll.ohx_synchronize()
+    dll.ohx_profile_enable(0)
+    tab, host_sync = _profile_table(dll)
+    roof, per = _roofline(tab)
+    out = {
+        "metric": "elements/sec per adapt pass (%s, 1 GPU)" % ("FP32-mixed" if args.precision == "mixed" else "FP64"),
+        "value": round(elems / pass_time, 1) if pass_time else None,
+        "unit": "elements/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64" if args.precision == "fp64"
+        else "f32+f64", "data": "synthetic",
+        "config": {"workload": {"c3": "3D Kuhn box 64^3x6 = %d tets, tanh(50(x+y+z-1.5)) Hessian metric "
+                                      "(eps 0.01, h in [0.002, 0.1]), 8 cycles of mesh_adapt (<=50 ops)",
+                                "c5": "3D Kuhn box 160^3x6 = %d tets, rotating metric 1e-3 x 20:1, first 8 "
+                                      "passes of mesh_adapt"}[args.workload] % t_in0,
+                   "precision": args.precision, "cycles": cycles, "cycles_at_op_limit": limited,
+                   "passes": passes, "elements_over_passes": elems, "pass_seconds": round(pass_time, 4),
+                   "adapt_seconds": round(adapt_time, 4), "host_field_seconds": round(field_time, 4),
+                   "setup_seconds": round(t_setup, 4), "final_counts": m.counts(),
+                   "final_min_quality": m.min_quality()},
+        "roofline": roof,
+        "kernels": per,
+        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
+        "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
+        "final_digest": digest(m) if args.workload == "c3" else None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", choices=["c2", "c3", "c5"], default="c2")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole mesh lives in one GPU's HBM); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     lib = ML.load("product")
     if lib is None:
         sys.exit("bench.py: product library not built (run __graft_entry__.build())")
     dll = lib.dll
     _set_profile_api(dll)
+    if args.workload != "c2":
+        run_large(args, lib)
+        return
     base = prepare(lib, True)
     t_in = base.count(2)
 

@@ -161,6 +161,11 @@ void ohx_set_verbose(int on) OSH_PUBLIC;
  * whose equal-goodness candidates form long chains (each round settles at
  * least the lowest-numbered candidate of a tie chain) */
 void ohx_set_indset_max_rounds(unsigned n) OSH_PUBLIC;
+/* 0: the FP64 path (default, the reference's arithmetic); 1: FP32-mixed --
+ * qualities and metric lengths are screened in FP32 and every value within
+ * a margin of a decision (a threshold or a competing value) is re-evaluated
+ * in FP64, so every decision equals the FP64 path's (extension) */
+void ohx_set_precision(int mixed) OSH_PUBLIC;
 /* per-kernel timing: while enabled, each timed launch carries a start and a
  * stop event in its own dispatch (hipExtLaunchKernel), so the interval is
  * the kernel's execution alone */

@@ -15,6 +15,8 @@ namespace ohx {
 int g_verbose = 0;
 // indset.cpp:33 stops after 100 rounds; raising it is an extension
 u32 g_indset_max_rounds = 100;
+// FP32 screening with FP64 re-evaluation near every decision (extension)
+bool g_mixed = false;
 
 static double now_s() {
   using namespace std::chrono;

@@ -30,5 +30,6 @@ void swap_int_edges_host(u32 ring_size, u32 code, u32* out);
 
 extern int g_verbose;
 extern u32 g_indset_max_rounds;
+extern bool g_mixed;
 
 }  // namespace ohx

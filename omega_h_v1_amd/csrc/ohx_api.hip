@@ -496,6 +496,7 @@ unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qf) {
 
 osh_t ohx_mesh_osh(ohx_mesh* m) { return reinterpret_cast<osh_t>(m); }
 
+void ohx_set_precision(int mixed) { g_mixed = mixed != 0; }
 void ohx_set_indset_max_rounds(unsigned n) { g_indset_max_rounds = n ? n : 100; }
 void ohx_set_verbose(int on) { g_verbose = on; }
 void ohx_profile_enable(int on) { prof_enable(on != 0); }

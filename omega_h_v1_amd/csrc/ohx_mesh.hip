@@ -247,26 +247,27 @@ __global__ __launch_bounds__(BLOCK) void upfd_fill(u32 nuses, u32 const* __restr
   out[offs[d] + j] = i;
 }
 
-__global__ __launch_bounds__(BLOCK) void upfd_sort_split(u32 nlows, u32 deg, u32 const* __restrict__ offs,
-                                                         u32* __restrict__ adj, u32* __restrict__ dirs) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= nlows) return;
-  u32 f = offs[i], e = offs[i + 1];
-  // insertion sort of the bucket (a dozen to a few dozen entries)
-  for (u32 j = f + 1; j < e; ++j) {
-    u32 v = adj[j];
-    u32 k = j;
-    while (k > f && adj[k - 1] > v) {
-      adj[k] = adj[k - 1];
-      --k;
-    }
-    adj[k] = v;
-  }
-  for (u32 j = f; j < e; ++j) {
-    u32 both = adj[j];
-    u32 h = both / deg;
-    adj[j] = h;
-    dirs[j] = both - h * deg;
+// G lanes per low entity: each use's final slot is its rank among the
+// bucket's uses (use indices are distinct, so the rank is the count of
+// smaller ones); the bucket is read from the fill buffer and written sorted,
+// split into (high, direction).  Lanes of a group read the same word at a
+// time (one transaction per group), and a bucket costs k^2 / G compares.
+template <u32 G>
+__global__ __launch_bounds__(BLOCK) void upfd_rank_split(u32 nlows, u32 deg, u32 const* __restrict__ offs,
+                                                         u32 const* __restrict__ tmp, u32* __restrict__ adj,
+                                                         u32* __restrict__ dirs) {
+  size_t gid = (size_t) blockIdx.x * BLOCK + threadIdx.x;
+  size_t grp = gid / G;
+  u32 lane = (u32) (gid % G);
+  if (grp >= nlows) return;
+  u32 f = offs[grp], e = offs[grp + 1];
+  for (u32 j = f + lane; j < e; j += G) {
+    u32 v = tmp[j];
+    u32 r = 0;
+    for (u32 k = f; k < e; ++k) r += tmp[k] < v ? 1u : 0u;
+    u32 h = v / deg;
+    adj[f + r] = h;
+    dirs[f + r] = v - h * deg;
   }
 }
 
@@ -281,32 +282,87 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
                  dim3(BLOCK), nuses, lows_of_highs, counts.get());
   }
   u.offsets = exscan(counts.get(), nlows);
-  u.adj = Buf<u32>(nuses);
-  u.dirs = Buf<u32>(nuses);
+  Buf<u32> tmp(nuses);
   if (nuses) {
     // down map and uses out, offsets and counters
     launch_timed("upfd_fill", (double) nuses * 8 + (double) nlows * 12, upfd_fill, dim3(nblocks(nuses)),
-                 dim3(BLOCK), nuses, lows_of_highs, u.offsets.get(), counts.get(), u.adj.get());
+                 dim3(BLOCK), nuses, lows_of_highs, u.offsets.get(), counts.get(), tmp.get());
   }
-  if (nlows) {
-    // offsets, uses in, (high, direction) out
-    launch_timed("upfd_sort_split", (double) (nlows + 1) * 4 + (double) nuses * 12, upfd_sort_split,
-                 dim3(nblocks(nlows)), dim3(BLOCK), nlows, deg, u.offsets.get(), u.adj.get(), u.dirs.get());
+  counts.release();
+  u.adj = Buf<u32>(nuses);
+  u.dirs = Buf<u32>(nuses);
+  if (nlows && nuses) {
+    // offsets, uses in, (high, direction) out; lanes per bucket from the
+    // mean bucket size
+    double bytes = (double) (nlows + 1) * 4 + (double) nuses * 12;
+    u32 mean = (nuses + nlows - 1) / nlows;
+    u32 const* o = u.offsets.get();
+    if (mean <= 4)
+      launch_timed("upfd_rank_split", bytes, upfd_rank_split<4>, dim3(nblocks((size_t) nlows * 4)), dim3(BLOCK),
+                   nlows, deg, o, tmp.get(), u.adj.get(), u.dirs.get());
+    else if (mean <= 8)
+      launch_timed("upfd_rank_split", bytes, upfd_rank_split<8>, dim3(nblocks((size_t) nlows * 8)), dim3(BLOCK),
+                   nlows, deg, o, tmp.get(), u.adj.get(), u.dirs.get());
+    else
+      launch_timed("upfd_rank_split", bytes, upfd_rank_split<16>, dim3(nblocks((size_t) nlows * 16)),
+                   dim3(BLOCK), nlows, deg, o, tmp.get(), u.adj.get(), u.dirs.get());
   }
 }
 
 // ---------------------------------------------------------- reflect_down
-// For every high entity and each of its canonical lows, find the low with
-// those vertices among the lows adjacent to the first vertex
-// (reflect_down.cpp:13-81, find_by_verts.hpp:14-76).  The answer is unique
-// in a conforming mesh, so the search order does not matter.
+// For every high entity and each of its canonical lows, the low with those
+// vertices (reflect_down.cpp:13-81, find_by_verts.hpp:14-76).  The answer is
+// unique in a conforming mesh, so any search finds the reference's: here an
+// open-addressing hash table keyed by the sorted vertex tuple (lows inserted
+// once, two to four probes per lookup) replaces the scan of the first
+// vertex's upward adjacency, whose random reads grow with vertex valence.
+template <u32 N>
+__device__ inline void sort_small(u32 (&v)[N]) {
+#pragma unroll
+  for (u32 i = 1; i < N; ++i)
+#pragma unroll
+    for (u32 j = i; j > 0; --j)
+      if (v[j - 1] > v[j]) {
+        u32 t = v[j - 1];
+        v[j - 1] = v[j];
+        v[j] = t;
+      }
+}
+
+template <u32 N>
+__device__ inline u32 tuple_hash(u32 const (&v)[N], u32 mask) {
+  unsigned long long h = 0x9E3779B97F4A7C15ull;
+#pragma unroll
+  for (u32 i = 0; i < N; ++i) {
+    h ^= v[i] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h *= 0xBF58476D1CE4E5B9ull;
+  }
+  h ^= h >> 31;
+  return (u32) h & mask;
+}
+
+template <u32 VPL>
+__global__ __launch_bounds__(BLOCK) void lows_hash_insert(u32 nl, u32 const* __restrict__ vl, u32 mask,
+                                                          u32* __restrict__ table, u32* __restrict__ err) {
+  u32 e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= nl) return;
+  u32 v[VPL];
+#pragma unroll
+  for (u32 k = 0; k < VPL; ++k) v[k] = vl[(size_t) e * VPL + k];
+  sort_small(v);
+  u32 slot = tuple_hash(v, mask);
+  for (u32 probe = 0; probe <= mask; ++probe) {
+    if (atomicCAS(&table[slot], INVALID, e) == INVALID) return;
+    slot = (slot + 1) & mask;
+  }
+  *err = ERR_REFLECT_NOT_FOUND;
+}
+
 template <u32 HD, u32 LD>
-__global__ __launch_bounds__(BLOCK) void reflect_down_kernel(u32 nh, u32 const* __restrict__ vh,
-                                                             u32 const* __restrict__ vl,
-                                                             u32 const* __restrict__ uoff,
-                                                             u32 const* __restrict__ uadj,
-                                                             u32 const* __restrict__ udir,
-                                                             u32* __restrict__ out, u32* __restrict__ err) {
+__global__ __launch_bounds__(BLOCK) void reflect_down_hash(u32 nh, u32 const* __restrict__ vh,
+                                                           u32 const* __restrict__ vl, u32 mask,
+                                                           u32 const* __restrict__ table, u32* __restrict__ out,
+                                                           u32* __restrict__ err) {
   u32 i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nh) return;
   constexpr u32 VPH = HD + 1, VPL = LD + 1;
@@ -317,55 +373,70 @@ __global__ __launch_bounds__(BLOCK) void reflect_down_kernel(u32 nh, u32 const* 
 #pragma unroll
   for (u32 j = 0; j < LPH; ++j) {
     u32 const* lc = canon(HD, LD, 0, j);
-    u32 w0 = hv[lc[0]], w1 = hv[lc[1]], w2 = (VPL == 3) ? hv[lc[VPL - 1]] : 0u;
+    u32 w[VPL];
+#pragma unroll
+    for (u32 k = 0; k < VPL; ++k) w[k] = hv[lc[k]];
+    sort_small(w);
+    u32 slot = tuple_hash(w, mask);
     u32 found = INVALID;
-    for (u32 k = uoff[w0]; k < uoff[w0 + 1]; ++k) {
-      u32 e = uadj[k], d = udir[k];
-      if (VPL == 2) {
-        if (vl[2 * (size_t) e + (1 - d)] == w1) {
-          found = e;
-          break;
-        }
-      } else {
-        u32 const* t = vl + 3 * (size_t) e;
-        u32 a = t[(d + 1) % 3], b = t[(d + 2) % 3];
-        if ((a == w1 && b == w2) || (a == w2 && b == w1)) {
-          found = e;
-          break;
-        }
+    for (u32 probe = 0; probe <= mask; ++probe) {
+      u32 e = table[slot];
+      if (e == INVALID) break;
+      u32 t[VPL];
+#pragma unroll
+      for (u32 k = 0; k < VPL; ++k) t[k] = vl[(size_t) e * VPL + k];
+      sort_small(t);
+      bool same = true;
+#pragma unroll
+      for (u32 k = 0; k < VPL; ++k) same = same && (t[k] == w[k]);
+      if (same) {
+        found = e;
+        break;
       }
+      slot = (slot + 1) & mask;
     }
     if (found == INVALID) *err = ERR_REFLECT_NOT_FOUND;
     out[(size_t) i * LPH + j] = found;
   }
 }
 
-static u32 read_nuses(Up const& u, u32 n) { return u.adj.size(); }
-
 static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   u32 nh = count(m, hd);
+  u32 nl = count(m, ld);
   u32 const* vh = ask_down(m, hd, 0);
-  Up const& lv = ask_up(m, 0, ld);
   u32 const* vl = ask_down(m, ld, 0);
   u32 lph = h_down_deg[hd][ld];
   Buf<u32> out((size_t) nh * lph);
   if (!nh) return out;
   u32* err = dev_error_word();
-  // compulsory bytes: high vertices in, low ids out, lows-of-vertex CSR and
-  // low vertices read once
-  double prof_bytes = (double) nh * ((hd + 1) * 4 + lph * 4) +
-                                   (double) (count(m, 0) + 1) * 4 + (double) read_nuses(lv, count(m, 0)) * 8 +
-                                   (double) count(m, ld) * (ld + 1) * 4;
+  // table of at least twice as many slots as lows (load factor <= 1/2)
+  u32 cap = 1024;
+  while (cap < 2 * (size_t) nl) cap *= 2;
+  u32 mask = cap - 1;
+  Buf<u32> table = filled(cap, INVALID);
+  // compulsory bytes: low vertices in, the table written once
+  double ins_bytes = (double) nl * (ld + 1) * 4 + (double) cap * 4;
+  if (nl) {
+    if (ld == 1)
+      launch_timed("lows_hash_insert", ins_bytes, lows_hash_insert<2>, dim3(nblocks(nl)), dim3(BLOCK), nl, vl, mask,
+                   table.get(), err);
+    else
+      launch_timed("lows_hash_insert", ins_bytes, lows_hash_insert<3>, dim3(nblocks(nl)), dim3(BLOCK), nl, vl, mask,
+                   table.get(), err);
+  }
+  // compulsory bytes: high vertices in, low ids out, the table and the low
+  // vertices read once
+  double prof_bytes = (double) nh * ((hd + 1) * 4 + lph * 4) + (double) cap * 4 + (double) nl * (ld + 1) * 4;
   dim3 g(nblocks(nh)), b(BLOCK);
   if (hd == 3 && ld == 1)
-    launch_timed("reflect_down", prof_bytes, reflect_down_kernel<3, 1>, g, b, nh, vh, vl, lv.offsets.get(),
-                       lv.adj.get(), lv.dirs.get(), out.get(), err);
+    launch_timed("reflect_down", prof_bytes, reflect_down_hash<3, 1>, g, b, nh, vh, vl, mask, table.get(), out.get(),
+                 err);
   else if (hd == 3 && ld == 2)
-    launch_timed("reflect_down", prof_bytes, reflect_down_kernel<3, 2>, g, b, nh, vh, vl, lv.offsets.get(),
-                       lv.adj.get(), lv.dirs.get(), out.get(), err);
+    launch_timed("reflect_down", prof_bytes, reflect_down_hash<3, 2>, g, b, nh, vh, vl, mask, table.get(), out.get(),
+                 err);
   else if (hd == 2 && ld == 1)
-    launch_timed("reflect_down", prof_bytes, reflect_down_kernel<2, 1>, g, b, nh, vh, vl, lv.offsets.get(),
-                       lv.adj.get(), lv.dirs.get(), out.get(), err);
+    launch_timed("reflect_down", prof_bytes, reflect_down_hash<2, 1>, g, b, nh, vh, vl, mask, table.get(), out.get(),
+                 err);
   else
     fail("reflect_down: unsupported dimensions", __FILE__, __LINE__);
   return out;
