@@ -196,6 +196,16 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy_bytes": COPY_BYTES, "copies": 5}))
 
 
+def tuning_report(dll):
+    f = dll.ohx_tuning_report
+    f.restype = C.c_uint
+    f.argtypes = [C.c_char_p, C.c_uint]
+    n = f(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    f(buf, n + 1)
+    return [x for x in buf.value.decode().split("\n") if x]
+
+
 def _profile_table(dll):
     tab = {name: profile_get(dll, name) for name in profile_names(dll)}
     host_sync = tab.pop("host_sync", None)
@@ -218,18 +228,18 @@ def _roofline(tab):
     return roof, per
 
 
-def run_large(args, lib):
-    """The 3D anisotropic workloads (c3, c5); every kernel timed by
-    dispatch-carried events over the whole run."""
+def _large_run(lib, workload, mixed):
+    """One run of c3 or c5 from a fresh mesh; every kernel timed by
+    dispatch-carried events."""
     dll = lib.dll
-    dll.ohx_set_precision.argtypes = [C.c_int]
-    dll.ohx_set_precision(1 if args.precision == "mixed" else 0)
-    dll.ohx_device_bytes_high_water.restype = C.c_ulonglong
+    dll.ohx_set_precision(1 if mixed else 0)
+    dll.ohx_mixed_stats_reset()
     dll.ohx_set_indset_max_rounds(2000)
-    if args.workload == "c3":
+    if workload == "c3":
         n, cycles, max_ops = 64, 8, 50
     else:
         n, cycles, max_ops = 160, 1, 8
+    dll.ohx_device_reset_high_water()
     t0 = time.perf_counter()
     m = ML.Mesh.box_on_device(lib, 3, n)
     m.derive_model_full()
@@ -246,7 +256,7 @@ def run_large(args, lib):
         x = m.coords().reshape(-1, 3)
         if m.tag(0, "metric", f64=True) is not None:
             m.free_field(0, "metric")
-        if args.workload == "c3":
+        if workload == "c3":
             m.add_field(0, "hessian", W.tanh_front_hessian(x, a=50.0, s0=1.5, dim=3).ravel())
             m.metric_from_hessian("hessian", 0.01, 0.002, 0.1)
             m.free_field(0, "hessian")
@@ -263,25 +273,61 @@ def run_large(args, lib):
     dll.ohx_profile_enable(0)
     tab, host_sync = _profile_table(dll)
     roof, per = _roofline(tab)
-    out = {
-        "metric": "elements/sec per adapt pass (%s, 1 GPU)" % ("FP32-mixed" if args.precision == "mixed" else "FP64"),
-        "value": round(elems / pass_time, 1) if pass_time else None,
-        "unit": "elements/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64" if args.precision == "fp64"
-        else "f32+f64", "data": "synthetic",
-        "config": {"workload": {"c3": "3D Kuhn box 64^3x6 = %d tets, tanh(50(x+y+z-1.5)) Hessian metric "
-                                      "(eps 0.01, h in [0.002, 0.1]), 8 cycles of mesh_adapt (<=50 ops)",
-                                "c5": "3D Kuhn box 160^3x6 = %d tets, rotating metric 1e-3 x 20:1, first 8 "
-                                      "passes of mesh_adapt"}[args.workload] % t_in0,
-                   "precision": args.precision, "cycles": cycles, "cycles_at_op_limit": limited,
-                   "passes": passes, "elements_over_passes": elems, "pass_seconds": round(pass_time, 4),
-                   "adapt_seconds": round(adapt_time, 4), "host_field_seconds": round(field_time, 4),
-                   "setup_seconds": round(t_setup, 4), "final_counts": m.counts(),
-                   "final_min_quality": m.min_quality()},
-        "roofline": roof,
-        "kernels": per,
+    err, screened, reeval = C.c_double(), C.c_ulonglong(), C.c_ulonglong()
+    dll.ohx_mixed_stats(C.byref(err), C.byref(screened), C.byref(reeval))
+    dll.ohx_set_precision(0)
+    return {
+        "value": elems / pass_time if pass_time else None,
+        "t_in0": t_in0, "cycles": cycles, "cycles_at_op_limit": limited, "passes": passes,
+        "elements_over_passes": elems, "pass_seconds": round(pass_time, 4), "adapt_seconds": round(adapt_time, 4),
+        "host_field_seconds": round(field_time, 4), "setup_seconds": round(t_setup, 4),
+        "final_counts": m.counts(), "final_min_quality": m.min_quality(),
+        "roofline": roof, "kernels": per,
         "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
         "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
-        "final_digest": digest(m) if args.workload == "c3" else None,
+        "digest": digest(m) if workload == "c3" else None,
+        "mixed": {"max_abs_q32_minus_q64": err.value, "decisions_in_fp32": screened.value,
+                  "fp64_reevaluations": reeval.value} if mixed else None,
+    }
+
+
+def run_large(args, lib):
+    """c3: the FP64 path and the FP32-mixed path from the same input, with
+    their final meshes compared; c5: one run at the real shape."""
+    dll = lib.dll
+    dll.ohx_set_precision.argtypes = [C.c_int]
+    dll.ohx_mixed_stats.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+    dll.ohx_device_bytes_high_water.restype = C.c_ulonglong
+    desc = {"c3": "3D Kuhn box 64^3x6 = %d tets, tanh(50(x+y+z-1.5)) Hessian metric (eps 0.01, h in [0.002, 0.1]), "
+                  "8 cycles of mesh_adapt (<=50 ops each)",
+            "c5": "3D Kuhn box 160^3x6 = %d tets, rotating metric 1e-3 x 20:1, first 8 passes of mesh_adapt"}
+    if args.workload == "c3":
+        runs = {"fp64": _large_run(lib, "c3", False), "mixed": _large_run(lib, "c3", True)}
+        main_run = runs["mixed"] if args.precision == "mixed" else runs["fp64"]
+        cmp = {"fp64_value": runs["fp64"]["value"], "mixed_value": runs["mixed"]["value"],
+               "mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
+               "identical_final_mesh": runs["fp64"]["digest"] == runs["mixed"]["digest"],
+               "equal_final_min_quality": runs["fp64"]["final_min_quality"] == runs["mixed"]["final_min_quality"],
+               "mixed_stats": runs["mixed"]["mixed"]}
+    else:
+        main_run = _large_run(lib, "c5", args.precision == "mixed")
+        cmp = None
+    r = main_run
+    out = {
+        "metric": "elements/sec per adapt pass (%s, 1 GPU)" % ("FP32-mixed" if args.precision == "mixed" else "FP64"),
+        "value": round(r["value"], 1) if r["value"] else None,
+        "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": dict({"workload": desc[args.workload] % r["t_in0"], "precision": args.precision},
+                       **{k: r[k] for k in ("cycles", "cycles_at_op_limit", "passes", "elements_over_passes",
+                                            "pass_seconds", "adapt_seconds", "host_field_seconds", "setup_seconds",
+                                            "final_counts", "final_min_quality")}),
+        "fp64_vs_mixed": cmp,
+        "roofline": r["roofline"],
+        "kernels": r["kernels"],
+        "host_sync": r["host_sync"],
+        "peak_device_bytes": r["peak_device_bytes"],
+        "autotune": tuning_report(dll),
     }
     print(json.dumps(out))
 
@@ -398,6 +444,7 @@ def main():
         "kernels_note": "one untimed step with every kernel timed by dispatch-carried events",
         "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
         "peak_device_bytes": int(peak_bytes(dll)),
+        "autotune": tuning_report(dll),
     }
     print(json.dumps(out))
 

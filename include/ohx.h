@@ -166,6 +166,17 @@ void ohx_set_indset_max_rounds(unsigned n) OSH_PUBLIC;
  * a margin of a decision (a threshold or a competing value) is re-evaluated
  * in FP64, so every decision equals the FP64 path's (extension) */
 void ohx_set_precision(int mixed) OSH_PUBLIC;
+/* FP32-mixed statistics since the last reset: the largest |q32 - q64| seen
+ * where both were computed, decisions settled in FP32, FP64 re-evaluations */
+void ohx_mixed_stats(double* max_err, unsigned long long* screened,
+    unsigned long long* reeval) OSH_PUBLIC;
+void ohx_mixed_stats_reset(void) OSH_PUBLIC;
+/* the online autotuner's choices so far, "kernel:2^class=variant" per line
+ * (variant: lanes per item); returns the full length */
+unsigned ohx_tuning_report(char* buf, unsigned cap) OSH_PUBLIC;
+/* pin the tuner of that name (as in the report) to one of its variants;
+ * variant < 0 lets it tune again.  Every variant gives identical results. */
+void ohx_tuning_force(char const* name, int variant) OSH_PUBLIC;
 /* per-kernel timing: while enabled, each timed launch carries a start and a
  * stop event in its own dispatch (hipExtLaunchKernel), so the interval is
  * the kernel's execution alone */

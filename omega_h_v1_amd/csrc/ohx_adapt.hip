@@ -24,7 +24,33 @@ static double now_s() {
 }
 
 // scratch word slots (zeroed by a fill kernel before each use)
-enum { W_ANY = 0, W_FLAGS = 8 };
+enum { W_ANY = 0, W_FLAGS = 8, W_STATS = 40 };
+// FP32-mixed statistics: u64 at words 40-41 (decisions settled in FP32),
+// u64 at 42-43 (FP64 re-evaluations), float bits at 44 (max |q32 - q64|)
+struct ScreenStats {
+  unsigned long long* screened;
+  unsigned long long* reeval;
+  u32* max_err;
+};
+static ScreenStats screen_stats() {
+  u32* w = scratch_words() + W_STATS;
+  return ScreenStats{reinterpret_cast<unsigned long long*>(w), reinterpret_cast<unsigned long long*>(w + 2), w + 4};
+}
+
+void mixed_stats(double* max_err, unsigned long long* screened, unsigned long long* reeval) {
+  u32 h[5];
+  to_host(h, scratch_words() + W_STATS, 5);
+  unsigned long long a, b;
+  std::memcpy(&a, h, 8);
+  std::memcpy(&b, h + 2, 8);
+  float e;
+  std::memcpy(&e, h + 4, 4);
+  if (max_err) *max_err = e;
+  if (screened) *screened = a;
+  if (reeval) *reeval = b;
+}
+
+void mixed_stats_reset() { fill(scratch_words() + W_STATS, 5, 0); }
 
 static u32 read_word(u32 k) { return read_at(scratch_words(), k); }
 static void zero_words(u32 first, u32 n) { fill(scratch_words() + first, n, 0); }
@@ -41,7 +67,7 @@ __global__ __launch_bounds__(BLOCK) void indset_round(u32 n, u32 const* __restri
                                                       double const* __restrict__ good,
                                                       u32 const* __restrict__ old, u32* __restrict__ state,
                                                       u32* __restrict__ flag) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   u32 s = old[i];
   if (s != UNKNOWN) {
@@ -107,13 +133,13 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
 // --------------------------------------------------------------- refine
 // Worst child quality of every candidate edge split
 // (refine_qualities.cpp:16-82); rejected candidates are cleared.
-template <u32 DIM>
+template <u32 DIM, bool MIXED>
 __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     u32 ns, u32 const* __restrict__ vs, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ ve, double const* __restrict__ x,
     double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor, u32* __restrict__ cand,
-    double* __restrict__ out, u32* __restrict__ words, u32* __restrict__ err) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+    double* __restrict__ out, u32* __restrict__ words, u32* __restrict__ err, ScreenStats st) {
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ns || !cand[i]) return;
   constexpr u32 VPE = DIM + 1;
   V3 a = ld3(x + 3 * (size_t) vs[2 * i]), b = ld3(x + 3 * (size_t) vs[2 * i + 1]);
@@ -137,8 +163,44 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
       msplit.m[k] = s;
     }
   }
-  double minq = 1, old_minq = 1;
   u32 const* bo = opp(DIM, 0);
+  float minq32 = 1.f;
+  if (MIXED && !eq) {
+    // FP32 screen: a split certainly below the floor is rejected here; one
+    // with a child near zero goes to FP64, which also checks q > 0
+    bool sure = true;
+    for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
+      u32 dir = ud[j], e = ua[j];
+      u32 const* evs = canon(DIM, 1, 0, dir);
+      for (u32 k = 0; k < 2; ++k) {
+        u32 const* bv = canon(DIM, DIM - 1, 0, bo[evs[k]]);
+        V3 p[VPE];
+#pragma unroll
+        for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) ve[(size_t) e * VPE + bv[l]]);
+        p[VPE - 1] = split;
+        float q;
+        if (mt) {
+          Sym pm[VPE];
+#pragma unroll
+          for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) ve[(size_t) e * VPE + bv[l]]);
+          pm[VPE - 1] = msplit;
+          q = quality32<DIM>(p, pm, false);
+        } else {
+          q = quality32<DIM>(p, nullptr, false);
+        }
+        if (!(q > SCREEN_MARGIN)) sure = false;
+        minq32 = q < minq32 ? q : minq32;
+      }
+    }
+    bool reject = sure && (minq32 < (float) qual_floor - SCREEN_MARGIN);
+    wave_count(st.screened, reject);
+    if (reject) {
+      cand[i] = 0;
+      return;
+    }
+    wave_count(st.reeval, true);
+  }
+  double minq = 1, old_minq = 1;
   for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
     u32 dir = ud[j], e = ua[j];
     if (eq && eq[e] < old_minq) old_minq = eq[e];
@@ -146,19 +208,26 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     for (u32 k = 0; k < 2; ++k) {
       u32 const* bv = canon(DIM, DIM - 1, 0, bo[evs[k]]);
       V3 p[VPE];
-      Sym pm[VPE];
 #pragma unroll
-      for (u32 l = 0; l + 1 < VPE; ++l) {
-        u32 v = ve[(size_t) e * VPE + bv[l]];
-        p[l] = ld3(x + 3 * (size_t) v);
-        if (mt) pm[l] = ld6(mt + 6 * (size_t) v);
-      }
+      for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) ve[(size_t) e * VPE + bv[l]]);
       p[VPE - 1] = split;
-      pm[VPE - 1] = msplit;
-      double q = mt ? metric_quality<DIM>(p, pm, false) : entity_quality(DIM, p);
+      double q;
+      if (mt) {
+        Sym pm[VPE];
+#pragma unroll
+        for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) ve[(size_t) e * VPE + bv[l]]);
+        pm[VPE - 1] = msplit;
+        q = metric_quality<DIM>(p, pm, false);
+      } else {
+        q = entity_quality(DIM, p);
+      }
       if (!(q > 0)) *err = ERR_REFINE_QUALITY;
       if (q < minq) minq = q;
     }
+  }
+  if (MIXED && !eq && minq == minq) {
+    float d = (float) minq - minq32;
+    note_max_err(st.max_err, d < 0 ? -d : d);
   }
   if ((minq < qual_floor) || (eq && (minq <= old_minq)))
     cand[i] = 0;
@@ -174,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void refine_candidate_f64(u32 n, u32 const* 
                                                               double const* __restrict__ h,
                                                               double const* __restrict__ mt,
                                                               u32* __restrict__ cand, u32* __restrict__ words) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   u32 c = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt) > 1.0 ? 1u : 0u;
   cand[i] = c;
@@ -192,14 +261,34 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   Buf<double> out(ns);
   zero_words(0, 1);
   double prof_bytes = (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24;
-  if (dim == 3)
-    launch_timed("refine_quality_f64", prof_bytes, refine_quality_f64<3>, dim3(nblocks(ns)), dim3(BLOCK), ns, vs,
-                       u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(),
-                       qual_floor, cand, out.get(), scratch_words(), dev_error_word());
+  ScreenStats sst = screen_stats();
+  // in the mixed mode the FP32 screen is a variant chosen online (it pays
+  // only where it rejects enough splits); both give identical results
+  static Tuner tuners[2] = {Tuner("refine_quality_2d_mixed", {0, 1}), Tuner("refine_quality_3d_mixed", {0, 1})};
+  auto go = [&](auto kernel, char const* kn, hipEvent_t ea, hipEvent_t eb) {
+    if (ea) {
+      hipExtLaunchKernelGGL(kernel, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ea, eb, 0, ns, vs, u.offsets.get(),
+                            u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, cand,
+                            out.get(), scratch_words(), dev_error_word(), sst);
+      check_launch(kn);
+    } else {
+      launch_timed(kn, prof_bytes, kernel, dim3(nblocks(ns)), dim3(BLOCK), ns, vs, u.offsets.get(), u.adj.get(),
+                   u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, cand, out.get(), scratch_words(),
+                   dev_error_word(), sst);
+    }
+  };
+  auto run = [&](u32 screen, hipEvent_t ea, hipEvent_t eb) {
+    if (dim == 3)
+      screen ? go(refine_quality_f64<3, true>, "refine_quality_mixed", ea, eb)
+             : go(refine_quality_f64<3, false>, "refine_quality_f64", ea, eb);
+    else
+      screen ? go(refine_quality_f64<2, true>, "refine_quality_mixed", ea, eb)
+             : go(refine_quality_f64<2, false>, "refine_quality_f64", ea, eb);
+  };
+  if (g_mixed)
+    tuned(tuners[dim == 3], ns, run);
   else
-    launch_timed("refine_quality_f64", prof_bytes, refine_quality_f64<2>, dim3(nblocks(ns)), dim3(BLOCK), ns, vs,
-                       u.offsets.get(), u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(),
-                       qual_floor, cand, out.get(), scratch_words(), dev_error_word());
+    run(0, nullptr, nullptr);
   return out;
 }
 
@@ -227,7 +316,7 @@ static void tag_list_add(TagList& L, Tag& t, void* dst) {
 // One thread per output vertex, all tags in the same launch.
 __global__ __launch_bounds__(BLOCK) void refine_vertex_tags(u32 nv, u32 ns, u32 const* __restrict__ gen_off,
                                                             u32 const* __restrict__ vs, TagList L) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= nv + ns) return;
   u32 v0, v1;
   bool split = i >= nv;
@@ -310,7 +399,7 @@ struct RefineArgs {
 // refine_topology.cpp:26-107: the base opposite each of the edge's
 // opposite-dim entities + the new vertex), classified like the domain.
 __global__ __launch_bounds__(BLOCK) void refine_products(RefineArgs a) {
-  u32 t = blockIdx.x * BLOCK + threadIdx.x;
+  u32 t = xcd_block() * BLOCK + threadIdx.x;
   if (t >= a.seg[4]) return;
   u32 pd = a.pd;
   if (t < a.seg[0]) {
@@ -574,27 +663,84 @@ static void check_exposed_sides(Mesh& m, u32 ed, u32* codes) {
 }
 
 // coarsen_quality (coarsen_qualities.cpp:12-79)
-template <u32 DIM>
+// G lanes per edge: the elements around the collapsing vertex are split
+// over the group and the minima combined with shuffles (a minimum does not
+// depend on the order, so the result is the serial loop's).
+template <u32 G>
+__device__ inline double group_min(double v) {
+#pragma unroll
+  for (u32 d = G / 2; d >= 1; d >>= 1) {
+    double o = __shfl_xor(v, d, G);
+    if (o < v) v = o;
+  }
+  return v;
+}
+
+template <u32 DIM, u32 G, bool MIXED>
 __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
     u32 ne, u32 const* __restrict__ ev, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ vel, double const* __restrict__ x,
     double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor, u32* __restrict__ codes,
-    double* __restrict__ out, u32* __restrict__ words) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= ne) return;
+    double* __restrict__ out, u32* __restrict__ words, ScreenStats st) {
+  size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
+  size_t i = gid / G;
+  u32 lane = (u32) (gid % G);
+  if (i >= ne) return;  // whole groups leave together
   u32 c = codes[i];
   if (c == DONT_COLLAPSE) return;
   constexpr u32 VPE = DIM + 1;
   u32 const* bo = opp(DIM, 0);
   for (u32 j = 0; j < 2; ++j) {
     if (!collapses(c, j)) continue;
-    double minq = 1, old_minq = 1;
     u32 col = ev[2 * i + j], gen = ev[2 * i + (1 - j)];
-    V3 p[VPE];
-    Sym pm[VPE];
-    p[VPE - 1] = ld3(x + 3 * (size_t) gen);
-    if (mt) pm[VPE - 1] = ld6(mt + 6 * (size_t) gen);
-    for (u32 k = uo[col]; k < uo[col + 1]; ++k) {
+    V3 const pg = ld3(x + 3 * (size_t) gen);
+    Sym mg;
+    if (mt) mg = ld6(mt + 6 * (size_t) gen);
+    float minq32 = 1.f;
+    if (MIXED && !eq) {
+      // FP32 screen: a cavity certainly below the floor is rejected here
+      bool nan = false;
+      for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
+        u32 const* v = vel + (size_t) ua[k] * VPE;
+        bool adj = false;
+#pragma unroll
+        for (u32 l = 0; l < VPE; ++l)
+          if (v[l] == gen) adj = true;
+        if (adj) continue;
+        u32 const* bv = canon(DIM, DIM - 1, 0, bo[ud[k]]);
+        V3 p[VPE];
+        p[VPE - 1] = pg;
+#pragma unroll
+        for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
+        float q;
+        if (mt) {
+          Sym pm[VPE];
+          pm[VPE - 1] = mg;
+#pragma unroll
+          for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
+          q = quality32<DIM>(p, pm, true);
+        } else {
+          q = quality32<DIM>(p, nullptr, true);
+        }
+        if (!(q == q)) nan = true;
+        minq32 = q < minq32 ? q : minq32;
+      }
+#pragma unroll
+      for (u32 d = G / 2; d >= 1; d >>= 1) {
+        float o = __shfl_xor(minq32, d, G);
+        minq32 = o < minq32 ? o : minq32;
+        nan = nan || __shfl_xor((int) nan, d, G);
+      }
+      bool reject = !nan && (minq32 < (float) qual_floor - SCREEN_MARGIN);
+      if (lane == 0) wave_count(st.screened, reject);
+      if (reject) {
+        c = dont_collapse(c, j);
+        continue;
+      }
+      if (lane == 0) wave_count(st.reeval, true);
+    }
+    double minq = 1, old_minq = 1;
+    for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
       u32 e = ua[k];
       u32 const* v = vel + (size_t) e * VPE;
       if (eq && eq[e] < old_minq) old_minq = eq[e];
@@ -604,21 +750,37 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
         if (v[l] == gen) adj = true;
       if (adj) continue;
       u32 const* bv = canon(DIM, DIM - 1, 0, bo[ud[k]]);
+      V3 p[VPE];
+      p[VPE - 1] = pg;
 #pragma unroll
-      for (u32 l = 0; l + 1 < VPE; ++l) {
-        p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
-        if (mt) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
+      for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
+      double q;
+      if (mt) {
+        Sym pm[VPE];
+        pm[VPE - 1] = mg;
+#pragma unroll
+        for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
+        q = metric_quality<DIM>(p, pm, true);
+      } else {
+        q = element_quality(DIM, p);
       }
-      double q = mt ? metric_quality<DIM>(p, pm, true) : element_quality(DIM, p);
       if (q < minq) minq = q;
+    }
+    minq = group_min<G>(minq);
+    if (eq) old_minq = group_min<G>(old_minq);
+    if (MIXED && !eq && lane == 0 && minq == minq) {
+      float d = (float) minq - minq32;
+      note_max_err(st.max_err, d < 0 ? -d : d);
     }
     if ((minq < qual_floor) || (eq && (minq <= old_minq)))
       c = dont_collapse(c, j);
-    else
+    else if (lane == 0)
       out[2 * i + j] = minq;
   }
-  codes[i] = c;
-  if (c != DONT_COLLAPSE) words[W_ANY] = 1u;
+  if (lane == 0) {
+    codes[i] = c;
+    if (c != DONT_COLLAPSE) words[W_ANY] = 1u;
+  }
 }
 
 // coarsen_size_code (coarsen.cpp:13-21), fused with measure_edge
@@ -627,7 +789,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const*
                                                                double const* __restrict__ h,
                                                                double const* __restrict__ mt, double floor_,
                                                                u32* __restrict__ codes, u32* __restrict__ words) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   double sz = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
   u32 c = (sz < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
@@ -638,7 +800,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const*
 // tags_subset (subset.cpp:15-43): every tag of the surviving vertices,
 // one thread per old vertex, all tags in one launch
 __global__ __launch_bounds__(BLOCK) void subset_tags(u32 n, u32 const* __restrict__ soff, TagList L) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   u32 o = soff[i];
   if (o == soff[i + 1]) return;
@@ -677,7 +839,7 @@ struct CoarsenArgs {
 // first, then the entities rebuilt on the collapse target (the base
 // opposite the collapsing vertex, then the target), vertex ids renumbered.
 __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= a.n) return;
   u32 d = a.d, vpe = d + 1;
   u32 const* v = a.vent + (size_t) i * vpe;
@@ -713,14 +875,57 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   zero_words(0, 1);
   {
     double prof_bytes = (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28;
-    if (dim == 3)
-      launch_timed("coarsen_quality_f64", prof_bytes, coarsen_quality_f64<3>, dim3(nblocks(ne)), dim3(BLOCK), ne, ev,
-                         ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), metric_of(m), eq.get(),
-                         qual_floor, codes.get(), cq.get(), scratch_words());
-    else
-      launch_timed("coarsen_quality_f64", prof_bytes, coarsen_quality_f64<2>, dim3(nblocks(ne)), dim3(BLOCK), ne, ev,
-                         ue.offsets.get(), ue.adj.get(), ue.dirs.get(), vel, coords(m), metric_of(m), eq.get(),
-                         qual_floor, codes.get(), cq.get(), scratch_words());
+    // variants, chosen online per size class: lanes per edge (1, 2, 4 or 8)
+    // and, in the mixed mode, whether the FP32 screen runs (it pays only
+    // where it rejects enough candidates); all give identical results
+    static Tuner tuners[2][2] = {
+        {Tuner("coarsen_quality_2d_f64", {1, 2, 4, 8}), Tuner("coarsen_quality_2d_mixed", {1, 2, 4, 8, 17, 18, 20, 24})},
+        {Tuner("coarsen_quality_3d_f64", {1, 2, 4, 8}), Tuner("coarsen_quality_3d_mixed", {1, 2, 4, 8, 17, 18, 20, 24})}};
+    Tuner& tuner = tuners[dim == 3][g_mixed ? 1 : 0];
+    ScreenStats sst = screen_stats();
+    u32 const* uoff = ue.offsets.get();
+    u32 const* uadj = ue.adj.get();
+    u32 const* udir = ue.dirs.get();
+    double const* xs = coords(m);
+    double const* mts = metric_of(m);
+    auto go = [&](auto kernel, char const* kn, u32 g, hipEvent_t ea, hipEvent_t eb) {
+      dim3 grid(nblocks((size_t) ne * g));
+      if (ea) {
+        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ne, ev, uoff, uadj, udir, vel, xs,
+                              mts, eq.get(), qual_floor, codes.get(), cq.get(), scratch_words(), sst);
+        check_launch(kn);
+      } else {
+        launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ne, ev, uoff, uadj, udir, vel, xs, mts, eq.get(),
+                     qual_floor, codes.get(), cq.get(), scratch_words(), sst);
+      }
+    };
+    tuned(tuner, ne, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
+      u32 g = var & 15u;
+      bool screen = (var & 16u) != 0;
+      char const* kf = "coarsen_quality_f64";
+      char const* km = "coarsen_quality_mixed";
+      if (dim == 3 && screen) {
+        if (g == 1) go(coarsen_quality_f64<3, 1, true>, km, g, ea, eb);
+        else if (g == 2) go(coarsen_quality_f64<3, 2, true>, km, g, ea, eb);
+        else if (g == 4) go(coarsen_quality_f64<3, 4, true>, km, g, ea, eb);
+        else go(coarsen_quality_f64<3, 8, true>, km, g, ea, eb);
+      } else if (dim == 3) {
+        if (g == 1) go(coarsen_quality_f64<3, 1, false>, kf, g, ea, eb);
+        else if (g == 2) go(coarsen_quality_f64<3, 2, false>, kf, g, ea, eb);
+        else if (g == 4) go(coarsen_quality_f64<3, 4, false>, kf, g, ea, eb);
+        else go(coarsen_quality_f64<3, 8, false>, kf, g, ea, eb);
+      } else if (screen) {
+        if (g == 1) go(coarsen_quality_f64<2, 1, true>, km, g, ea, eb);
+        else if (g == 2) go(coarsen_quality_f64<2, 2, true>, km, g, ea, eb);
+        else if (g == 4) go(coarsen_quality_f64<2, 4, true>, km, g, ea, eb);
+        else go(coarsen_quality_f64<2, 8, true>, km, g, ea, eb);
+      } else {
+        if (g == 1) go(coarsen_quality_f64<2, 1, false>, kf, g, ea, eb);
+        else if (g == 2) go(coarsen_quality_f64<2, 2, false>, kf, g, ea, eb);
+        else if (g == 4) go(coarsen_quality_f64<2, 4, false>, kf, g, ea, eb);
+        else go(coarsen_quality_f64<2, 8, false>, kf, g, ea, eb);
+      }
+    });
   }
   eq.release();
   if (!read_word(W_ANY)) return 0;
@@ -1168,7 +1373,7 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
     u32 const* __restrict__ ev, u32 const* __restrict__ vt, double const* __restrict__ x,
     double const* __restrict__ mt, double const* __restrict__ eq, u32* __restrict__ cand,
     double* __restrict__ out, u32* __restrict__ rsz, u32* __restrict__ words, u32* __restrict__ err) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ne || !cand[i]) return;
   double old_minq = 1;
   for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
@@ -1219,7 +1424,7 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_f64(
     u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
     u32 const* __restrict__ vt, double const* __restrict__ x, double const* __restrict__ mt,
     u32* __restrict__ out) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ne || !indset[i]) return;
   u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
   u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
@@ -1274,7 +1479,7 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_f64(
 }
 
 __global__ __launch_bounds__(BLOCK) void copy_tags(u32 n, TagList L) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   for (u32 t = 0; t < L.n; ++t) {
     u32 w = L.ncomps[t];
@@ -1308,7 +1513,7 @@ struct SwapKeepArgs {
 // survivors (their own) and of products (their edge's, setup_swap,
 // inherit.cpp:158-166): threads [0, n) are entities, [n, n + ne) edges
 __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
-  u32 t = blockIdx.x * BLOCK + threadIdx.x;
+  u32 t = xcd_block() * BLOCK + threadIdx.x;
   if (t < a.n) {
     u32 o = a.soff[t];
     if (o == a.soff[t + 1]) return;

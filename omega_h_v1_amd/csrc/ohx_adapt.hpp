@@ -31,5 +31,7 @@ void swap_int_edges_host(u32 ring_size, u32 code, u32* out);
 extern int g_verbose;
 extern u32 g_indset_max_rounds;
 extern bool g_mixed;
+void mixed_stats(double* max_err, unsigned long long* screened, unsigned long long* reeval);
+void mixed_stats_reset();
 
 }  // namespace ohx

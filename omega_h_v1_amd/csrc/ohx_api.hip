@@ -497,6 +497,26 @@ unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qf) {
 osh_t ohx_mesh_osh(ohx_mesh* m) { return reinterpret_cast<osh_t>(m); }
 
 void ohx_set_precision(int mixed) { g_mixed = mixed != 0; }
+void ohx_mixed_stats(double* max_err, unsigned long long* screened, unsigned long long* reeval) {
+  guard_void([&]() { mixed_stats(max_err, screened, reeval); });
+}
+void ohx_mixed_stats_reset(void) {
+  guard_void([&]() { mixed_stats_reset(); });
+}
+void ohx_tuning_force(char const* name, int variant) {
+  guard_void([&]() { tuning_force(name, variant); });
+}
+unsigned ohx_tuning_report(char* buf, unsigned cap) {
+  return guard(0u, [&]() {
+    std::string r = tuning_report();
+    if (buf && cap) {
+      size_t n = r.size() < cap - 1 ? r.size() : cap - 1;
+      std::memcpy(buf, r.data(), n);
+      buf[n] = 0;
+    }
+    return (unsigned) r.size();
+  });
+}
 void ohx_set_indset_max_rounds(unsigned n) { g_indset_max_rounds = n ? n : 100; }
 void ohx_set_verbose(int on) { g_verbose = on; }
 void ohx_profile_enable(int on) { prof_enable(on != 0); }

@@ -184,12 +184,31 @@ void d2d(T* dst, T const* src, size_t n) {
 
 // ---------------------------------------------------------------- launch
 constexpr u32 BLOCK = 256;
+
+// The dispatcher hands consecutive workgroups to the 8 XCDs in turn, and
+// each XCD has its own L2.  xcd_block() can permute the block index so that
+// each XCD works on one contiguous eighth of the items (a bijection on
+// [0, gridDim.x)).  Measured on the adapt kernels (C3/C5 shapes) that was
+// 10-20% slower than the plain order -- neighbouring blocks on different XCDs
+// share the Infinity Cache and spread over more HBM channels at once -- so
+// the permutation is compiled out unless OHX_XCD_REMAP=1.
+constexpr u32 NXCD = 8;
+#ifndef OHX_XCD_REMAP
+#define OHX_XCD_REMAP 0
+#endif
+__device__ inline u32 xcd_block() {
+  if (!OHX_XCD_REMAP) return blockIdx.x;
+  u32 b = blockIdx.x, n = gridDim.x;
+  u32 x = b % NXCD, k = b / NXCD;
+  u32 per = n / NXCD, rem = n % NXCD;
+  return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
 inline u32 nblocks(size_t n, u32 block = BLOCK) { return static_cast<u32>((n + block - 1) / block); }
 void check_launch(char const* name);
 
 template <class F>
 __global__ __launch_bounds__(BLOCK) void for_kernel(u32 n, F f) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i < n) f(i);
 }
 // one thread per index; the body is a __device__ lambda capturing by value
@@ -230,5 +249,46 @@ void launch_timed(char const* name, double bytes, K kernel, dim3 grid, dim3 bloc
     check_launch(name);
   }
 }
+
+// ---------------------------------------------------------------- autotune
+// Online choice among launch variants that compute identical results (e.g.
+// lanes per item).  Per size class (log2 of the item count) every variant
+// is tried on live launches with start/stop events in its dispatch; once
+// each has been timed, later launches of that class use the fastest (time
+// per item).  Timings are read only when their events have completed, so
+// tuning adds no synchronisation.
+struct Tuner {
+  char const* name;
+  std::vector<u32> variants;
+  struct Trial {
+    u32 cls, v;
+    double items;
+    hipEvent_t a, b;
+  };
+  struct Cls {
+    std::vector<double> best;  // ms per item per variant, < 0 if untimed
+    int chosen = -1;
+    u32 next = 0;
+  };
+  std::vector<Cls> classes = std::vector<Cls>(33);
+  std::vector<Trial> pending;
+  std::vector<hipEvent_t> pool;
+  Tuner(char const* n, std::vector<u32> v) : name(n), variants(std::move(v)) {}
+  // the variant for n items; events to put in the dispatch (null if none)
+  u32 pick(u32 n, hipEvent_t* a, hipEvent_t* b);
+  void harvest();
+  int chosen(u32 n) const;
+};
+// run launch(variant, start_event, stop_event) with the tuner's choice
+template <class L>
+void tuned(Tuner& t, u32 n, L launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  u32 v = t.pick(n, &a, &b);
+  launch(v, a, b);
+}
+// tuning decisions so far, "name:class=variant" per line
+std::string tuning_report();
+// pin a tuner to one of its variants (variant < 0: tune again)
+void tuning_force(char const* name, int variant);
 
 }  // namespace ohx

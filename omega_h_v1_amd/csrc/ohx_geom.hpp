@@ -282,3 +282,86 @@ __host__ __device__ inline void metric_from_hessian(double const* h6, double eps
 }
 
 }  // namespace ohx
+
+namespace ohx {
+
+// ------------------------------------------------------------------------
+// FP32 screening (the FP32-mixed precision mode).  Edge vectors are formed
+// in FP64 (so no cancellation error reaches FP32) and everything after in
+// FP32 with the hardware's fast square root; the result differs from the
+// FP64 value by far less than SCREEN_MARGIN for any element the adapt loop
+// meets, and every decision whose FP32 input lies within the margin of its
+// threshold is re-evaluated in FP64.
+constexpr float SCREEN_MARGIN = 1e-4f;
+
+struct F3 {
+  float x, y, z;
+};
+__device__ inline F3 f3(V3 a, V3 b) { return F3{(float) (a.x - b.x), (float) (a.y - b.y), (float) (a.z - b.z)}; }
+__device__ inline F3 fsub(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline F3 fcross(F3 a, F3 b) {
+  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ inline float fdot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// the v1 measures from the edge vectors e1 = x1 - x0, e2 = x2 - x0 (, e3):
+// triangle  q = A / (L_rms^2) / (sqrt(3)/4),  tet  q = V / A_rms^{3/2} / c,
+// with the hardware's single-precision square root and reciprocals (about
+// one ulp each)
+__device__ inline float tri_q32(F3 e1, F3 e2, bool signed_z) {
+  F3 e12 = fsub(e2, e1);
+  float s = fdot(e1, e1) + fdot(e12, e12) + fdot(e2, e2);
+  F3 c = fcross(e1, e2);
+  float a2 = signed_z ? c.z : __builtin_amdgcn_sqrtf(fdot(c, c));  // twice the area
+  // A / (s / 3) / (sqrt(3) / 4) = (a2 / 2) * 12 / (sqrt(3) s)
+  return a2 * 3.46410162f * __builtin_amdgcn_rcpf(s);
+}
+__device__ inline float tet_q32(F3 e1, F3 e2, F3 e3) {
+  F3 c0 = fcross(e1, e2);                                   // face 0 1 2
+  F3 c1 = fcross(F3{-e1.x, -e1.y, -e1.z}, fsub(e3, e1));    // face 1 0 3
+  F3 c2 = fcross(fsub(e1, e2), fsub(e3, e2));               // face 2 1 3
+  F3 c3 = fcross(e2, e3);                                   // face 0 2 3
+  float s = fdot(c0, c0) + fdot(c1, c1) + fdot(c2, c2) + fdot(c3, c3);  // 4 * sum of A^2
+  float vol6 = fdot(c0, e3);
+  // A_rms^2 = s / 16, so A_rms^{3/2} = s^{3/4} / 8 and
+  // q = (vol6 / 6) * 8 s^{-3/4} / perfect_tet = vol6 s^{-3/4} * 3.2237097
+  float t = __builtin_amdgcn_rsqf(s);
+  return vol6 * (t * __builtin_amdgcn_sqrtf(t)) * 3.22370979f;
+}
+
+// quality of a simplex from its FP64 vertices (and per-vertex metrics).
+// The edge vectors, and with a metric the element metric's Cholesky factor
+// and the mapped edge vectors, are formed in FP64 exactly as the FP64 path
+// forms them: the factor of a strongly anisotropic metric is ill-conditioned
+// in FP32 (an error of cond(M) * 2^-24), the mapped vectors are not.  Only the
+// quality formula itself runs in FP32.
+template <u32 DIM>
+__device__ inline float quality32(V3 const* p, Sym const* pm, bool signed_z) {
+  V3 d1 = sub(p[1], p[0]), d2 = sub(p[2], p[0]);
+  V3 d3 = DIM == 3 ? sub(p[3], p[0]) : V3{0.0, 0.0, 0.0};
+  if (pm) {
+    Chol c = cholesky(sym_average<DIM + 1>(pm));
+    d1 = chol_map(c, d1);
+    d2 = chol_map(c, d2);
+    d3 = chol_map(c, d3);
+  }
+  F3 e1{(float) d1.x, (float) d1.y, (float) d1.z}, e2{(float) d2.x, (float) d2.y, (float) d2.z};
+  F3 e3{(float) d3.x, (float) d3.y, (float) d3.z};
+  return DIM == 3 ? tet_q32(e1, e2, e3) : tri_q32(e1, e2, signed_z);
+}
+
+// screening statistics in device words: decisions settled in FP32, FP64
+// re-evaluations, and the largest |q32 - q64| among re-evaluated values
+// (float bits, so unsigned order is float order)
+__device__ inline void wave_count(unsigned long long* w, bool pred) {
+  unsigned long long m = __ballot(pred);
+  unsigned long long act = __ballot(1);
+  if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) atomicAdd(w, (unsigned long long) __popcll(m));
+}
+__device__ inline void note_max_err(u32* w, float err) {
+  // err >= 0: the atomic is skipped once a larger value is recorded
+  u32 b = __float_as_uint(err);
+  if (b > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(w, b);
+}
+
+}  // namespace ohx

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(BLOCK) void elem_quality_f64(u32 n, u32 const* __re
                                                           double const* __restrict__ x,
                                                           double const* __restrict__ mt,
                                                           double* __restrict__ out) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   out[i] = elem_q<DIM>(i, ev, x, mt);
 }
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(BLOCK) void elem_min_quality_f64(u32 n, u32 begin,
                                                               double* __restrict__ part) {
   __shared__ double lds[BLOCK / 64];
   double v = 1.7976931348623157e308;
-  for (u32 i = begin + blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+  for (u32 i = begin + xcd_block() * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
     double q = elem_q<DIM>(i, ev, x, mt);
     if (q < v) v = q;
   }
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(BLOCK) void measure_edge_f64(u32 n, u32 const* __re
                                                           double const* __restrict__ h,
                                                           double const* __restrict__ mt,
                                                           double* __restrict__ out) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   out[i] = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
 }
@@ -231,7 +231,7 @@ Buf<double> measure_edges(Mesh& m) {
 // from atomics; each bucket is then sorted, which makes the result exact.
 __global__ __launch_bounds__(BLOCK) void upfd_count(u32 nuses, u32 const* __restrict__ in,
                                                     u32* __restrict__ counts) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i < nuses) atomicAdd(&counts[in[i]], 1u);
 }
 
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(BLOCK) void upfd_count(u32 nuses, u32 const* __rest
 __global__ __launch_bounds__(BLOCK) void upfd_fill(u32 nuses, u32 const* __restrict__ in,
                                                    u32 const* __restrict__ offs, u32* __restrict__ counts,
                                                    u32* __restrict__ out) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= nuses) return;
   u32 d = in[i];
   u32 j = atomicSub(&counts[d], 1u) - 1u;
@@ -256,7 +256,7 @@ template <u32 G>
 __global__ __launch_bounds__(BLOCK) void upfd_rank_split(u32 nlows, u32 deg, u32 const* __restrict__ offs,
                                                          u32 const* __restrict__ tmp, u32* __restrict__ adj,
                                                          u32* __restrict__ dirs) {
-  size_t gid = (size_t) blockIdx.x * BLOCK + threadIdx.x;
+  size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
   size_t grp = gid / G;
   u32 lane = (u32) (gid % G);
   if (grp >= nlows) return;
@@ -292,20 +292,34 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
   u.adj = Buf<u32>(nuses);
   u.dirs = Buf<u32>(nuses);
   if (nlows && nuses) {
-    // offsets, uses in, (high, direction) out; lanes per bucket from the
-    // mean bucket size
+    // offsets, uses in, (high, direction) out; lanes per bucket (4 to 32)
+    // chosen online per (high, low) dimension pair and size class
+    static Tuner tuners[4][4] = {
+        {Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
+        {Tuner("upfd_rank_split_1_0", {4, 8, 16, 32}), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
+        {Tuner("upfd_rank_split_2_0", {4, 8, 16, 32}), Tuner("upfd_rank_split_2_1", {4, 8, 16, 32}), Tuner("-", {4}),
+         Tuner("-", {4})},
+        {Tuner("upfd_rank_split_3_0", {4, 8, 16, 32}), Tuner("upfd_rank_split_3_1", {4, 8, 16, 32}),
+         Tuner("upfd_rank_split_3_2", {4, 8, 16, 32}), Tuner("-", {4})}};
     double bytes = (double) (nlows + 1) * 4 + (double) nuses * 12;
-    u32 mean = (nuses + nlows - 1) / nlows;
     u32 const* o = u.offsets.get();
-    if (mean <= 4)
-      launch_timed("upfd_rank_split", bytes, upfd_rank_split<4>, dim3(nblocks((size_t) nlows * 4)), dim3(BLOCK),
-                   nlows, deg, o, tmp.get(), u.adj.get(), u.dirs.get());
-    else if (mean <= 8)
-      launch_timed("upfd_rank_split", bytes, upfd_rank_split<8>, dim3(nblocks((size_t) nlows * 8)), dim3(BLOCK),
-                   nlows, deg, o, tmp.get(), u.adj.get(), u.dirs.get());
-    else
-      launch_timed("upfd_rank_split", bytes, upfd_rank_split<16>, dim3(nblocks((size_t) nlows * 16)),
-                   dim3(BLOCK), nlows, deg, o, tmp.get(), u.adj.get(), u.dirs.get());
+    auto go = [&](auto kernel, u32 g, hipEvent_t ea, hipEvent_t eb) {
+      dim3 grid(nblocks((size_t) nlows * g));
+      if (ea) {
+        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, nlows, deg, o, tmp.get(),
+                              u.adj.get(), u.dirs.get());
+        check_launch("upfd_rank_split");
+      } else {
+        launch_timed("upfd_rank_split", bytes, kernel, grid, dim3(BLOCK), nlows, deg, o, tmp.get(), u.adj.get(),
+                     u.dirs.get());
+      }
+    };
+    tuned(tuners[hd][ld], nlows, [&](u32 g, hipEvent_t ea, hipEvent_t eb) {
+      if (g == 4) go(upfd_rank_split<4>, g, ea, eb);
+      else if (g == 8) go(upfd_rank_split<8>, g, ea, eb);
+      else if (g == 16) go(upfd_rank_split<16>, g, ea, eb);
+      else go(upfd_rank_split<32>, g, ea, eb);
+    });
   }
 }
 
@@ -344,7 +358,7 @@ __device__ inline u32 tuple_hash(u32 const (&v)[N], u32 mask) {
 template <u32 VPL>
 __global__ __launch_bounds__(BLOCK) void lows_hash_insert(u32 nl, u32 const* __restrict__ vl, u32 mask,
                                                           u32* __restrict__ table, u32* __restrict__ err) {
-  u32 e = blockIdx.x * BLOCK + threadIdx.x;
+  u32 e = xcd_block() * BLOCK + threadIdx.x;
   if (e >= nl) return;
   u32 v[VPL];
 #pragma unroll
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(BLOCK) void reflect_down_hash(u32 nh, u32 const* __
                                                            u32 const* __restrict__ vl, u32 mask,
                                                            u32 const* __restrict__ table, u32* __restrict__ out,
                                                            u32* __restrict__ err) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= nh) return;
   constexpr u32 VPH = HD + 1, VPL = LD + 1;
   constexpr u32 LPH = (HD == 3) ? (LD == 1 ? 6 : 4) : 3;
@@ -865,7 +879,7 @@ __global__ __launch_bounds__(BLOCK) void mark_sliver_f64(u32 n, u32 const* __res
                                                          double const* __restrict__ x,
                                                          double const* __restrict__ mt, double good,
                                                          u32* __restrict__ out) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   out[i] = (elem_q<DIM>(i, ev, x, mt) < good) ? 1u : 0u;
 }

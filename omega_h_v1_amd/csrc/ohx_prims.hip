@@ -212,7 +212,11 @@ void compact_u32(u32* out, u32 const* a, u32 const* offs, u32 n, u32 width) {
 
 u32* scratch_words() {
   // allocated once for the life of the process; never returned to the arena
-  static u32* s = static_cast<u32*>(dev_alloc(64 * sizeof(u32)));
+  static u32* s = [] {
+    u32* w = static_cast<u32*>(dev_alloc(64 * sizeof(u32)));
+    fill(w, 64, 0);
+    return w;
+  }();
   return s;
 }
 

@@ -7,6 +7,7 @@
 
 #include "ohx_common.hpp"
 
+#include <algorithm>
 #include <chrono>
 
 namespace ohx {
@@ -278,6 +279,119 @@ std::vector<std::string> prof_names() {
   std::vector<std::string> v;
   for (auto& kv : prof().totals) v.push_back(kv.first);
   return v;
+}
+
+}  // namespace ohx
+
+namespace ohx {
+
+static std::vector<Tuner*>& tuners() {
+  static std::vector<Tuner*> v;
+  return v;
+}
+
+static std::map<std::string, u32>& forced_variants() {
+  static std::map<std::string, u32> m;
+  return m;
+}
+
+void tuning_force(char const* name, int variant) {
+  if (variant < 0)
+    forced_variants().erase(name);
+  else
+    forced_variants()[name] = (u32) variant;
+}
+
+static u32 size_class(u32 n) {
+  u32 c = 0;
+  while ((1u << c) < n && c < 32) ++c;
+  return c;
+}
+
+void Tuner::harvest() {
+  for (size_t k = 0; k < pending.size();) {
+    Trial& tr = pending[k];
+    if (hipEventQuery(tr.b) != hipSuccess) {
+      ++k;
+      continue;
+    }
+    float ms = 0;
+    OHX_CHECK(hipEventElapsedTime(&ms, tr.a, tr.b));
+    Cls& c = classes[tr.cls];
+    double per = ms / (tr.items > 0 ? tr.items : 1);
+    if (c.best[tr.v] < 0 || per < c.best[tr.v]) c.best[tr.v] = per;
+    pool.push_back(tr.a);
+    pool.push_back(tr.b);
+    pending[k] = pending.back();
+    pending.pop_back();
+  }
+  for (Cls& c : classes) {
+    if (c.chosen >= 0 || c.best.empty()) continue;
+    bool all = true;
+    for (double b : c.best) all = all && b >= 0;
+    if (!all) continue;
+    u32 arg = 0;
+    for (u32 v = 1; v < c.best.size(); ++v)
+      if (c.best[v] < c.best[arg]) arg = v;
+    c.chosen = (int) arg;
+  }
+}
+
+u32 Tuner::pick(u32 n, hipEvent_t* a, hipEvent_t* b) {
+  if (std::find(tuners().begin(), tuners().end(), this) == tuners().end()) tuners().push_back(this);
+  auto f = forced_variants().find(name);
+  if (f != forced_variants().end()) {
+    for (u32 v : variants)
+      if (v == f->second) return v;
+  }
+  u32 cls = size_class(n);
+  Cls& c = classes[cls];
+  if (c.best.empty()) c.best.assign(variants.size(), -1.0);
+  if (c.chosen < 0 && !pending.empty()) harvest();
+  if (c.chosen >= 0) return variants[c.chosen];
+  // the next variant without a timing (round robin while results are due)
+  u32 v = c.next % variants.size();
+  for (u32 k = 0; k < variants.size(); ++k) {
+    u32 cand = (c.next + k) % variants.size();
+    if (c.best[cand] < 0) {
+      v = cand;
+      break;
+    }
+  }
+  c.next = v + 1;
+  for (int k = 0; k < 2; ++k) {
+    hipEvent_t e;
+    if (!pool.empty()) {
+      e = pool.back();
+      pool.pop_back();
+    } else {
+      OHX_CHECK(hipEventCreate(&e));
+    }
+    (k ? *b : *a) = e;
+  }
+  pending.push_back(Trial{cls, v, (double) n, *a, *b});
+  return variants[v];
+}
+
+int Tuner::chosen(u32 n) const {
+  int c = classes[size_class(n)].chosen;
+  return c < 0 ? -1 : (int) variants[c];
+}
+
+std::string tuning_report() {
+  std::string r;
+  for (Tuner* t : tuners()) {
+    t->harvest();
+    for (u32 c = 0; c < t->classes.size(); ++c) {
+      Tuner::Cls const& k = t->classes[c];
+      if (k.best.empty()) continue;
+      r += t->name;
+      r += ":2^" + std::to_string(c) + "=";
+      r += k.chosen < 0 ? std::string("?") : std::to_string(t->variants[k.chosen]);
+      r += "\n";
+    }
+  }
+  return r;
 }
 
 }  // namespace ohx

@@ -18,7 +18,7 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-PRODUCT_LIB = os.path.join(ROOT, "omega_h_v1_amd", "lib", "libomega_h_amd.so")
+PRODUCT_LIB = os.environ.get("OHX_PRODUCT_LIB") or os.path.join(ROOT, "omega_h_v1_amd", "lib", "libomega_h_amd.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libohv1_oracle.so")
 REF_SERIAL_LIB = os.path.join(ROOT, "oracle", "_ref", "libohv1_ref_serial.so")
 REF_OMP_LIB = os.path.join(ROOT, "oracle", "_ref", "libohv1_ref_omp.so")
