@@ -151,6 +151,21 @@ int ohx_mesh_adapt_timed(ohx_mesh* m, double size_floor, double good_qual,
 /* mesh_warp_to_limit (warp_to_limit.cpp:61-81) */
 unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qual_floor) OSH_PUBLIC;
 
+/* Batches (no counterpart in the reference, which adapts one mesh per call):
+ * n full, classified meshes of one dimension, carrying the same fields,
+ * become one batch mesh (the inputs are left as they are).  Adapting the
+ * batch runs one launch sequence for all of them and leaves every component
+ * exactly as mesh_adapt would leave it alone; results[c] is that call's
+ * return value (1, 0, or -1 at the operation limit), stats as in
+ * ohx_mesh_adapt_timed (input elements count the components a pass changed).
+ * Returns the number of components that reached the operation limit. */
+ohx_mesh* ohx_mesh_batch_new(unsigned n, ohx_mesh* const* meshes) OSH_PUBLIC;
+unsigned ohx_mesh_batch_size(ohx_mesh* m) OSH_PUBLIC;
+/* component c as a mesh of its own (entities in batch order, own class ids) */
+ohx_mesh* ohx_mesh_batch_extract(ohx_mesh* m, unsigned c) OSH_PUBLIC;
+int ohx_mesh_batch_adapt(ohx_mesh* m, double size_floor, double good_qual,
+    unsigned nlayers, unsigned max_ops, int* results, double* stats) OSH_PUBLIC;
+
 /* the osh_t handle of a seam mesh (same object) */
 osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;
 

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "lib", "libomega_h_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_api.hip"]
+SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_batch.hip", "ohx_api.hip"]
 
 FLAGS = [
     "--offload-arch=" + ARCH,

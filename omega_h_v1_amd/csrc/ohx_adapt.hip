@@ -137,10 +137,13 @@ template <u32 DIM, bool MIXED>
 __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     u32 ns, u32 const* __restrict__ vs, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ ve, double const* __restrict__ x,
-    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor, u32* __restrict__ cand,
-    double* __restrict__ out, u32* __restrict__ words, u32* __restrict__ err, ScreenStats st) {
+    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor_all,
+    double const* __restrict__ qfe, u32* __restrict__ cand, double* __restrict__ out, u32* __restrict__ words,
+    u32* __restrict__ err, ScreenStats st) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ns || !cand[i]) return;
+  // one floor for the mesh, or its component's in a batch
+  double const qual_floor = qfe ? qfe[i] : qual_floor_all;
   constexpr u32 VPE = DIM + 1;
   V3 a = ld3(x + 3 * (size_t) vs[2 * i]), b = ld3(x + 3 * (size_t) vs[2 * i + 1]);
   // average_element_field (element_field.hpp:8-25): ((0 + a) + b) / 2
@@ -250,7 +253,8 @@ __global__ __launch_bounds__(BLOCK) void refine_candidate_f64(u32 n, u32 const* 
   if (c) words[W_ANY] = 1u;
 }
 
-static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool require_better) {
+static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool require_better,
+                                    double const* qfe) {
   u32 dim = m.dim;
   u32 ns = count(m, 1);
   u32 const* vs = ask_down(m, 1, 0);
@@ -268,13 +272,13 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   auto go = [&](auto kernel, char const* kn, hipEvent_t ea, hipEvent_t eb) {
     if (ea) {
       hipExtLaunchKernelGGL(kernel, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ea, eb, 0, ns, vs, u.offsets.get(),
-                            u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, cand,
-                            out.get(), scratch_words(), dev_error_word(), sst);
+                            u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, qfe,
+                            cand, out.get(), scratch_words(), dev_error_word(), sst);
       check_launch(kn);
     } else {
       launch_timed(kn, prof_bytes, kernel, dim3(nblocks(ns)), dim3(BLOCK), ns, vs, u.offsets.get(), u.adj.get(),
-                   u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, cand, out.get(), scratch_words(),
-                   dev_error_word(), sst);
+                   u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, qfe, cand, out.get(),
+                   scratch_words(), dev_error_word(), sst);
     }
   };
   auto run = [&](u32 screen, hipEvent_t ea, hipEvent_t eb) {
@@ -559,12 +563,14 @@ static void check_index_space(Mesh& m) {
     fail("mesh too large for 32-bit adjacency offsets", __FILE__, __LINE__);
 }
 
-static u32 refine_common(Mesh& m, Buf<u32>& cand, double qual_floor, bool require_better) {
+static u32 refine_common(Mesh& m, Buf<u32>& cand, double qual_floor, bool require_better,
+                         double const* qfe = nullptr) {
   check_index_space(m);
   u32 ns = count(m, 1);
-  Buf<double> q = refine_qualities(m, cand.get(), qual_floor, require_better);
+  Buf<double> q = refine_qualities(m, cand.get(), qual_floor, require_better, qfe);
   if (!read_word(W_ANY)) return 0;
   Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
+  if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();
   q.release();
   Buf<u32> gen_off(ns + 1);
@@ -590,6 +596,11 @@ u32 refine_by_size(Mesh& m, double qual_floor) {
                        ask_down(m, 1, 0), coords(m), h, mt, cand.get(), scratch_words());
   }
   if (!read_word(W_ANY)) return 0;
+  if (g_batch) {
+    batch_mask(m, 1, cand.get());
+    Buf<double> qfe = batch_floors(m, 1);
+    return refine_common(m, cand, qual_floor, false, qfe.get());
+  }
   return refine_common(m, cand, qual_floor, false);
 }
 
@@ -680,14 +691,17 @@ template <u32 DIM, u32 G, bool MIXED>
 __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
     u32 ne, u32 const* __restrict__ ev, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ vel, double const* __restrict__ x,
-    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor, u32* __restrict__ codes,
-    double* __restrict__ out, u32* __restrict__ words, ScreenStats st) {
+    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor_all,
+    double const* __restrict__ qfe, u32* __restrict__ codes, double* __restrict__ out, u32* __restrict__ words,
+    ScreenStats st) {
   size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
   size_t i = gid / G;
   u32 lane = (u32) (gid % G);
   if (i >= ne) return;  // whole groups leave together
   u32 c = codes[i];
   if (c == DONT_COLLAPSE) return;
+  // one floor for the mesh, or its component's in a batch
+  double const qual_floor = qfe ? qfe[i] : qual_floor_all;
   constexpr u32 VPE = DIM + 1;
   u32 const* bo = opp(DIM, 0);
   for (u32 j = 0; j < 2; ++j) {
@@ -859,7 +873,8 @@ __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
   if (a.out_ci) a.out_ci[o] = a.ci[i];
 }
 
-static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better) {
+static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better,
+                          double const* qfe = nullptr) {
   check_index_space(m);
   u32 dim = m.dim;
   u32 ne = count(m, 1);
@@ -892,11 +907,11 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       dim3 grid(nblocks((size_t) ne * g));
       if (ea) {
         hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ne, ev, uoff, uadj, udir, vel, xs,
-                              mts, eq.get(), qual_floor, codes.get(), cq.get(), scratch_words(), sst);
+                              mts, eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), sst);
         check_launch(kn);
       } else {
         launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ne, ev, uoff, uadj, udir, vel, xs, mts, eq.get(),
-                     qual_floor, codes.get(), cq.get(), scratch_words(), sst);
+                     qual_floor, qfe, codes.get(), cq.get(), scratch_words(), sst);
       }
     };
     tuned(tuner, ne, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
@@ -961,6 +976,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     });
   }
   Buf<u32> indset = find_indset(m, 0, cand.get(), vq.get());
+  if (g_batch) batch_note_changed(m, 0, indset.get());
   cand.release();
   // vertex_dest (collapses_to_verts.cpp:72-135)
   Buf<u32> gvv(nv);
@@ -1140,6 +1156,11 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
                        ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words());
   }
   if (!read_word(W_ANY)) return 0;
+  if (g_batch) {
+    batch_mask(m, 1, codes.get());
+    Buf<double> qfe = batch_floors(m, 1);
+    return coarsen_common(m, codes, qual_floor, false, qfe.get());
+  }
   return coarsen_common(m, codes, qual_floor, false);
 }
 
@@ -1160,6 +1181,7 @@ u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers) {
     if (mk[ev[2 * i + 1]]) r |= 2u;
     c[i] = r;
   });
+  if (g_batch) batch_mask(m, 1, c);
   if (max_u32(c, ne) == DONT_COLLAPSE) return 0;
   return coarsen_common(m, codes, 0.0, true);
 }
@@ -1554,6 +1576,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   eq.release();
   if (!read_word(W_ANY)) return 0;
   Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
+  if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();
   q.release();
   u32 total = sum_u32(indset.get(), ne);
@@ -1649,6 +1672,7 @@ u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers) {
   Buf<u32> sl = mark_slivers(m, good_qual, nlayers);
   Buf<u32> c = mark_down(m, m.dim, 1, sl.get());
   sl.release();
+  if (g_batch) batch_mask(m, 1, c.get());
   return swap_common(m, c);
 }
 

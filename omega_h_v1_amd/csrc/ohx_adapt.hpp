@@ -28,6 +28,53 @@ u32 warp_to_limit(Mesh& m, double qual_floor);
 u32 swap_nint_edges(u32 ring_size);
 void swap_int_edges_host(u32 ring_size, u32 code, u32* out);
 
+// ------------------------------------------------------------- batching
+// A batch is one mesh holding many independent meshes (components): each
+// component's entities keep their relative order through every pass, and
+// its class ids are shifted into a range of its own per class dimension, so
+// the component of an entity is found from its class tags.  A batch pass
+// runs on the components the driver marks active, with each component's own
+// quality floor, and reports which components it changed; the driver steps
+// every component through mesh_adapt's sequence as if it were alone.
+struct BatchCtx {
+  u32 ncomp = 0;
+  Buf<u32> off;      // 4 rows of ncomp + 1: first class id of each component per class dim
+  Buf<u32> active;   // per component: the current pass may change it
+  Buf<double> qf;    // per component: quality floor of the size passes
+  Buf<u32> changed;  // per component: the current pass changed it
+};
+extern BatchCtx* g_batch;
+
+__device__ inline u32 batch_comp(u32 const* off, u32 ncomp, u32 cd, u32 cid) {
+  u32 const* row = off + (size_t) cd * (ncomp + 1);
+  u32 lo = 0, hi = ncomp;  // row[lo] <= cid < row[hi] (row[ncomp] is all ones)
+  while (hi - lo > 1) {
+    u32 mid = (lo + hi) / 2;
+    if (row[mid] <= cid)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+// zero flags[e] of entities of dimension d in inactive components
+void batch_mask(Mesh& m, u32 d, u32* flags);
+// the quality floor of each entity's component (dimension d)
+Buf<double> batch_floors(Mesh& m, u32 d);
+// record the components owning a set entity of dimension d
+void batch_note_changed(Mesh& m, u32 d, u32 const* indset);
+
+// the union of classified full meshes of one dimension, as one mesh
+void batch_union(std::vector<Mesh*> const& parts, Mesh& out, BatchCtx& b);
+// component c of a batch as a mesh of its own (entities in batch order)
+void batch_extract(Mesh& u, BatchCtx& b, u32 c, Mesh& out);
+// per-component element counts and min qualities (the serial skip of each
+// component's first element included)
+void batch_quality(Mesh& m, BatchCtx& b, std::vector<double>& minq, std::vector<double>& nelems);
+// mesh_adapt on every component; results[c] as adapt()'s return value
+void batch_adapt(Mesh& m, BatchCtx& b, double size_floor, double good_qual, u32 nlayers, u32 max_ops, int* results,
+                 AdaptStats* st);
+
 extern int g_verbose;
 extern u32 g_indset_max_rounds;
 extern bool g_mixed;

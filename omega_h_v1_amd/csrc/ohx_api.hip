@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <string>
 
 #include "../../include/ohx.h"
@@ -30,6 +31,7 @@ struct ohx_mesh {
   Buf<u32> own_rank[4], own_id[4];
   Buf<unsigned long> global[4];
   std::vector<std::string> field_names;
+  std::unique_ptr<BatchCtx> batch;  // set on the mesh of a batch of meshes
   ohx_mesh(u32 d, bool f) : m(d, f) {}
 };
 
@@ -487,6 +489,59 @@ int ohx_mesh_adapt_timed(ohx_mesh* m, double sf, double gq, unsigned nl, unsigne
     stats[2] = st.tpass;
     stats[3] = st.ttotal;
     return r;
+  });
+}
+
+ohx_mesh* ohx_mesh_batch_new(unsigned n, ohx_mesh* const* meshes) {
+  return guard((ohx_mesh*) nullptr, [&]() {
+    OHX_REQUIRE(n > 0 && meshes, "batch: no meshes");
+    std::vector<Mesh*> parts;
+    for (unsigned i = 0; i < n; ++i) parts.push_back(&meshes[i]->m);
+    auto* mm = new ohx_mesh(parts[0]->dim, true);
+    mm->batch.reset(new BatchCtx);
+    try {
+      batch_union(parts, mm->m, *mm->batch);
+      sync();
+    } catch (...) {
+      delete mm;
+      throw;
+    }
+    return mm;
+  });
+}
+
+unsigned ohx_mesh_batch_size(ohx_mesh* m) { return m->batch ? m->batch->ncomp : 0; }
+
+ohx_mesh* ohx_mesh_batch_extract(ohx_mesh* m, unsigned c) {
+  return guard((ohx_mesh*) nullptr, [&]() {
+    OHX_REQUIRE(m->batch, "batch: not a batch mesh");
+    auto* mm = new ohx_mesh(m->m.dim, m->m.full);
+    try {
+      batch_extract(m->m, *m->batch, c, mm->m);
+      sync();
+    } catch (...) {
+      delete mm;
+      throw;
+    }
+    return mm;
+  });
+}
+
+int ohx_mesh_batch_adapt(ohx_mesh* m, double sf, double gq, unsigned nl, unsigned mo, int* results,
+                         double* stats) {
+  return guard(-2, [&]() {
+    OHX_REQUIRE(m->batch, "batch: not a batch mesh");
+    AdaptStats st;
+    batch_adapt(m->m, *m->batch, sf, gq, nl, mo, results, &st);
+    if (stats) {
+      stats[0] = st.npass;
+      stats[1] = st.nelems;
+      stats[2] = st.tpass;
+      stats[3] = st.ttotal;
+    }
+    int limited = 0;
+    for (u32 c = 0; c < m->batch->ncomp; ++c) limited += results[c] < 0;
+    return limited;
   });
 }
 

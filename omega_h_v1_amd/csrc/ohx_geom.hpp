@@ -364,4 +364,22 @@ __device__ inline void note_max_err(u32* w, float err) {
   if (b > __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(w, b);
 }
 
+// element quality, isotropic (v1) or in the vertex metric when mt != null
+template <u32 DIM>
+__device__ inline double elem_q(u32 i, u32 const* __restrict__ ev, double const* __restrict__ x,
+                                double const* __restrict__ mt) {
+  V3 p[DIM + 1];
+  u32 v[DIM + 1];
+#pragma unroll
+  for (u32 j = 0; j <= DIM; ++j) {
+    v[j] = ev[(size_t) i * (DIM + 1) + j];
+    p[j] = ld3(x + 3 * (size_t) v[j]);
+  }
+  if (!mt) return element_quality(DIM, p);
+  Sym m[DIM + 1];
+#pragma unroll
+  for (u32 j = 0; j <= DIM; ++j) m[j] = ld6(mt + 6 * (size_t) v[j]);
+  return metric_quality<DIM>(p, m, true);
+}
+
 }  // namespace ohx

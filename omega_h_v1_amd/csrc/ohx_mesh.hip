@@ -108,23 +108,6 @@ u32 const* u32_tag(Mesh& m, u32 d, char const* name) {
 // ---------------------------------------------------------------- quality
 // One thread per element: gather the vertex coordinates, evaluate the
 // reference's quality formula in FP64 (quality.cpp:13-45).
-// element quality, isotropic (v1) or in the vertex metric when mt != null
-template <u32 DIM>
-__device__ inline double elem_q(u32 i, u32 const* __restrict__ ev, double const* __restrict__ x,
-                                double const* __restrict__ mt) {
-  V3 p[DIM + 1];
-  u32 v[DIM + 1];
-#pragma unroll
-  for (u32 j = 0; j <= DIM; ++j) {
-    v[j] = ev[(size_t) i * (DIM + 1) + j];
-    p[j] = ld3(x + 3 * (size_t) v[j]);
-  }
-  if (!mt) return element_quality(DIM, p);
-  Sym m[DIM + 1];
-#pragma unroll
-  for (u32 j = 0; j <= DIM; ++j) m[j] = ld6(mt + 6 * (size_t) v[j]);
-  return metric_quality<DIM>(p, m, true);
-}
 
 template <u32 DIM>
 __global__ __launch_bounds__(BLOCK) void elem_quality_f64(u32 n, u32 const* __restrict__ ev,

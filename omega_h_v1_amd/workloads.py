@@ -71,3 +71,37 @@ def rotating_metric(coords, h_short, aspect, dim=3):
     m[:, 3] = l1 * s * s + l2 * c * c
     m[:, 5] = l2
     return m
+
+
+def random_rotation(rng):
+    """Uniform rotation on SO(3): a normalised 4D Gaussian quaternion."""
+    q = rng.normal(size=4)
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+def constant_metric(nverts, rot, lengths):
+    """The same SPD metric R diag(1/h^2) R^T at every vertex (6 components)."""
+    m = rot @ np.diag(1.0 / np.asarray(lengths, dtype=np.float64) ** 2) @ rot.T
+    row = np.array([m[0, 0], m[0, 1], m[0, 2], m[1, 1], m[1, 2], m[2, 2]])
+    return np.tile(row, (nverts, 1))
+
+
+def batch_specs(count, seed, n_lo=8, n_hi=32, h_lo=0.005, h_hi=0.1):
+    """Meshes of a batch: cubes per axis n log-uniform in [n_lo, n_hi]
+    (rounded), a uniform random rotation and three axis lengths log-uniform
+    in [h_lo, h_hi] per mesh, from numpy's PCG64 generator seeded with seed.
+    Each mesh is the Kuhn box [0, L]^3 with L = n (h1 h2 h3)^(1/3), so its
+    mesh spacing is the metric's mean length."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        rot = random_rotation(rng)
+        h = np.exp(rng.uniform(np.log(h_lo), np.log(h_hi), size=3))
+        out.append({"n": n, "rot": rot, "lengths": h, "side": n * float(np.prod(h)) ** (1.0 / 3.0)})
+    return out

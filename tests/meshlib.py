@@ -273,6 +273,53 @@ class Mesh:
         r = self.call("adapt_timed", size_floor, good_qual, nlayers, max_ops, _ptr(stats))
         return int(r), stats
 
+    # ---- batches (product only)
+    @classmethod
+    def batch(cls, lib, meshes):
+        """The product's batch of full, classified meshes (ohx_mesh_batch_new)."""
+        d = lib.dll
+        d.ohx_mesh_batch_new.restype = C.c_void_p
+        d.ohx_mesh_batch_new.argtypes = [_u, C.POINTER(C.c_void_p)]
+        arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.dim = meshes[0].dim
+        self.h = d.ohx_mesh_batch_new(len(meshes), arr)
+        if not self.h:
+            raise RuntimeError("ohx_mesh_batch_new failed: %s" % d.ohx_last_error())
+        return self
+
+    def batch_size(self):
+        f = self.lib.dll.ohx_mesh_batch_size
+        f.restype, f.argtypes = _u, [_p]
+        return int(f(self.h))
+
+    def batch_extract(self, c):
+        d = self.lib.dll
+        d.ohx_mesh_batch_extract.restype = C.c_void_p
+        d.ohx_mesh_batch_extract.argtypes = [_p, _u]
+        other = Mesh.__new__(Mesh)
+        other.lib = self.lib
+        other.dim = self.dim
+        other.h = d.ohx_mesh_batch_extract(self.h, c)
+        if not other.h:
+            raise RuntimeError("ohx_mesh_batch_extract failed: %s" % d.ohx_last_error())
+        return other
+
+    def batch_adapt(self, size_floor=1.0 / 3.0, good_qual=0.3, nlayers=4, max_ops=50):
+        d = self.lib.dll
+        f = d.ohx_mesh_batch_adapt
+        f.restype = C.c_int
+        f.argtypes = [_p, _d, _d, _u, _u, C.c_void_p, C.c_void_p]
+        res = np.zeros(self.batch_size(), dtype=np.int32)
+        stats = np.zeros(4, dtype=np.float64)
+        d.ohx_clear_error()
+        r = f(self.h, size_floor, good_qual, nlayers, max_ops, _ptr(res), _ptr(stats))
+        msg = d.ohx_last_error()
+        if msg:
+            raise RuntimeError("ohx_mesh_batch_adapt: %s" % msg)
+        return int(r), res, stats
+
     def metric_from_hessian(self, hessian, eps, hmin, hmax, out="metric"):
         self.call("metric_from_hessian", hessian.encode(), eps, hmin, hmax, out.encode())
 
