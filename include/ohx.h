@@ -163,8 +163,29 @@ ohx_mesh* ohx_mesh_batch_new(unsigned n, ohx_mesh* const* meshes) OSH_PUBLIC;
 unsigned ohx_mesh_batch_size(ohx_mesh* m) OSH_PUBLIC;
 /* component c as a mesh of its own (entities in batch order, own class ids) */
 ohx_mesh* ohx_mesh_batch_extract(ohx_mesh* m, unsigned c) OSH_PUBLIC;
+/* A pass costs the whole batch, so during ohx_mesh_batch_adapt the finished
+ * components are set aside once they hold more than half of the batch's
+ * elements and the batch has at least this many (default 1e6); the result
+ * is the same batch either way. */
+void ohx_set_batch_retire_min(double elements) OSH_PUBLIC;
+/* When the batch pass that would serve the most elements covers less than
+ * this fraction of the batch (default 0.25), the live components finish
+ * their sequences one mesh at a time; 0 never hands over, 1 always does.
+ * The result is the same batch either way. */
+void ohx_set_batch_handoff(double fraction) OSH_PUBLIC;
 int ohx_mesh_batch_adapt(ohx_mesh* m, double size_floor, double good_qual,
     unsigned nlayers, unsigned max_ops, int* results, double* stats) OSH_PUBLIC;
+
+/* mesh_adapt on n independent meshes at once (no counterpart in the
+ * reference): worker threads (at least nthreads; they persist) each adapt
+ * meshes on a HIP stream of their own.  results[i] is ohx_mesh_adapt's
+ * value for meshes[i]; stats[0..2] sum the meshes' passes, input elements
+ * and pass seconds, stats[3] is the wall time of the call.  Returns the
+ * number of meshes that reached the operation limit, or -2 with
+ * ohx_last_error() if any mesh failed. */
+int ohx_adapt_many(unsigned n, ohx_mesh* const* meshes, double size_floor,
+    double good_qual, unsigned nlayers, unsigned max_ops, unsigned nthreads,
+    int* results, double* stats) OSH_PUBLIC;
 
 /* the osh_t handle of a seam mesh (same object) */
 osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;

@@ -3,6 +3,7 @@
 // a gfx950 kernel; the host only sequences passes and reads counts.
 #include <chrono>
 #include <cstring>
+#include <mutex>
 
 #include "ohx_adapt.hpp"
 #include "ohx_geom.hpp"
@@ -268,7 +269,7 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   ScreenStats sst = screen_stats();
   // in the mixed mode the FP32 screen is a variant chosen online (it pays
   // only where it rejects enough splits); both give identical results
-  static Tuner tuners[2] = {Tuner("refine_quality_2d_mixed", {0, 1}), Tuner("refine_quality_3d_mixed", {0, 1})};
+  static thread_local Tuner tuners[2] = {Tuner("refine_quality_2d_mixed", {0, 1}), Tuner("refine_quality_3d_mixed", {0, 1})};
   auto go = [&](auto kernel, char const* kn, hipEvent_t ea, hipEvent_t eb) {
     if (ea) {
       hipExtLaunchKernelGGL(kernel, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ea, eb, 0, ns, vs, u.offsets.get(),
@@ -893,7 +894,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     // variants, chosen online per size class: lanes per edge (1, 2, 4 or 8)
     // and, in the mixed mode, whether the FP32 screen runs (it pays only
     // where it rejects enough candidates); all give identical results
-    static Tuner tuners[2][2] = {
+    static thread_local Tuner tuners[2][2] = {
         {Tuner("coarsen_quality_2d_f64", {1, 2, 4, 8}), Tuner("coarsen_quality_2d_mixed", {1, 2, 4, 8, 17, 18, 20, 24})},
         {Tuner("coarsen_quality_3d_f64", {1, 2, 4, 8}), Tuner("coarsen_quality_3d_mixed", {1, 2, 4, 8, 17, 18, 20, 24})}};
     Tuner& tuner = tuners[dim == 3][g_mixed ? 1 : 0];
@@ -1237,6 +1238,7 @@ static u32 h_sw_nint[8];
 static u32 h_sw_int_off[8];
 static unsigned char h_sw_int[444];
 static bool g_swap_ready = false;
+static std::mutex g_swap_mu;
 
 // The interior edges of each triangulation, derived the way the
 // reference's generator does (test_print_swap_edges.cpp:29-64): the
@@ -1245,6 +1247,7 @@ static bool g_swap_ready = false;
 // disjoint components; side derivation numbers each component's edges
 // contiguously and in the same order as a mesh of its own would.
 static void make_swap_tables() {
+  std::lock_guard<std::mutex> lock(g_swap_mu);
   if (g_swap_ready) return;
   u32 off = 0;
   for (u32 n = 0; n < 4; ++n) {

@@ -43,7 +43,13 @@ struct BatchCtx {
   Buf<double> qf;    // per component: quality floor of the size passes
   Buf<u32> changed;  // per component: the current pass changed it
 };
-extern BatchCtx* g_batch;
+extern thread_local BatchCtx* g_batch;
+// a batch retires its finished components once they hold more than half of
+// its elements and it has at least this many elements
+extern double g_batch_retire_min;
+// a batch hands its live components to single-mesh runs when the best batch
+// pass would change components holding less than this fraction of it
+extern double g_batch_handoff;
 
 __device__ inline u32 batch_comp(u32 const* off, u32 ncomp, u32 cd, u32 cid) {
   u32 const* row = off + (size_t) cd * (ncomp + 1);

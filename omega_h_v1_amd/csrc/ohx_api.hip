@@ -5,6 +5,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <chrono>
+#include <mutex>
+#include <functional>
+#include <condition_variable>
+#include <thread>
 #include <memory>
 #include <string>
 
@@ -512,6 +517,9 @@ ohx_mesh* ohx_mesh_batch_new(unsigned n, ohx_mesh* const* meshes) {
 
 unsigned ohx_mesh_batch_size(ohx_mesh* m) { return m->batch ? m->batch->ncomp : 0; }
 
+void ohx_set_batch_retire_min(double elements) { g_batch_retire_min = elements; }
+void ohx_set_batch_handoff(double fraction) { g_batch_handoff = fraction; }
+
 ohx_mesh* ohx_mesh_batch_extract(ohx_mesh* m, unsigned c) {
   return guard((ohx_mesh*) nullptr, [&]() {
     OHX_REQUIRE(m->batch, "batch: not a batch mesh");
@@ -541,6 +549,102 @@ int ohx_mesh_batch_adapt(ohx_mesh* m, double sf, double gq, unsigned nl, unsigne
     }
     int limited = 0;
     for (u32 c = 0; c < m->batch->ncomp; ++c) limited += results[c] < 0;
+    return limited;
+  });
+}
+
+// Worker threads that adapt independent meshes concurrently, each on a
+// stream of its own: a small mesh's pass is a chain of short kernels and
+// host decisions, so meshes on several streams keep the GPU busy while each
+// one waits on its own synchronisations.  The workers live until the
+// process ends.
+namespace {
+struct Pool {
+  std::mutex mu;
+  std::condition_variable work, done_cv;
+  std::vector<std::thread> workers;
+  std::function<void(u32)> fn;
+  u32 n = 0, next = 0, done = 0;
+  unsigned long long gen = 0;
+  int device = 0;
+  void grow(u32 k) {
+    while (workers.size() < k) {
+      workers.emplace_back([this] { loop(); });
+      workers.back().detach();
+    }
+  }
+  void loop() {
+    (void) hipSetDevice(device);
+    unsigned long long seen = 0;
+    std::unique_lock<std::mutex> l(mu);
+    for (;;) {
+      work.wait(l, [&] { return gen != seen && next < n; });
+      while (next < n) {
+        u32 i = next++;
+        l.unlock();
+        fn(i);
+        l.lock();
+        if (++done == n) done_cv.notify_all();
+      }
+      seen = gen;
+    }
+  }
+  void run(u32 k, u32 count, std::function<void(u32)> f) {
+    std::unique_lock<std::mutex> l(mu);
+    grow(k);
+    fn = std::move(f);
+    n = count;
+    next = 0;
+    done = 0;
+    ++gen;
+    work.notify_all();
+    done_cv.wait(l, [&] { return done == n; });
+  }
+};
+Pool& pool() {
+  static Pool* p = new Pool;  // never destroyed: its threads outlive static teardown
+  return *p;
+}
+}  // namespace
+
+int ohx_adapt_many(unsigned n, ohx_mesh* const* meshes, double sf, double gq, unsigned nl, unsigned mo,
+                   unsigned nthreads, int* results, double* stats) {
+  return guard(-2, [&]() {
+    OHX_REQUIRE(meshes && results, "adapt_many: null arguments");
+    sync();  // the meshes' arrays are complete before other streams read them
+    std::vector<AdaptStats> st(n);
+    std::vector<std::string> err(n);
+    Pool& p = pool();
+    p.device = rt().device;
+    auto t0 = std::chrono::steady_clock::now();
+    p.run(nthreads ? nthreads : 1, n, [&](u32 i) {
+      try {
+        results[i] = adapt(meshes[i]->m, sf, gq, nl, mo, &st[i]);
+      } catch (std::exception const& e) {
+        results[i] = -2;
+        err[i] = e.what();
+      }
+    });
+    double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int bad = 0;
+    std::string first;
+    for (u32 i = 0; i < n; ++i)
+      if (results[i] == -2) {
+        if (!bad) first = err[i];
+        ++bad;
+      }
+    if (stats) {
+      stats[0] = stats[1] = stats[2] = 0;
+      for (auto& s : st) {
+        stats[0] += s.npass;
+        stats[1] += s.nelems;
+        stats[2] += s.tpass;
+      }
+      stats[3] = wall;
+    }
+    if (bad) fail(("adapt_many: " + first).c_str(), __FILE__, __LINE__);
+    int limited = 0;
+    for (u32 i = 0; i < n; ++i) limited += results[i] < 0;
     return limited;
   });
 }

@@ -17,6 +17,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cstring>
+#include <memory>
 
 #include "ohx_adapt.hpp"
 #include "ohx_geom.hpp"
@@ -25,14 +26,11 @@
 
 namespace ohx {
 
-BatchCtx* g_batch = nullptr;
+thread_local BatchCtx* g_batch = nullptr;
+double g_batch_retire_min = 1e6;
+double g_batch_handoff = 0.25;
 
 namespace {
-
-struct BatchScope {
-  explicit BatchScope(BatchCtx* b) { g_batch = b; }
-  ~BatchScope() { g_batch = nullptr; }
-};
 
 // order-preserving map of doubles to unsigned keys (NaN excluded by callers)
 __device__ inline unsigned long long dkey(double q) {
@@ -327,60 +325,193 @@ namespace {
 enum Phase { P_REFINE, P_COARSEN, P_CHECK, P_SWAP, P_CSLIVERS, P_DONE, P_ABORTED };
 }
 
+// mesh_adapt's control flow for one mesh, resumed from a given phase,
+// operation count and quality floor (a batch hands its last components over
+// to this when batch passes would mostly copy finished ones)
+static void adapt_continue(Mesh& m, int& phase, u32& ops, double qf, double size_floor, double good_qual,
+                           u32 nlayers, u32 max_ops, AdaptStats& stats) {
+  auto now = [] {
+    sync();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  };
+  auto run = [&](int k) {
+    double tin = count(m, m.dim);
+    double t0 = now();
+    u32 ok = 0;
+    switch (k) {
+      case 0: ok = refine_by_size(m, qf); break;
+      case 1: ok = coarsen_by_size(m, qf, size_floor); break;
+      case 2: ok = swap_slivers(m, good_qual, nlayers); break;
+      case 3: ok = coarsen_slivers(m, good_qual, nlayers); break;
+    }
+    double t1 = now();
+    if (ok) {
+      stats.npass += 1;
+      stats.nelems += tin;
+      stats.tpass += t1 - t0;
+    }
+    return ok != 0;
+  };
+  auto incr = [&] {
+    if (ops > max_ops) {
+      phase = P_ABORTED;
+      return false;
+    }
+    ++ops;
+    return true;
+  };
+  while (phase != P_DONE && phase != P_ABORTED) {
+    switch (phase) {
+      case P_REFINE:
+        if (run(0))
+          incr();
+        else
+          phase = P_COARSEN;
+        break;
+      case P_COARSEN:
+        if (run(1))
+          incr();
+        else
+          phase = P_CHECK;
+        break;
+      case P_CHECK:
+        phase = mesh_min_quality(m) >= good_qual ? P_DONE : (m.dim == 3 ? P_SWAP : P_CSLIVERS);
+        break;
+      case P_SWAP:
+        if (!run(2))
+          phase = P_CSLIVERS;
+        else if (incr())
+          phase = P_CHECK;
+        break;
+      case P_CSLIVERS:
+        if (!run(3))
+          phase = P_DONE;
+        else if (incr())
+          phase = P_CHECK;
+        break;
+    }
+  }
+}
+
 void batch_adapt(Mesh& m, BatchCtx& b, double size_floor, double good_qual, u32 nlayers, u32 max_ops, int* results,
                  AdaptStats* st) {
-  BatchScope scope(&b);
-  u32 nc = b.ncomp;
+  u32 const nc = b.ncomp;
   auto now = [] {
     sync();
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
   };
   double t_start = now();
   AdaptStats stats;
+  // per original component
   std::vector<int> phase(nc, P_REFINE);
   std::vector<u32> ops(nc, 0);
-  std::vector<double> minq, nel;
-  // satisfy_size (adapt.cpp:47-56): each component's own floor
-  batch_quality(m, b, minq, nel);
   std::vector<double> qf(nc);
+  std::vector<std::unique_ptr<Mesh>> retired(nc);
+  // the live batch: components still adapting, live[k] = original index of
+  // live component k.  A pass costs the whole live batch (every survivor is
+  // copied, every adjacency rebuilt), so once most of its elements belong to
+  // finished components those are extracted and the rest re-batched.
+  Mesh* cur = &m;
+  BatchCtx* cb = &b;
+  std::unique_ptr<Mesh> own_mesh;
+  std::unique_ptr<BatchCtx> own_ctx;
+  std::vector<u32> live(nc);
+  for (u32 c = 0; c < nc; ++c) live[c] = c;
+  std::vector<double> minq, nel;
+  g_batch = cb;
+  struct Reset {
+    ~Reset() { g_batch = nullptr; }
+  } reset;
+  // satisfy_size (adapt.cpp:47-56): each component's own floor
+  batch_quality(*cur, *cb, minq, nel);
   for (u32 c = 0; c < nc; ++c) qf[c] = good_qual < minq[c] ? good_qual : minq[c];
-  {
-    Buf<double> q = to_device(qf.data(), nc);
-    d2d(b.qf.get(), q.get(), nc);
-  }
-  std::vector<u32> act(nc), ch(nc);
+  auto upload_floors = [&] {
+    std::vector<double> q(live.size());
+    for (size_t k = 0; k < live.size(); ++k) q[k] = qf[live[k]];
+    Buf<double> d = to_device(q.data(), q.size());
+    d2d(cb->qf.get(), d.get(), q.size());
+  };
+  upload_floors();
   bool nel_fresh = true;
-  // one batch pass of kind k on the components in phase ph
+  auto finished = [&](u32 c) { return phase[c] == P_DONE || phase[c] == P_ABORTED; };
+  auto compact = [&] {
+    if (!nel_fresh) {
+      batch_counts(*cur, *cb, nel);
+      nel_fresh = true;
+    }
+    double all = 0, open = 0;
+    u32 nopen = 0;
+    for (size_t k = 0; k < live.size(); ++k) {
+      all += nel[k];
+      if (!finished(live[k])) {
+        open += nel[k];
+        ++nopen;
+      }
+    }
+    if (nopen == live.size() || all < g_batch_retire_min || open > 0.5 * all) return;
+    std::vector<std::unique_ptr<Mesh>> keep;
+    std::vector<u32> next;
+    for (size_t k = 0; k < live.size(); ++k) {
+      auto part = std::make_unique<Mesh>(cur->dim, cur->full);
+      batch_extract(*cur, *cb, (u32) k, *part);
+      if (finished(live[k])) {
+        retired[live[k]] = std::move(part);
+      } else {
+        keep.push_back(std::move(part));
+        next.push_back(live[k]);
+      }
+    }
+    auto nm = std::make_unique<Mesh>(cur->dim, true);
+    auto nb = std::make_unique<BatchCtx>();
+    if (!keep.empty()) {
+      std::vector<Mesh*> ps;
+      for (auto& p : keep) ps.push_back(p.get());
+      batch_union(ps, *nm, *nb);
+    }
+    own_mesh = std::move(nm);
+    own_ctx = std::move(nb);
+    cur = own_mesh.get();
+    cb = own_ctx.get();
+    g_batch = cb;
+    live = next;
+    nel_fresh = false;
+    if (!live.empty()) upload_floors();
+  };
+  std::vector<u32> act, ch;
+  // one batch pass of kind k on the live components in phase ph
   auto pass = [&](int ph, int k) {
+    u32 nl = (u32) live.size();
+    act.assign(nl, 0);
     u32 any = 0;
-    for (u32 c = 0; c < nc; ++c) {
-      act[c] = phase[c] == ph;
-      any |= act[c];
+    for (u32 j = 0; j < nl; ++j) {
+      act[j] = phase[live[j]] == ph;
+      any |= act[j];
     }
     if (!any) return false;
     if (!nel_fresh) {
-      batch_counts(m, b, nel);
+      batch_counts(*cur, *cb, nel);
       nel_fresh = true;
     }
     {
-      Buf<u32> a = to_device(act.data(), nc);
-      d2d(b.active.get(), a.get(), nc);
+      Buf<u32> a = to_device(act.data(), nl);
+      d2d(cb->active.get(), a.get(), nl);
     }
-    fill(b.changed.get(), nc, 0);
+    fill(cb->changed.get(), nl, 0);
     double t0 = now();
     switch (k) {
-      case 0: refine_by_size(m, good_qual); break;
-      case 1: coarsen_by_size(m, good_qual, size_floor); break;
-      case 2: swap_slivers(m, good_qual, nlayers); break;
-      case 3: coarsen_slivers(m, good_qual, nlayers); break;
+      case 0: refine_by_size(*cur, good_qual); break;
+      case 1: coarsen_by_size(*cur, good_qual, size_floor); break;
+      case 2: swap_slivers(*cur, good_qual, nlayers); break;
+      case 3: coarsen_slivers(*cur, good_qual, nlayers); break;
     }
-    to_host(ch.data(), b.changed.get(), nc);
+    ch.assign(nl, 0);
+    to_host(ch.data(), cb->changed.get(), nl);
     double t1 = now();
     double tin = 0;
     bool changed_any = false;
-    for (u32 c = 0; c < nc; ++c)
-      if (act[c] && ch[c]) {
-        tin += nel[c];
+    for (u32 j = 0; j < nl; ++j)
+      if (act[j] && ch[j]) {
+        tin += nel[j];
         changed_any = true;
       }
     if (changed_any) {
@@ -400,59 +531,105 @@ void batch_adapt(Mesh& m, BatchCtx& b, double size_floor, double good_qual, u32 
     ++ops[c];
     return true;
   };
-  while (true) {
-    if (pass(P_REFINE, 0)) {
-      for (u32 c = 0; c < nc; ++c)
-        if (act[c]) {
-          if (!ch[c])
+  // the batch pass that serves the most elements next; when even that one
+  // would mostly copy components it does not change, the rest of every live
+  // component's sequence runs one mesh at a time
+  static int const kind_phase[4] = {P_REFINE, P_COARSEN, P_SWAP, P_CSLIVERS};
+  while (!live.empty()) {
+    compact();
+    if (live.empty()) break;
+    u32 nl = (u32) live.size();
+    bool checking = false;
+    for (u32 c : live) checking |= phase[c] == P_CHECK;
+    if (checking) {  // satisfy_shape (adapt.cpp:58-80)
+      batch_quality(*cur, *cb, minq, nel);
+      nel_fresh = true;
+      for (u32 j = 0; j < nl; ++j) {
+        u32 c = live[j];
+        if (phase[c] == P_CHECK) phase[c] = minq[j] >= good_qual ? P_DONE : (cur->dim == 3 ? P_SWAP : P_CSLIVERS);
+      }
+      continue;
+    }
+    if (!nel_fresh) {
+      batch_counts(*cur, *cb, nel);
+      nel_fresh = true;
+    }
+    double total = 0, best = 0;
+    int kind = -1;
+    for (u32 j = 0; j < nl; ++j) total += nel[j];
+    for (int k = 0; k < 4; ++k) {
+      double a = 0;
+      for (u32 j = 0; j < nl; ++j)
+        if (phase[live[j]] == kind_phase[k]) a += nel[j];
+      if (a > best) {
+        best = a;
+        kind = k;
+      }
+    }
+    if (kind < 0) break;  // every live component has finished
+    if (nl > 1 && best < g_batch_handoff * total) {
+      for (u32 j = 0; j < nl; ++j) {
+        u32 c = live[j];
+        auto part = std::make_unique<Mesh>(cur->dim, cur->full);
+        batch_extract(*cur, *cb, j, *part);
+        g_batch = nullptr;
+        adapt_continue(*part, phase[c], ops[c], qf[c], size_floor, good_qual, nlayers, max_ops, stats);
+        g_batch = cb;
+        retired[c] = std::move(part);
+      }
+      live.clear();
+      break;
+    }
+    pass(kind_phase[kind], kind);
+    for (u32 j = 0; j < nl; ++j) {
+      if (!act[j]) continue;
+      u32 c = live[j];
+      switch (kind) {
+        case 0:
+          if (!ch[j])
             phase[c] = P_COARSEN;
           else
             incr(c);
-        }
-      continue;
-    }
-    if (pass(P_COARSEN, 1)) {
-      for (u32 c = 0; c < nc; ++c)
-        if (act[c]) {
-          if (!ch[c])
+          break;
+        case 1:
+          if (!ch[j])
             phase[c] = P_CHECK;
           else
             incr(c);
-        }
-      continue;
-    }
-    bool checking = false;
-    for (u32 c = 0; c < nc; ++c) checking |= phase[c] == P_CHECK;
-    if (checking) {  // satisfy_shape (adapt.cpp:58-80)
-      batch_quality(m, b, minq, nel);
-      nel_fresh = true;
-      for (u32 c = 0; c < nc; ++c)
-        if (phase[c] == P_CHECK) phase[c] = minq[c] >= good_qual ? P_DONE : (m.dim == 3 ? P_SWAP : P_CSLIVERS);
-      continue;
-    }
-    if (pass(P_SWAP, 2)) {
-      for (u32 c = 0; c < nc; ++c)
-        if (act[c]) {
-          if (!ch[c])
+          break;
+        case 2:
+          if (!ch[j])
             phase[c] = P_CSLIVERS;
           else if (incr(c))
             phase[c] = P_CHECK;
-        }
-      continue;
-    }
-    if (pass(P_CSLIVERS, 3)) {
-      for (u32 c = 0; c < nc; ++c)
-        if (act[c]) {
-          if (!ch[c])
+          break;
+        case 3:
+          if (!ch[j])
             phase[c] = P_DONE;  // "shape quality could not be satisfied"
           else if (incr(c))
             phase[c] = P_CHECK;
-        }
-      continue;
+          break;
+      }
     }
-    break;
   }
   for (u32 c = 0; c < nc; ++c) results[c] = phase[c] == P_ABORTED ? -1 : (ops[c] > 0 ? 1 : 0);
+  // components retired along the way go back into one batch, in order
+  bool any_retired = false;
+  for (u32 c = 0; c < nc; ++c) any_retired = any_retired || retired[c] != nullptr;
+  if (any_retired) {
+    std::vector<std::unique_ptr<Mesh>> tail(nc);
+    for (size_t k = 0; k < live.size(); ++k) {
+      tail[live[k]] = std::make_unique<Mesh>(cur->dim, cur->full);
+      batch_extract(*cur, *cb, (u32) k, *tail[live[k]]);
+    }
+    std::vector<Mesh*> all(nc);
+    for (u32 c = 0; c < nc; ++c) all[c] = retired[c] ? retired[c].get() : tail[c].get();
+    Mesh out(m.dim, true);
+    BatchCtx nb;
+    batch_union(all, out, nb);
+    overwrite(m, std::move(out));
+    b = std::move(nb);
+  }
   stats.ttotal = now() - t_start;
   if (st) *st = stats;
 }

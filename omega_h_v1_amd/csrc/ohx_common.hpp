@@ -43,8 +43,10 @@ void fail(char const* what, char const* file, int line);
   } while (0)
 
 // --------------------------------------------------------------- runtime
-// The library's stream and a small pinned staging area for the scalars the
-// host reads back (scan totals, reduction results, convergence flags).
+// Per host thread: the library's stream and a small pinned staging area for
+// the scalars the host reads back (scan totals, reduction results,
+// convergence flags).  Threads driving different meshes therefore run on
+// streams of their own and never share scratch state.
 struct Runtime {
   hipStream_t stream = nullptr;
   void* pinned = nullptr;      // 4 KiB of mapped pinned host memory ...
@@ -53,7 +55,7 @@ struct Runtime {
 };
 Runtime& rt();
 // a caller-provided stream for the duration of one kernel-ABI call
-extern hipStream_t g_stream_override;
+extern thread_local hipStream_t g_stream_override;
 inline hipStream_t stream() { return g_stream_override ? g_stream_override : rt().stream; }
 struct StreamScope {
   hipStream_t prev;

@@ -277,7 +277,7 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
   if (nlows && nuses) {
     // offsets, uses in, (high, direction) out; lanes per bucket (4 to 32)
     // chosen online per (high, low) dimension pair and size class
-    static Tuner tuners[4][4] = {
+    static thread_local Tuner tuners[4][4] = {
         {Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
         {Tuner("upfd_rank_split_1_0", {4, 8, 16, 32}), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
         {Tuner("upfd_rank_split_2_0", {4, 8, 16, 32}), Tuner("upfd_rank_split_2_1", {4, 8, 16, 32}), Tuner("-", {4}),

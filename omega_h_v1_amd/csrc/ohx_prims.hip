@@ -77,14 +77,15 @@ T reduce(T const* a, u32 n, u32 begin, T init) {
 }  // namespace
 
 namespace {
+// per thread (the thread's stream): tile words never cleared, a ticket
 struct ScanState {
-  u64* tiles = nullptr;  // process-lifetime tile words (never cleared)
+  u64* tiles = nullptr;
   u32 cap = 0;
   u64* ticket = nullptr;
   u32 epoch = 0;
 };
 ScanState& scan_state() {
-  static ScanState st;
+  static thread_local ScanState st;
   return st;
 }
 }  // namespace
@@ -211,13 +212,13 @@ void compact_u32(u32* out, u32 const* a, u32 const* offs, u32 n, u32 width) {
 }
 
 u32* scratch_words() {
-  // allocated once for the life of the process; never returned to the arena
-  static u32* s = [] {
-    u32* w = static_cast<u32*>(dev_alloc(64 * sizeof(u32)));
+  // one per thread, for the life of the thread
+  static thread_local u32* w = nullptr;
+  if (!w) {
+    w = static_cast<u32*>(dev_alloc(64 * sizeof(u32)));
     fill(w, 64, 0);
-    return w;
-  }();
-  return s;
+  }
+  return w;
 }
 
 }  // namespace ohx

@@ -12,8 +12,8 @@
 
 namespace ohx {
 
-static std::string g_last_error;
-hipStream_t g_stream_override = nullptr;
+static thread_local std::string g_last_error;
+thread_local hipStream_t g_stream_override = nullptr;
 
 void fail(char const* what, char const* file, int line) {
   char buf[512];
@@ -25,8 +25,10 @@ void fail(char const* what, char const* file, int line) {
 std::string const& last_error() { return g_last_error; }
 void set_last_error(std::string const& s) { g_last_error = s; }
 
+// threads that drive meshes live until the process ends (the worker pool
+// never retires them), so a thread's stream and staging are never released
 Runtime& rt() {
-  static Runtime r;
+  static thread_local Runtime r;
   if (!r.stream) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -101,15 +103,21 @@ void read_many(unsigned const* const* ptrs, unsigned n, unsigned* out) {
 }
 
 // ------------------------------------------------------------------ arena
+// One caching arena for the process, guarded by a mutex.  A freed block is
+// reused only by allocations on the stream it was freed on: work on that
+// stream is ordered after the block's last use, so no synchronisation is
+// needed, and streams of different threads never hand blocks to each other
+// while their kernels may still read them.
 namespace {
 struct Arena {
-  std::map<size_t, std::vector<void*>> free_lists;
+  std::mutex mu;
+  std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> free_lists;
   std::unordered_map<void*, size_t> live;  // ptr -> class size
   size_t in_use = 0, high = 0, cached = 0;
 };
 Arena& arena() {
-  static Arena a;
-  return a;
+  static Arena* a = new Arena;  // never destroyed: blocks may outlive static teardown
+  return *a;
 }
 size_t size_class(size_t bytes) {
   size_t c = 256;
@@ -120,23 +128,29 @@ size_t size_class(size_t bytes) {
 
 void* dev_alloc(size_t bytes) {
   rt();
+  hipStream_t s = stream();
   Arena& a = arena();
   size_t c = size_class(bytes ? bytes : 1);
   void* p = nullptr;
-  auto it = a.free_lists.find(c);
-  if (it != a.free_lists.end() && !it->second.empty()) {
-    p = it->second.back();
-    it->second.pop_back();
-    a.cached -= c;
-  } else {
+  {
+    std::lock_guard<std::mutex> lock(a.mu);
+    auto it = a.free_lists.find({s, c});
+    if (it != a.free_lists.end() && !it->second.empty()) {
+      p = it->second.back();
+      it->second.pop_back();
+      a.cached -= c;
+    }
+  }
+  if (!p) {
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
-      // give cached blocks back and retry once
+      // give this stream's cached blocks back and retry once
       (void) hipGetLastError();
       dev_trim();
       OHX_CHECK(hipMalloc(&p, c));
     }
   }
+  std::lock_guard<std::mutex> lock(a.mu);
   a.live[p] = c;
   a.in_use += c;
   if (a.in_use > a.high) a.high = a.in_use;
@@ -146,28 +160,47 @@ void* dev_alloc(size_t bytes) {
 void dev_free(void* p) {
   if (!p) return;
   Arena& a = arena();
+  hipStream_t s = stream();
+  std::lock_guard<std::mutex> lock(a.mu);
   auto it = a.live.find(p);
   if (it == a.live.end()) fail("dev_free of an unknown pointer", __FILE__, __LINE__);
   size_t c = it->second;
   a.live.erase(it);
   a.in_use -= c;
-  a.free_lists[c].push_back(p);
+  a.free_lists[{s, c}].push_back(p);
   a.cached += c;
 }
 
-void dev_trim() {
+void trim_stream(hipStream_t s) {
   Arena& a = arena();
-  OHX_CHECK(hipStreamSynchronize(rt().stream));
+  std::lock_guard<std::mutex> lock(a.mu);
   for (auto& kv : a.free_lists) {
-    for (void* p : kv.second) OHX_CHECK(hipFree(p));
+    if (kv.first.first != s) continue;
+    for (void* p : kv.second) {
+      (void) hipFree(p);
+      a.cached -= kv.first.second;
+    }
     kv.second.clear();
   }
-  a.cached = 0;
 }
 
-size_t dev_bytes_in_use() { return arena().in_use; }
-size_t dev_bytes_high_water() { return arena().high; }
-void dev_reset_high_water() { arena().high = arena().in_use; }
+void dev_trim() {
+  OHX_CHECK(hipStreamSynchronize(stream()));
+  trim_stream(stream());
+}
+
+size_t dev_bytes_in_use() {
+  std::lock_guard<std::mutex> lock(arena().mu);
+  return arena().in_use;
+}
+size_t dev_bytes_high_water() {
+  std::lock_guard<std::mutex> lock(arena().mu);
+  return arena().high;
+}
+void dev_reset_high_water() {
+  std::lock_guard<std::mutex> lock(arena().mu);
+  arena().high = arena().in_use;
+}
 
 // ---------------------------------------------------------------- timing
 namespace {
@@ -189,7 +222,7 @@ struct Prof {
   std::map<std::string, Total> totals;
 };
 Prof& prof() {
-  static Prof p;
+  static thread_local Prof p;  // each thread times the kernels it launches
   return p;
 }
 hipEvent_t take_event() {
@@ -286,10 +319,11 @@ std::vector<std::string> prof_names() {
 namespace ohx {
 
 static std::vector<Tuner*>& tuners() {
-  static std::vector<Tuner*> v;
+  static thread_local std::vector<Tuner*> v;
   return v;
 }
 
+// pinned variants are set between runs (not while threads adapt meshes)
 static std::map<std::string, u32>& forced_variants() {
   static std::map<std::string, u32> m;
   return m;

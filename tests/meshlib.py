@@ -273,6 +273,23 @@ class Mesh:
         r = self.call("adapt_timed", size_floor, good_qual, nlayers, max_ops, _ptr(stats))
         return int(r), stats
 
+    # ---- many meshes at once (product only)
+    @staticmethod
+    def adapt_many(lib, meshes, size_floor=1.0 / 3.0, good_qual=0.3, nlayers=4, max_ops=50, nthreads=4):
+        d = lib.dll
+        f = d.ohx_adapt_many
+        f.restype = C.c_int
+        f.argtypes = [_u, C.POINTER(C.c_void_p), _d, _d, _u, _u, _u, C.c_void_p, C.c_void_p]
+        arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
+        res = np.zeros(len(meshes), dtype=np.int32)
+        stats = np.zeros(4, dtype=np.float64)
+        d.ohx_clear_error()
+        r = f(len(meshes), arr, size_floor, good_qual, nlayers, max_ops, nthreads, _ptr(res), _ptr(stats))
+        msg = d.ohx_last_error()
+        if msg:
+            raise RuntimeError("ohx_adapt_many: %s" % msg)
+        return int(r), res, stats
+
     # ---- batches (product only)
     @classmethod
     def batch(cls, lib, meshes):

@@ -96,3 +96,62 @@ def test_batch_operation_limit_is_per_component(product, oracle):
     singles = [_aniso_mesh(oracle, s) for s in specs]
     res = _check(batch, singles, 5)
     assert (res < 0).any() and (res > 0).any()
+
+
+@pytest.mark.parametrize("handoff", [0.0, 1.0])
+def test_batch_schedule_extremes_change_nothing(product, oracle, handoff):
+    """All batch passes (never handing over) and immediate hand-over to
+    single-mesh runs both give the single runs' meshes."""
+    import ctypes as C
+    d = product.dll
+    d.ohx_set_batch_handoff.argtypes = [C.c_double]
+    d.ohx_set_batch_handoff(handoff)
+    try:
+        specs = W.batch_specs(5, seed=5, n_lo=2, n_hi=4, h_lo=0.05, h_hi=0.5)
+        batch = ML.Mesh.batch(product, [_aniso_mesh(product, s) for s in specs])
+        singles = [_aniso_mesh(oracle, s) for s in specs]
+        _check(batch, singles, 50)
+    finally:
+        d.ohx_set_batch_handoff(0.25)
+
+
+def test_retiring_finished_components_changes_nothing(product, oracle):
+    """With retirement forced at every step the batch still equals the
+    single runs (finished components leave the live batch and come back at
+    the end, in order)."""
+    import ctypes as C
+    d = product.dll
+    d.ohx_set_batch_retire_min.argtypes = [C.c_double]
+    d.ohx_set_batch_retire_min(0.0)
+    d.ohx_set_batch_handoff.argtypes = [C.c_double]
+    d.ohx_set_batch_handoff(0.0)
+    try:
+        specs = W.batch_specs(5, seed=3, n_lo=2, n_hi=4, h_lo=0.05, h_hi=0.5)
+        batch = ML.Mesh.batch(product, [_aniso_mesh(product, s) for s in specs])
+        singles = [_aniso_mesh(oracle, s) for s in specs]
+        _check(batch, singles, 50)
+        cases = [(6, 0.08, True), (4, 0.3, True), (8, 0.02, True)]
+        batch = ML.Mesh.batch(product, [_iso2d_mesh(product, *c) for c in cases])
+        singles = [_iso2d_mesh(oracle, *c) for c in cases]
+        _check(batch, singles, 50)
+    finally:
+        d.ohx_set_batch_retire_min(1e6)
+        d.ohx_set_batch_handoff(0.25)
+
+
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_adapt_many_concurrent_streams_equal_single_runs(product, oracle, nthreads):
+    """Independent meshes adapted concurrently by worker threads, each on a
+    stream of its own, end as their single runs (3D anisotropic and 2D
+    isotropic meshes in one call)."""
+    specs = W.batch_specs(6, seed=9, n_lo=2, n_hi=4, h_lo=0.05, h_hi=0.5)
+    cases = [(6, 0.08, True), (10, 0.05, False), (8, 0.02, True)]
+    meshes = [_aniso_mesh(product, s) for s in specs] + [_iso2d_mesh(product, *c) for c in cases]
+    singles = [_aniso_mesh(oracle, s) for s in specs] + [_iso2d_mesh(oracle, *c) for c in cases]
+    r, res, stats = ML.Mesh.adapt_many(product, meshes, max_ops=50, nthreads=nthreads)
+    assert r == int((res < 0).sum())
+    for i, (m, s) in enumerate(zip(meshes, singles)):
+        assert int(res[i]) == s.adapt(1.0 / 3.0, 0.3, 4, 50), i
+        d = P.diff_states(m, s)
+        assert d is None, "mesh %d: %s" % (i, d)
+    assert stats[0] > 0 and stats[1] > 0 and stats[3] > 0
