@@ -63,8 +63,10 @@ H = 1.0 / 200.0
 WARP_FLOOR = 0.2
 ADAPT_ARGS = (1.0 / 3.0, 0.3, 4, 50)
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# on this workload, scaled by the same counters read for a copy kernel
-TRAFFIC = {}
+# on this workload (mean over the dispatches of a 2-step run), scaled by the
+# same counters read for a copy kernel of known size (FETCH_SIZE x 2,
+# WRITE_SIZE x 1)
+TRAFFIC = {"scan": 2641950}
 
 
 def prepare(lib, on_device):

@@ -104,8 +104,10 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
     parallel_for(n, OHX_LAMBDA(u32 i) { s[i] = filter[i] ? UNKNOWN : NOT_IN_SET; });
   }
   if (!n) return a;
-  // compulsory bytes per round: state in and out, goodness, the star CSR
-  double round_bytes = (double) n * 16 + (double) (n + 1) * 4 + (double) g.adj.size() * 4;
+  // compulsory bytes per round: every state read and written once (the
+  // goodness and the star are read only around undecided entities, a count
+  // the host does not know, so they are left out: a lower bound)
+  double round_bytes = (double) n * 8;
   // rounds are launched in batches; a round after convergence copies the
   // state unchanged, so running past the converged round is harmless
   constexpr u32 BATCH = 4;
