@@ -95,7 +95,12 @@ __global__ __launch_bounds__(BLOCK) void indset_round(u32 n, u32 const* __restri
   state[i] = IN_SET;
 }
 
-Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good) {
+// rounds == 0: rounds in batches with a host check after each, up to the
+// limit (the reference's loop); rounds > 0: exactly that many rounds and no
+// host check, *last_flag then points at the device word that is nonzero if
+// entities were still undecided after the last round
+static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double const* good, u32 rounds,
+                              u32 const** last_flag) {
   u32 n = count(m, ent_dim);
   Graph const& g = ask_star(m, ent_dim, m.dim);
   Buf<u32> a(n), b(n);
@@ -103,29 +108,60 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
     u32* s = a.get();
     parallel_for(n, OHX_LAMBDA(u32 i) { s[i] = filter[i] ? UNKNOWN : NOT_IN_SET; });
   }
-  if (!n) return a;
+  if (!n) {
+    if (rounds) {
+      fill(scratch_words() + W_FLAGS, 1, 0);
+      *last_flag = scratch_words() + W_FLAGS;
+    }
+    return a;
+  }
   // compulsory bytes per round: every state read and written once (the
   // goodness and the star are read only around undecided entities, a count
   // the host does not know, so they are left out: a lower bound)
   double round_bytes = (double) n * 8;
   // rounds are launched in batches; a round after convergence copies the
   // state unchanged, so running past the converged round is harmless
-  constexpr u32 BATCH = 4;
+  // rounds per host check: most sets settle within 8 rounds
+  constexpr u32 BATCH = 8;
   u32 const MAX_ROUNDS = g_indset_max_rounds;
   u32* flags = scratch_words() + W_FLAGS;
+  if (rounds) {
+    u32 nr = rounds < BATCH ? rounds : BATCH;
+    fill(flags, nr, 0);
+    for (u32 r = 0; r < nr; ++r) {
+      launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
+                   g.adj.get(), good, a.get(), b.get(), flags + r);
+      std::swap(a, b);
+    }
+    *last_flag = flags + nr - 1;
+    return a;
+  }
   for (u32 done = 0; done < MAX_ROUNDS; done += BATCH) {
-    fill(flags, BATCH, 0);
-    for (u32 r = 0; r < BATCH; ++r) {
+    // the last batch stops at the limit, as the reference's loop does
+    u32 nr = MAX_ROUNDS - done < BATCH ? MAX_ROUNDS - done : BATCH;
+    fill(flags, nr, 0);
+    for (u32 r = 0; r < nr; ++r) {
       launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
                          g.adj.get(), good, a.get(), b.get(), flags + r);
       std::swap(a, b);
     }
     u32 h[BATCH];
-    to_host(h, flags, BATCH);
-    if (!h[BATCH - 1]) return a;
+    to_host(h, flags, nr);
+    if (!h[nr - 1]) return a;
   }
   fail("independent set did not converge within the round limit (indset.cpp:107-116)", __FILE__, __LINE__);
   return a;
+}
+
+Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good) {
+  return indset_rounds(m, ent_dim, filter, good, 0, nullptr);
+}
+
+// a first batch of rounds with no host check; if the set has not settled
+// the caller starts over with find_indset (same rounds, same result)
+static Buf<u32> find_indset_bounded(Mesh& m, u32 ent_dim, u32 const* filter, double const* good,
+                                    u32 const** last_flag) {
+  return indset_rounds(m, ent_dim, filter, good, 8, last_flag);
 }
 
 // ------------------------------------------------------------- inherit
@@ -463,39 +499,39 @@ __global__ __launch_bounds__(BLOCK) void refine_products(RefineArgs a) {
   }
 }
 
-static void refine_ents(Mesh& m, Mesh& mo, u32 const* gen_off) {
-  OHX_REQUIRE(m.full, "refine needs a full-topology mesh");
-  u32 dim = m.dim;
-  RefineArgs base{};
-  base.dim = dim;
-  base.nv = count(m, 0);
-  Buf<u32> off[4];
-  base.off[1] = gen_off;  // an edge splits iff it is in the independent set
-  for (u32 d = 0; d <= dim; ++d) {
-    base.n[d] = count(m, d);
-    base.verts_of[d] = d ? ask_down(m, d, 0) : nullptr;
-  }
-  for (u32 d = 2; d <= dim; ++d) {
+// offsets of the splitting domains of each dimension d >= 2 (a domain
+// splits iff one of its edges does), enqueued without reading their totals
+static void refine_dom_offsets(Mesh& m, u32 const* gen_off, Buf<u32>* off) {
+  for (u32 d = 2; d <= m.dim; ++d) {
+    u32 n = count(m, d);
     u32 const* ed = ask_down(m, d, 1);
-    base.edges_of[d] = ed;
     u32 spd = h_down_deg[d][1];
-    off[d] = Buf<u32>(base.n[d] + 1);
-    exscan_fn(base.n[d], OHX_LAMBDA(u32 i) {
+    off[d] = Buf<u32>(n + 1);
+    exscan_fn(n, OHX_LAMBDA(u32 i) {
       for (u32 j = 0; j < spd; ++j) {
         u32 e = ed[(size_t) i * spd + j];
         if (gen_off[e] != gen_off[e + 1]) return 1u;
       }
       return 0u;
     }, off[d].get());
-    base.off[d] = off[d].get();
   }
-  u32 nsplit[4] = {0, 0, 0, 0};
-  {
-    u32 const* ptrs[3] = {base.off[1] + base.n[1], dim >= 2 ? base.off[2] + base.n[2] : nullptr,
-                          dim >= 3 ? base.off[3] + base.n[3] : nullptr};
-    u32 got[3];
-    read_many(ptrs, 3, got);
-    for (u32 d = 1; d <= dim; ++d) nsplit[d] = got[d - 1];
+}
+
+// nsplit[d]: splitting domains of dimension d (from the offsets' totals)
+static void refine_ents(Mesh& m, Mesh& mo, u32 const* gen_off, Buf<u32>* off, u32 const* nsplit) {
+  OHX_REQUIRE(m.full, "refine needs a full-topology mesh");
+  u32 dim = m.dim;
+  RefineArgs base{};
+  base.dim = dim;
+  base.nv = count(m, 0);
+  base.off[1] = gen_off;  // an edge splits iff it is in the independent set
+  for (u32 d = 0; d <= dim; ++d) {
+    base.n[d] = count(m, d);
+    base.verts_of[d] = d ? ask_down(m, d, 0) : nullptr;
+  }
+  for (u32 d = 2; d <= dim; ++d) {
+    base.edges_of[d] = ask_down(m, d, 1);
+    base.off[d] = off[d].get();
   }
   bool has_class = find_tag(m, 0, "class_dim") != nullptr;
   bool has_id = find_tag(m, 0, "class_id") != nullptr;
@@ -569,19 +605,40 @@ static void check_index_space(Mesh& m) {
 static u32 refine_common(Mesh& m, Buf<u32>& cand, double qual_floor, bool require_better,
                          double const* qfe = nullptr) {
   check_index_space(m);
-  u32 ns = count(m, 1);
+  u32 ns = count(m, 1), dim = m.dim;
   Buf<double> q = refine_qualities(m, cand.get(), qual_floor, require_better, qfe);
-  if (!read_word(W_ANY)) return 0;
-  Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
+  // one synchronisation for the whole decision: a first batch of indset
+  // rounds, the split offsets of every dimension, then one read of the
+  // convergence flag and the totals (a candidate-free call ends with an
+  // empty set and returns 0)
+  u32 const* flag = nullptr;
+  Buf<u32> indset = find_indset_bounded(m, 1, cand.get(), q.get(), &flag);
+  Buf<u32> gen_off(ns + 1);
+  Buf<u32> off[4];
+  u32 got[4];
+  auto offsets_and_totals = [&] {
+    exscan(indset.get(), ns, gen_off.get());
+    refine_dom_offsets(m, gen_off.get(), off);
+    u32 const* ptrs[4] = {flag, gen_off.get() + ns, dim >= 2 ? off[2].get() + count(m, 2) : nullptr,
+                          dim >= 3 ? off[3].get() + count(m, 3) : nullptr};
+    read_many(ptrs, 4, got);
+  };
+  offsets_and_totals();
+  if (got[0]) {  // not settled in the first batch: the full loop
+    indset = find_indset(m, 1, cand.get(), q.get());
+    flag = nullptr;
+    offsets_and_totals();
+  }
   if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();
   q.release();
-  Buf<u32> gen_off(ns + 1);
-  u32 total = exscan_total(indset.get(), ns, gen_off.get());
   indset.release();
+  u32 total = got[1];
+  if (!total) return 0;
+  u32 nsplit[4] = {0, total, got[2], got[3]};
   Mesh mo(m.dim, m.full);
   refine_verts(m, mo, gen_off.get(), total);
-  refine_ents(m, mo, gen_off.get());
+  refine_ents(m, mo, gen_off.get(), off, nsplit);
   say("split", total, 1);
   overwrite(m, std::move(mo));
   return 1;
@@ -883,7 +940,8 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   u32 ne = count(m, 1);
   check_basic_class(m, codes.get());
   for (u32 d = 2; d <= dim; ++d) check_exposed_sides(m, d, codes.get());
-  if (max_u32(codes.get(), ne) == DONT_COLLAPSE) return 0;
+  // no read here: the quality kernel skips edges without a collapse and
+  // flags whether any survive, which is read once below
   Buf<double> eq;
   u32 const* ev = ask_down(m, 1, 0);
   u32 const* vel = ask_down(m, dim, 0);
@@ -978,12 +1036,15 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       vqp[i] = is ? maxq : 0.0;
     });
   }
-  Buf<u32> indset = find_indset(m, 0, cand.get(), vq.get());
-  if (g_batch) batch_note_changed(m, 0, indset.get());
-  cand.release();
-  // vertex_dest (collapses_to_verts.cpp:72-135)
+  // a first batch of indset rounds, the collapse destinations and the
+  // vertex offsets, then one read of the convergence flag and the total
+  u32 const* flag = nullptr;
+  Buf<u32> indset = find_indset_bounded(m, 0, cand.get(), vq.get(), &flag);
   Buf<u32> gvv(nv);
-  {
+  Buf<u32> goff(nv + 1);
+  u32 got[2];
+  auto dest_and_total = [&] {
+    // vertex_dest (collapses_to_verts.cpp:72-135)
     u32 const* uo = uv.offsets.get();
     u32 const* ua = uv.adj.get();
     u32 const* ud = uv.dirs.get();
@@ -993,8 +1054,9 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32 const* is = indset.get();
     u32* g = gvv.get();
     u32* err = dev_error_word();
+    u32 const* fl = flag;
     parallel_for(nv, OHX_LAMBDA(u32 i) {
-      if (!is[i]) {
+      if (!is[i] || (fl && *fl)) {  // not in the set, or the set is not settled yet
         g[i] = INVALID;
         return;
       }
@@ -1014,13 +1076,23 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       if (gv == INVALID) *err = ERR_NO_DESTINATION;
       g[i] = gv;
     });
+    // coarsen_interior (coarsen_common.cpp:191-219)
+    exscan(indset.get(), nv, goff.get());
+    u32 const* ptrs[2] = {flag, goff.get() + nv};
+    read_many(ptrs, 2, got);
+  };
+  dest_and_total();
+  if (got[0]) {  // not settled in the first batch: the full loop
+    indset = find_indset(m, 0, cand.get(), vq.get());
+    flag = nullptr;
+    dest_and_total();
   }
+  if (g_batch) batch_note_changed(m, 0, indset.get());
+  cand.release();
   codes.release();
   cq.release();
   vq.release();
-  // coarsen_interior (coarsen_common.cpp:191-219)
-  Buf<u32> goff(nv + 1);
-  u32 total = exscan_total(indset.get(), nv, goff.get());
+  u32 total = got[1];
   indset.release();
   Buf<u32> soff = negate_offsets(goff.get(), nv);
   u32 nv_out = nv - total;
@@ -1687,6 +1759,39 @@ namespace {
 struct Ctx {
   u32 ops = 0, max_ops = 0;
   AdaptStats st;
+  // passes that changed the mesh: events around them on the stream, read
+  // once at the end (a pass's time is the stream's, from the end of the work
+  // before it to the end of its own, host waits included)
+  struct Timed {
+    hipEvent_t a, b;
+    double tin;
+  };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t event() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    OHX_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  void finish() {
+    for (Timed& t : timed) {
+      float ms = 0;
+      OHX_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+      st.npass += 1;
+      st.nelems += t.tin;
+      st.tpass += ms * 1e-3;
+      pool.push_back(t.a);
+      pool.push_back(t.b);
+    }
+    timed.clear();
+    for (hipEvent_t e : pool) (void) hipEventDestroy(e);
+    pool.clear();
+  }
 };
 // adapt_summary (adapt.cpp:22-34) only prints: it reads the mesh and changes
 // nothing, so it runs only when the per-pass lines are printed
@@ -1708,7 +1813,8 @@ bool incr_op(Mesh& m, Ctx& c) {  // adapt.cpp:36-45
 enum Kind { REFINE, COARSEN, SWAP, COARSEN_SLIVERS };
 u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n) {
   double tin = count(m, m.dim);
-  double t0 = now_s();
+  hipEvent_t e0 = c.event(), e1 = c.event();
+  OHX_CHECK(hipEventRecord(e0, stream()));
   u32 ok = 0;
   switch (k) {
     case REFINE: ok = refine_by_size(m, a); break;
@@ -1716,12 +1822,12 @@ u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n) {
     case SWAP: ok = swap_slivers(m, a, n); break;
     case COARSEN_SLIVERS: ok = coarsen_slivers(m, a, n); break;
   }
-  sync();
-  double t1 = now_s();
-  if (ok) {
-    c.st.npass += 1;
-    c.st.nelems += tin;
-    c.st.tpass += t1 - t0;
+  OHX_CHECK(hipEventRecord(e1, stream()));
+  if (ok)
+    c.timed.push_back(Ctx::Timed{e0, e1, tin});
+  else {
+    c.pool.push_back(e0);
+    c.pool.push_back(e1);
   }
   return ok;
 }
@@ -1756,6 +1862,7 @@ int adapt(Mesh& m, double size_floor, double good_qual, u32 nlayers, u32 max_ops
   }
   result = aborted ? -1 : (c.ops > 0 ? 1 : 0);
   sync();
+  c.finish();
   c.st.ttotal = now_s() - t0;
   if (st) *st = c.st;
   return result;

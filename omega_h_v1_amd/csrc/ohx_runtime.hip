@@ -56,7 +56,21 @@ static char const* dev_error_text(unsigned e) {
   return "unknown device error";
 }
 
+#ifdef OHX_TRACE_SYNC
+}  // namespace ohx
+#include <execinfo.h>
+namespace ohx {
+#endif
+
 void sync() {
+#ifdef OHX_TRACE_SYNC
+  if (prof_on()) {
+    void* fr[6];
+    int k = backtrace(fr, 6);
+    backtrace_symbols_fd(fr + 1, k - 1, 2);
+    std::fprintf(stderr, "--\n");
+  }
+#endif
   if (prof_on()) {
     // host-side count and wall time of the stream synchronisations
     auto t0 = std::chrono::steady_clock::now();
