@@ -187,6 +187,16 @@ int ohx_adapt_many(unsigned n, ohx_mesh* const* meshes, double size_floor,
     double good_qual, unsigned nlayers, unsigned max_ops, unsigned nthreads,
     int* results, double* stats) OSH_PUBLIC;
 
+/* read_mesh_vtk / write_mesh_vtk (vtk_io.cpp:787-830): the .vtu piece (and
+ * with a .pvtu name the .pvtu index too) in the reference's format; write
+ * returns 0, or -1 with ohx_last_error() */
+ohx_mesh* ohx_mesh_read_vtk(char const* path) OSH_PUBLIC;
+int ohx_mesh_write_vtk(ohx_mesh* m, char const* path) OSH_PUBLIC;
+/* 1: write arrays zlib-compressed (as a reference build with zlib does);
+ * 0 (default): uncompressed, as a reference build without zlib, readable by
+ * either.  Reading handles both. */
+void ohx_set_vtk_compression(int on) OSH_PUBLIC;
+
 /* the osh_t handle of a seam mesh (same object) */
 osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;
 

@@ -649,6 +649,29 @@ int ohx_adapt_many(unsigned n, ohx_mesh* const* meshes, double sf, double gq, un
   });
 }
 
+ohx_mesh* ohx_mesh_read_vtk(char const* path) {
+  return guard((ohx_mesh*) nullptr, [&]() {
+    auto* mm = new ohx_mesh(3, false);
+    try {
+      read_vtk(path, mm->m);
+      sync();
+    } catch (...) {
+      delete mm;
+      throw;
+    }
+    return mm;
+  });
+}
+
+int ohx_mesh_write_vtk(ohx_mesh* m, char const* path) {
+  return guard(-1, [&]() {
+    write_vtk(m->m, path);
+    return 0;
+  });
+}
+
+void ohx_set_vtk_compression(int on) { g_vtk_compress = on != 0; }
+
 unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qf) {
   return guard(OHX_ERROR, [&]() { return warp_to_limit(m->m, qf); });
 }
@@ -726,14 +749,25 @@ void osh_fini(void) {
 
 void osh_free(osh_t m) { delete M(m); }
 
-osh_t osh_read_vtk(char const*) {
-  std::fprintf(stderr, "omega_h (MI355X): osh_read_vtk is not available in this build\n");
-  std::abort();
+osh_t osh_read_vtk(char const* filename) {
+  return osh_guard([&]() {
+    auto* mm = new ohx_mesh(3, false);
+    try {
+      read_vtk(filename, mm->m);
+      sync();
+    } catch (...) {
+      delete mm;
+      throw;
+    }
+    return reinterpret_cast<osh_t>(mm);
+  });
 }
 
-void osh_write_vtk(osh_t, char const*) {
-  std::fprintf(stderr, "omega_h (MI355X): osh_write_vtk is not available in this build\n");
-  std::abort();
+void osh_write_vtk(osh_t m, char const* filename) {
+  osh_guard([&]() {
+    write_vtk(M(m)->m, filename);
+    return 0;
+  });
 }
 
 osh_t osh_new(unsigned dim) {

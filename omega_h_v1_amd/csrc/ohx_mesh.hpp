@@ -98,4 +98,10 @@ void bridge_graph(u32 n, u32 const* offs, u32 const* adj, u32* p_nedges, Buf<u32
 // setup
 void derive_model(Mesh& m, double crease_angle);
 
+// the VTU format of osh_read_vtk / osh_write_vtk (zlib-compressed arrays
+// when g_vtk_compress is set; read either way)
+extern bool g_vtk_compress;
+void write_vtk(Mesh& m, char const* path);
+void read_vtk(char const* path, Mesh& out);
+
 }  // namespace ohx

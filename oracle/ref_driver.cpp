@@ -32,6 +32,7 @@
 #include "refine.hpp"
 #include "size.hpp"
 #include "swap.hpp"
+#include "vtk_io.hpp"
 #include "tables.hpp"
 #include "tag.hpp"
 #include "warp_to_limit.hpp"
@@ -345,4 +346,16 @@ int ref_adapt_timed(void* vm, double size_floor, double good_qual,
   return ops > 0;
 }
 
+
+/* read_mesh_vtk / write_mesh_vtk (vtk_io.cpp:787-830), for the file-format
+ * parity tests */
+void* ref_read_vtk(char const* path)
+{
+  return read_mesh_vtk(path);
+}
+
+void ref_write_vtk(void* m, char const* path)
+{
+  write_mesh_vtk(static_cast<struct mesh*>(m), path);
+}
 }

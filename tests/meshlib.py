@@ -6,8 +6,9 @@ Three libraries expose the same mesh-level seam under different prefixes:
   product: the mesh lives in HBM and every kernel is a gfx950 HIP kernel;
 * ``oracle/_build/libohv1_oracle.so`` (prefix ``orc_``) -- the serial C
   restatement used as the checker;
-* ``oracle/_ref/libohv1_ref_serial.so`` / ``..._omp.so`` (prefix ``ref_``) --
-  the reference itself, compiled from its sources (only where it was built).
+* ``oracle/_ref/libohv1_ref_serial.so`` / ``..._omp.so`` / ``..._zlib.so``
+  (prefix ``ref_``) -- the reference itself, compiled from its sources (only
+  where it was built; the zlib build writes compressed VTU arrays).
 
 The wrapper gives them one Python interface so a parity test reads the same
 for every pair.
@@ -22,6 +23,7 @@ PRODUCT_LIB = os.environ.get("OHX_PRODUCT_LIB") or os.path.join(ROOT, "omega_h_v
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libohv1_oracle.so")
 REF_SERIAL_LIB = os.path.join(ROOT, "oracle", "_ref", "libohv1_ref_serial.so")
 REF_OMP_LIB = os.path.join(ROOT, "oracle", "_ref", "libohv1_ref_omp.so")
+REF_ZLIB_LIB = os.path.join(ROOT, "oracle", "_ref", "libohv1_ref_zlib.so")
 
 _u = C.c_uint
 _d = C.c_double
@@ -337,6 +339,43 @@ class Mesh:
             raise RuntimeError("ohx_mesh_batch_adapt: %s" % msg)
         return int(r), res, stats
 
+    # ---- VTU files (product and reference)
+    @classmethod
+    def read_vtk(cls, lib, path):
+        """osh_read_vtk's mesh: ohx_mesh_read_vtk (product) or
+        read_mesh_vtk (reference)."""
+        d = lib.dll
+        if lib.prefix == "ohx_mesh_":
+            f, dimf = d.ohx_mesh_read_vtk, d.osh_dim
+        else:
+            f, dimf = d.ref_read_vtk, d.ref_dim
+        f.restype, f.argtypes = C.c_void_p, [_s]
+        dimf.restype, dimf.argtypes = _u, [_p]
+        if lib.prefix == "ohx_mesh_":
+            d.ohx_clear_error()
+        h = f(path.encode())
+        if not h:
+            msg = d.ohx_last_error() if lib.prefix == "ohx_mesh_" else b"?"
+            raise RuntimeError("read_vtk %s: %s" % (path, msg))
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.h = h
+        self.dim = int(dimf(h))
+        return self
+
+    def write_vtk(self, path):
+        d = self.lib.dll
+        if self.lib.prefix == "ohx_mesh_":
+            f = d.ohx_mesh_write_vtk
+            f.restype, f.argtypes = C.c_int, [_p, _s]
+            d.ohx_clear_error()
+            if f(self.h, path.encode()) != 0:
+                raise RuntimeError("write_vtk %s: %s" % (path, d.ohx_last_error()))
+        else:
+            f = d.ref_write_vtk
+            f.restype, f.argtypes = None, [_p, _s]
+            f(self.h, path.encode())
+
     def metric_from_hessian(self, hessian, eps, hmin, hmax, out="metric"):
         self.call("metric_from_hessian", hessian.encode(), eps, hmin, hmax, out.encode())
 
@@ -388,12 +427,14 @@ def box(dim, n):
 
 
 def load(which):
-    """which in {'product', 'oracle', 'ref', 'ref_omp'}; None if not built."""
+    """which in {'product', 'oracle', 'ref', 'ref_omp', 'ref_zlib'}; None if
+    not built."""
     path, prefix = {
         "product": (PRODUCT_LIB, "ohx_mesh_"),
         "oracle": (ORACLE_LIB, "orc_"),
         "ref": (REF_SERIAL_LIB, "ref_"),
         "ref_omp": (REF_OMP_LIB, "ref_"),
+        "ref_zlib": (REF_ZLIB_LIB, "ref_"),
     }[which]
     if not os.path.exists(path):
         return None

@@ -109,6 +109,31 @@ ANISO_CASES = {
 }
 
 
+def vtu_case(case, lib):
+    """Final meshes of the VTU file tests: a freshly derived box, the last
+    state of a workload above, a 3D box after one mesh_adapt, a 3D box
+    carrying a 6-component metric tag."""
+    if case == "box2d_n3":
+        return make_mesh(lib, 2, 3)
+    if case == "adapt3d_n2":
+        m = make_mesh(lib, 3, 2)
+        m.add_field(0, "adapt_size", W.uniform_size(m.count(0), 0.5))
+        m.last = m.adapt(1.0 / 3.0, 0.3, 4, 50)
+        return m
+    if case == "metric3d_n2":
+        m = make_mesh(lib, 3, 2)
+        x = m.coords().reshape(-1, 3)
+        m.add_field(0, "metric", W.rotating_metric(x, 0.1, 4.0, dim=3).ravel())
+        return m
+    m = None
+    for _, m in CASES[case](lib):
+        pass
+    return m
+
+
+VTU_CASES = ["box2d_n3", "warp2d_sine_n16", "adapt3d_n2", "metric3d_n2"]
+
+
 def diff_states(a, b):
     """None if the two mesh states are bit-identical, else a description."""
     sa, sb = a.state(), b.state()

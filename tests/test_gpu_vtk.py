@@ -1,0 +1,163 @@
+"""osh_read_vtk / osh_write_vtk of the product against the reference's files.
+
+The fixtures of tests/golden/vtu/ were written by the compiled reference
+(uncompressed, zlib-compressed, .pvtu index); the VTU oracle reproduces them
+byte for byte (tests/test_vtu_oracle.py).  Here the product must:
+
+* write every fixture mesh, built and adapted on the GPU, to the reference's
+  exact bytes (both array encodings, and the .pvtu index with its piece);
+* read every fixture into the mesh the oracle builds (same arrays, bit for
+  bit), and adapt the mesh it read exactly as the oracle adapts its own;
+* at sizes beyond the fixtures, write what the oracle's writer makes of the
+  same arrays and read it back unchanged (write -> read round trip).
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import meshlib as ML
+import parity_cases as P
+from omega_h_v1_amd import workloads as W
+
+sys.path.insert(0, os.path.join(ML.ROOT, "oracle"))
+import vtu as V  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vtu")
+
+
+def _gold(name):
+    with open(os.path.join(GOLD, name), "rb") as f:
+        return f.read()
+
+
+@pytest.fixture(scope="module")
+def product():
+    lib = ML.load("product")
+    if lib is None:
+        pytest.fail("product library not built: run __graft_entry__.build() first")
+    lib.dll.ohx_set_vtk_compression.argtypes = [C.c_int]
+    return lib
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    lib = ML.load("oracle")
+    if lib is None:
+        pytest.fail("oracle library not built: make -C oracle")
+    return lib
+
+
+def _write(product, m, path, compress):
+    product.dll.ohx_set_vtk_compression(1 if compress else 0)
+    try:
+        m.write_vtk(str(path))
+    finally:
+        product.dll.ohx_set_vtk_compression(0)
+    return path.read_bytes()
+
+
+@pytest.mark.parametrize("case", P.VTU_CASES)
+def test_product_writes_the_reference_bytes(case, product, tmp_path):
+    m = P.vtu_case(case, product)
+    assert _write(product, m, tmp_path / "a.vtu", False) == _gold(case + ".vtu")
+    assert _write(product, m, tmp_path / "b.vtu", True) == _gold(case + "_z.vtu")
+
+
+def test_product_writes_the_pvtu_index(product, tmp_path):
+    m = P.vtu_case("warp2d_sine_n16", product)
+    m.write_vtk(str(tmp_path / "warp2d_sine_n16.pvtu"))
+    assert (tmp_path / "warp2d_sine_n16.pvtu").read_bytes() == _gold("warp2d_sine_n16.pvtu")
+    assert (tmp_path / "warp2d_sine_n16.vtu").read_bytes() == _gold("warp2d_sine_n16.vtu")
+    # a .pvtu name reads its piece
+    got = ML.Mesh.read_vtk(product, str(tmp_path / "warp2d_sine_n16.pvtu"))
+    assert P.diff_states(got, m) is None
+
+
+@pytest.mark.parametrize("case", P.VTU_CASES)
+def test_product_reads_the_reference_files(case, product, oracle, tmp_path):
+    want = P.vtu_case(case, oracle)
+    for name in (case + ".vtu", case + "_z.vtu"):
+        got = ML.Mesh.read_vtk(product, os.path.join(GOLD, name))
+        d = P.diff_states(got, want)
+        assert d is None, "%s: %s" % (name, d)
+        # what it read writes back as the oracle writes the file's piece
+        # (coordinates first, as every reader orders them)
+        again = _write(product, got, tmp_path / "r.vtu", False)
+        assert again == V.write_vtu(V.read_vtu(_gold(name)))
+
+
+@pytest.mark.parametrize("case", ["warp2d_sine_n16", "adapt3d_n2"])
+def test_a_mesh_read_from_file_adapts_like_the_oracle(case, product, oracle):
+    """The file's mesh is a complete adapt input: a finer size field, one
+    mesh_adapt, and the result equals the oracle's own mesh adapted alike."""
+    got = ML.Mesh.read_vtk(product, os.path.join(GOLD, case + ".vtu"))
+    want = P.vtu_case(case, oracle)
+    for m in (got, want):
+        m.free_field(0, "adapt_size")
+        m.add_field(0, "adapt_size", W.uniform_size(m.count(0), 0.6 / (4 if m.dim == 2 else 3)))
+    rg, rw = got.adapt(1.0 / 3.0, 0.3, 4, 50), want.adapt(1.0 / 3.0, 0.3, 4, 50)
+    assert rg == rw and rg != 0
+    d = P.diff_states(got, want)
+    assert d is None, d
+
+
+def test_seam_entry_points_match(product, tmp_path):
+    """osh_write_vtk / osh_read_vtk (the reference's public names) write and
+    read the same files as the ohx_ entry points."""
+    d = product.dll
+    d.osh_write_vtk.argtypes = [C.c_void_p, C.c_char_p]
+    d.osh_write_vtk.restype = None
+    d.osh_read_vtk.argtypes = [C.c_char_p]
+    d.osh_read_vtk.restype = C.c_void_p
+    d.osh_count.argtypes = [C.c_void_p, C.c_uint]
+    d.osh_count.restype = C.c_uint
+    d.osh_dim.argtypes = [C.c_void_p]
+    d.osh_dim.restype = C.c_uint
+    d.osh_free.argtypes = [C.c_void_p]
+    m = P.vtu_case("adapt3d_n2", product)
+    path = str(tmp_path / "seam.vtu")
+    d.osh_write_vtk(m.h, path.encode())
+    assert open(path, "rb").read() == _gold("adapt3d_n2.vtu")
+    h = d.osh_read_vtk(path.encode())
+    assert h
+    try:
+        assert d.osh_dim(h) == 3
+        assert [d.osh_count(h, k) for k in range(4)] == m.counts()
+    finally:
+        d.osh_free(h)
+
+
+def test_unreadable_files_are_errors(product, tmp_path):
+    with pytest.raises(RuntimeError):
+        ML.Mesh.read_vtk(product, str(tmp_path / "missing.vtu"))
+    bad = tmp_path / "bad.vtu"
+    bad.write_bytes(_gold("box2d_n3.vtu")[:400])
+    with pytest.raises(RuntimeError):
+        ML.Mesh.read_vtk(product, str(bad))
+    with pytest.raises(RuntimeError):
+        ML.Mesh.read_vtk(product, str(tmp_path / "nosuffix"))
+
+
+@pytest.mark.parametrize("dim,n", [(2, 160), (3, 16)])
+def test_round_trip_at_size(dim, n, product, tmp_path):
+    """51,200 triangles / 24,576 tetrahedra with f64 fields of 1 and 6
+    components: the product writes the oracle writer's bytes of its own
+    arrays, compressed and not, and reads them back unchanged."""
+    m = ML.Mesh.box_on_device(product, dim, n)
+    m.derive_model_full()
+    x = m.coords().reshape(-1, 3)
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), 0.01))
+    m.add_field(0, "metric", W.rotating_metric(x, 0.05, 3.0, dim=3).ravel())
+    p = V.piece_of(m)
+    for comp in (False, True):
+        path = tmp_path / ("big%d.vtu" % comp)
+        data = _write(product, m, path, comp)
+        assert data == V.write_vtu(p, compress=comp)
+        got = ML.Mesh.read_vtk(product, str(path))
+        d = P.diff_states(got, m)
+        assert d is None, d
