@@ -197,6 +197,14 @@ int ohx_mesh_write_vtk(ohx_mesh* m, char const* path) OSH_PUBLIC;
  * either.  Reading handles both. */
 void ohx_set_vtk_compression(int on) OSH_PUBLIC;
 
+/* mesh_diff (mesh_diff.cpp:141-178): 0 if the meshes have the same dimension,
+ * entity counts, connectivity and tags (integers exactly, FP64 to relative
+ * tolerance tol above the absolute floor), else 1 after printing what
+ * differs, as the reference does; with allow_superset the second mesh may
+ * hold extra tags and dimensions.  OHX_ERROR on failure. */
+unsigned ohx_mesh_diff(ohx_mesh* a, ohx_mesh* b, double tol, double floor,
+    unsigned allow_superset) OSH_PUBLIC;
+
 /* the osh_t handle of a seam mesh (same object) */
 osh_t ohx_mesh_osh(ohx_mesh* m) OSH_PUBLIC;
 

@@ -1,4 +1,5 @@
-"""Builds omega_h_v1_amd/lib/libomega_h_amd.so with hipcc for gfx950.
+"""Builds omega_h_v1_amd/lib/libomega_h_amd.so with hipcc for gfx950, and
+the command-line tools of omega_h_v1_amd/bin/ (vtkdiff) against it.
 
 Each .hip source compiles to an object in _build/ (in parallel), then one
 shared library is linked.  Flags that matter for results:
@@ -20,7 +21,11 @@ LIB = os.path.join(HERE, "lib", "libomega_h_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_batch.hip", "ohx_vtk.hip", "ohx_api.hip"]
+SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_batch.hip", "ohx_vtk.hip",
+           "ohx_diff.hip", "ohx_api.hip"]
+# host-only command-line tools linked against the library
+TOOLS = {"vtkdiff": "ohx_vtkdiff.cpp"}
+BIN = os.path.join(HERE, "bin")
 
 FLAGS = [
     "--offload-arch=" + ARCH,
@@ -63,6 +68,14 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s" % r.stderr[-4000:])
     os.replace(tmp, LIB)
+    os.makedirs(BIN, exist_ok=True)
+    for name, src in TOOLS.items():
+        cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, src),
+               "-o", os.path.join(BIN, name), "-L" + os.path.dirname(LIB), "-lomega_h_amd",
+               "-Wl,-rpath,$ORIGIN/../lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("building %s failed:\n%s" % (name, r.stderr[-4000:]))
     return LIB
 
 

@@ -672,6 +672,14 @@ int ohx_mesh_write_vtk(ohx_mesh* m, char const* path) {
 
 void ohx_set_vtk_compression(int on) { g_vtk_compress = on != 0; }
 
+unsigned ohx_mesh_diff(ohx_mesh* a, ohx_mesh* b, double tol, double floor, unsigned allow_superset) {
+  return guard(OHX_ERROR, [&]() {
+    u32 r = mesh_diff(a->m, b->m, tol, floor, allow_superset != 0);
+    std::fflush(stdout);
+    return r;
+  });
+}
+
 unsigned ohx_mesh_warp_to_limit(ohx_mesh* m, double qf) {
   return guard(OHX_ERROR, [&]() { return warp_to_limit(m->m, qf); });
 }

@@ -103,5 +103,7 @@ void derive_model(Mesh& m, double crease_angle);
 extern bool g_vtk_compress;
 void write_vtk(Mesh& m, char const* path);
 void read_vtk(char const* path, Mesh& out);
+// mesh_diff (mesh_diff.cpp:141-178): 0 if the meshes agree, else 1 with a message
+u32 mesh_diff(Mesh& a, Mesh& b, double tol, double floor, bool superset);
 
 }  // namespace ohx

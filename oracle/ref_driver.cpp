@@ -28,6 +28,7 @@
 #include "indset.hpp"
 #include "loop.hpp"
 #include "mesh.hpp"
+#include "mesh_diff.hpp"
 #include "quality.hpp"
 #include "refine.hpp"
 #include "size.hpp"
@@ -357,5 +358,14 @@ void* ref_read_vtk(char const* path)
 void ref_write_vtk(void* m, char const* path)
 {
   write_mesh_vtk(static_cast<struct mesh*>(m), path);
+}
+
+/* mesh_diff (mesh_diff.cpp:141-178), stdout flushed so a caller sees the
+ * messages in order */
+unsigned ref_mesh_diff(void* a, void* b, double tol, double floor, unsigned superset)
+{
+  unsigned r = mesh_diff(M(a), M(b), tol, floor, superset);
+  fflush(stdout);
+  return r;
 }
 }

@@ -1,4 +1,5 @@
-"""CPU oracle of the VTU file format of osh_read_vtk / osh_write_vtk.
+"""CPU oracle of the VTU file format of osh_read_vtk / osh_write_vtk, and of
+mesh_diff.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the product's
 VTU writer and reader; never by the product.
@@ -20,7 +21,9 @@ layout and its .pvtu index:
   from the first cell type, tags in file order, Rep="Full" or reduced.
 
 Pinned against files written by the compiled reference (tests/golden/vtu/):
-byte-identical writes and equal arrays on reads.
+byte-identical writes and equal arrays on reads.  mesh_diff (mesh_diff.cpp:
+16-178) is pinned against the compiled reference's own mesh_diff on planted
+differences: the same verdict and the same messages.
 """
 import base64
 import re
@@ -242,3 +245,88 @@ def piece_of(mesh, full=True):
             lst.append((name, len(a) // n if n else 0, a))
         tags[d] = lst
     return Piece(dim, full, mesh.count(0), conn, tags)
+
+
+def mesh_diff(a, b, tol=1e-6, floor=0.0, superset=False):
+    """(0 or 1, the lines the reference prints) for two Pieces
+    (mesh_diff.cpp:141-178).  FP64 arrays: relative difference against the
+    larger magnitude unless both are below floor; the maximum is the serial
+    build's, over entries 1.. only and blind to NaN (doubles.cpp:100-109)."""
+    out = []
+    if a.dim != b.dim:
+        return 1, ["mesh dimensionalities %u and %u differ" % (a.dim, b.dim)]
+
+    def has(p, d):
+        return d == 0 or d in p.conn
+
+    def f64_differ(x, y, n):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            fa, fb = np.abs(x), np.abs(y)
+            fm = np.where(fb > fa, fb, fa)
+            diffs = np.abs(x - y) / fm
+            diffs = np.where((fa < floor) & (fb < floor), 0.0, diffs)
+        rest = diffs[1:n]
+        rest = rest[~np.isnan(rest)]
+        maxdiff = -np.finfo(np.float64).max
+        if len(rest):
+            maxdiff = max(maxdiff, float(rest.max()))
+        if not maxdiff > tol:
+            return False
+        out.append("max relative difference %e" % maxdiff)
+        for i in np.nonzero(diffs == maxdiff)[0]:
+            out.append("max difference pair is (%e, %e), #%u" % (x[i], y[i], i))
+        return True
+
+    def tag_differ(ta, tb, n):
+        (na, ca, xa), (_, cb, xb) = ta, tb
+        if ca != cb:
+            out.append('tag "%s" sizes %u and %u differ' % (na, ca, cb))
+            return True
+        if xa.dtype != xb.dtype:
+            out.append('tag "%s" types differ' % na)
+            return True
+        if xa.dtype == np.float64:
+            bad = f64_differ(xa, xb, n * ca)
+        else:
+            bad = not np.array_equal(xa, xb)
+        if bad:
+            out.append('tag "%s" contents differ' % na)
+        return bad
+
+    def tags_differ(d, n):
+        tb_by = {t[0]: t for t in b.tags.get(d, [])}
+        names_a = {t[0] for t in a.tags.get(d, [])}
+        for ta in a.tags.get(d, []):
+            tb = tb_by.get(ta[0])
+            if tb is None:
+                if superset:
+                    continue
+                out.append('tag "%s" only on first mesh' % ta[0])
+                return True
+            if tag_differ(ta, tb, n):
+                return True
+        if not superset:
+            for tb in b.tags.get(d, []):
+                if tb[0] not in names_a:
+                    out.append('tag "%s" only on second mesh' % tb[0])
+                    return True
+        return False
+
+    for d in range(a.dim + 1):
+        ha, hb = has(a, d), has(b, d)
+        if not ha and not hb:
+            continue
+        if ha and not hb:
+            return 1, out + ["only first mesh has dimension %u" % d]
+        if hb and not ha:
+            if superset:
+                continue
+            return 1, out + ["only second mesh has dimension %u" % d]
+        if a.count(d) != b.count(d):
+            return 1, out + ["counts %u and %u differ for dimension %u" % (a.count(d), b.count(d), d)]
+        n = a.count(d)
+        if d and not np.array_equal(a.conn[d], b.conn[d]):
+            return 1, out + ["connectivity differs for dimension %u" % d]
+        if tags_differ(d, n):
+            return 1, out + ["tags for dimension %u differ" % d]
+    return 0, out

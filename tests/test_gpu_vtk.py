@@ -161,3 +161,55 @@ def test_round_trip_at_size(dim, n, product, tmp_path):
         got = ML.Mesh.read_vtk(product, str(path))
         d = P.diff_states(got, m)
         assert d is None, d
+
+
+def _files(tmp_path, a, b):
+    pa, pb = str(tmp_path / "a.vtu"), str(tmp_path / "b.vtu")
+    for path, p in ((pa, a), (pb, b)):
+        with open(path, "wb") as f:
+            f.write(V.write_vtu(p))
+    return pa, pb
+
+
+def test_mesh_diff_matches_the_oracle(product, tmp_path, capfd):
+    """ohx_mesh_diff on the GPU gives the oracle's verdict and prints the
+    oracle's lines (the oracle's mesh_diff equals the reference's) on every
+    planted difference."""
+    import diff_cases as D
+    f = product.dll.ohx_mesh_diff
+    f.restype, f.argtypes = C.c_uint, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_uint]
+    for name, a, b, tol, floor, superset in D.cases():
+        pa, pb = _files(tmp_path, a, b)
+        ma, mb = ML.Mesh.read_vtk(product, pa), ML.Mesh.read_vtk(product, pb)
+        C.CDLL(None).fflush(None)  # what earlier calls left in C's stdout buffer
+        capfd.readouterr()
+        r = f(ma.h, mb.h, tol, floor, 1 if superset else 0)
+        got = (int(r), capfd.readouterr().out.splitlines())
+        assert got == V.mesh_diff(a, b, tol, floor, superset), name
+
+
+def test_mesh_diff_of_adapted_meshes(product, oracle, tmp_path, capfd):
+    """A GPU-adapted mesh and the oracle's, through files: no difference at
+    tolerance 0; a coordinate nudged by one ulp is found."""
+    import subprocess
+    got = P.vtu_case("warp2d_sine_n16", product)
+    want = P.vtu_case("warp2d_sine_n16", oracle)
+    pa, pb = str(tmp_path / "gpu.vtu"), str(tmp_path / "cpu.vtu")
+    got.write_vtk(pa)
+    with open(pb, "wb") as fh:
+        fh.write(V.write_vtu(V.piece_of(want)))
+    tool = os.path.join(ML.ROOT, "omega_h_v1_amd", "bin", "vtkdiff")
+    r = subprocess.run([tool, "-tolerance", "0", pa, pb], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    p = V.piece_of(want)
+    name, nc, x = [t for t in p.tags[0] if t[0] == "coordinates"][0]
+    x = x.copy()
+    x[10] = np.nextafter(x[10], 2.0)
+    p.tags[0] = [(n, c, (x if n == "coordinates" else a)) for n, c, a in p.tags[0]]
+    with open(pb, "wb") as fh:
+        fh.write(V.write_vtu(p))
+    r = subprocess.run([tool, "-tolerance", "0", pa, pb], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert r.stdout.splitlines() == V.mesh_diff(V.piece_of(got), p, 0.0, 0.0, False)[1]
+    r = subprocess.run([tool, pa, pb], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0  # one ulp is within the default 1e-6

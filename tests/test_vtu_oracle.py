@@ -107,3 +107,48 @@ def test_reference_reads_oracle_files(oracle, tmp_path):
                 f.write(V.write_vtu(p, compress=comp))
             got = ML.Mesh.read_vtk(lib, path)
             assert P.diff_states(got, m) is None, (case, comp)
+
+
+def _ref_diff(lib, a, b, tol, floor, superset, tmp_path, capfd):
+    import ctypes as C
+    pa, pb = str(tmp_path / "a.vtu"), str(tmp_path / "b.vtu")
+    for path, p in ((pa, a), (pb, b)):
+        with open(path, "wb") as f:
+            f.write(V.write_vtu(p))
+    ma, mb = ML.Mesh.read_vtk(lib, pa), ML.Mesh.read_vtk(lib, pb)
+    f = lib.dll.ref_mesh_diff
+    f.restype, f.argtypes = C.c_uint, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_uint]
+    C.CDLL(None).fflush(None)  # what earlier calls left in C's stdout buffer
+    capfd.readouterr()
+    r = f(ma.h, mb.h, tol, floor, 1 if superset else 0)
+    return int(r), capfd.readouterr().out.splitlines()
+
+
+def test_mesh_diff_oracle_matches_the_reference(tmp_path, capfd):
+    """Verdict and printed lines of the oracle's mesh_diff equal the compiled
+    reference's on every planted difference (where the reference is built)."""
+    import diff_cases as D
+    ref = ML.load("ref")
+    if ref is None:
+        pytest.skip("reference checker not built here (oracle/_ref)")
+    seen = set()
+    for name, a, b, tol, floor, superset in D.cases():
+        want = _ref_diff(ref, a, b, tol, floor, superset, tmp_path, capfd)
+        got = V.mesh_diff(a, b, tol, floor, superset)
+        assert got == want, name
+        seen.add(got[0])
+    assert seen == {0, 1}
+
+
+def test_mesh_diff_oracle_verdicts():
+    """The verdicts the planted differences are built to give (this holds
+    without the reference too)."""
+    import diff_cases as D
+    expect0 = {"same", "coordinate_within_tolerance", "entry_zero_not_looked_at", "below_floor",
+               "zeros_are_nan_and_ignored", "tag_only_on_first_superset", "tag_only_on_second_superset",
+               "only_second_has_edges_superset"}
+    for name, a, b, tol, floor, superset in D.cases():
+        r, lines = V.mesh_diff(a, b, tol, floor, superset)
+        assert r == (0 if name in expect0 else 1), (name, lines)
+        if name == "two_max_pairs":
+            assert sum("max difference pair" in ln for ln in lines) == 2
