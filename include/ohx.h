@@ -116,6 +116,11 @@ unsigned ohx_mesh_ntags(ohx_mesh* m, unsigned dim) OSH_PUBLIC;
 char const* ohx_mesh_tag_name(ohx_mesh* m, unsigned dim, unsigned i) OSH_PUBLIC;
 void ohx_mesh_add_field(ohx_mesh* m, unsigned dim, char const* name,
     unsigned ncomps, double const* data) OSH_PUBLIC;
+/* osh_new_field (omega_h.cpp:613-622) with its data: an FP64 field whose
+ * enum osh_transfer type (1 pointwise, 2 conserve) makes an element field
+ * follow refine, coarsen and swap */
+void ohx_mesh_add_field_tt(ohx_mesh* m, unsigned dim, char const* name,
+    unsigned ncomps, double const* data, int transfer) OSH_PUBLIC;
 void ohx_mesh_free_field(ohx_mesh* m, unsigned dim, char const* name) OSH_PUBLIC;
 /* uniform field on every vertex, set on the device */
 void ohx_mesh_set_uniform_field(ohx_mesh* m, char const* name, double value) OSH_PUBLIC;

@@ -579,6 +579,7 @@ static void refine_ents(Mesh& m, Mesh& mo, u32 const* gen_off, Buf<u32>* off, u3
       add_tag_u32(mo, pd, "class_dim", 1, std::move(cdo));
       if (has_id) add_tag_u32(mo, pd, "class_id", 1, std::move(cio));
     }
+    if (pd == dim) refine_transfer(m, mo, soff.get(), base.off[dim], a.ppd[dim], a.go[dim], total);
   }
 }
 
@@ -1213,6 +1214,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
       if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
     }
+    if (d == dim) coarsen_transfer(m, mo, goff.get(), total, egoff.get(), esoff.get(), nsame, ngen);
   }
   say("collapsed", total, 1);
   overwrite(m, std::move(mo));
@@ -1738,6 +1740,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
       add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
       if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
     }
+    if (d == m.dim) swap_transfer(m, mo, goff.get(), soff.get(), nsame, ngen);
   }
   say("swapped", total, 1);
   overwrite(m, std::move(mo));

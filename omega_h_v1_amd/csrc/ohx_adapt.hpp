@@ -87,4 +87,12 @@ extern bool g_mixed;
 void mixed_stats(double* max_err, unsigned long long* screened, unsigned long long* reeval);
 void mixed_stats_reset();
 
+// solution transfer of the element fields tagged OSH_TRANSFER_CONSERVE /
+// OSH_TRANSFER_POINTWISE, called by each operation once its output
+// elements (and their class tags) exist
+void refine_transfer(Mesh& m, Mesh& mo, u32 const* soff, u32 const* off, u32 ppd, u32 go, u32 nout);
+void coarsen_transfer(Mesh& m, Mesh& mo, u32 const* vgoff, u32 ncol, u32 const* egoff, u32 const* esoff, u32 nsame,
+                      u32 ngen);
+void swap_transfer(Mesh& m, Mesh& mo, u32 const* egoff, u32 const* soff, u32 nsame, u32 ngen);
+
 }  // namespace ohx

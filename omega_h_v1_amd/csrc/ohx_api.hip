@@ -407,6 +407,15 @@ void ohx_mesh_add_field(ohx_mesh* m, unsigned d, char const* name, unsigned nc, 
   });
 }
 
+void ohx_mesh_add_field_tt(ohx_mesh* m, unsigned d, char const* name, unsigned nc, double const* data,
+                           int transfer) {
+  guard_void([&]() {
+    OHX_REQUIRE(transfer >= 0 && transfer <= 3, "transfer must be an enum osh_transfer value");
+    size_t n = (size_t) count(m->m, d) * nc;
+    add_tag_f64(m->m, d, name, nc, to_device(data, n), transfer);
+  });
+}
+
 void ohx_mesh_free_field(ohx_mesh* m, unsigned d, char const* name) {
   guard_void([&]() { remove_tag(m->m, d, name); });
 }

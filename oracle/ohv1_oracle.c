@@ -283,6 +283,7 @@ typedef struct {
   char name[64];
   int type;
   u32 ncomps;
+  int transfer; /* enum osh_transfer: 0 not, 1 pointwise, 2 conserve */
   void* data;
 } otag;
 
@@ -299,7 +300,9 @@ static otag* tag_find(otags* ts, char const* name)
   return 0;
 }
 
-static void tag_add(otags* ts, int type, char const* name, u32 ncomps, void* data)
+/* add_tag2 (tag.cpp:64-77) */
+static void tag_add2(otags* ts, int type, char const* name, u32 ncomps, int transfer,
+    void* data)
 {
   if (tag_find(ts, name))
     die("tag added twice");
@@ -309,8 +312,15 @@ static void tag_add(otags* ts, int type, char const* name, u32 ncomps, void* dat
   snprintf(t->name, sizeof(t->name), "%s", name);
   t->type = type;
   t->ncomps = ncomps;
+  t->transfer = transfer;
   t->data = data;
   ts->at[ts->n++] = t;
+}
+
+/* add_tag (tag.cpp:58-62): transfer type OSH_TRANSFER_NOT */
+static void tag_add(otags* ts, int type, char const* name, u32 ncomps, void* data)
+{
+  tag_add2(ts, type, name, ncomps, 0, data);
 }
 
 static void tag_remove(otags* ts, char const* name)
@@ -1618,6 +1628,387 @@ static void inherit_class(orc_mesh* m, orc_mesh* mo, u32 prod_dim, prods_t const
     inherit_tag(m, mo, prod_dim, p, "class_id");
 }
 
+/* ------------------------------------------------- solution transfer
+ * Element fields tagged OSH_TRANSFER_CONSERVE or OSH_TRANSFER_POINTWISE
+ * follow every operation; every other element tag is dropped by it. */
+#define TR_POINTWISE 1
+#define TR_CONSERVE 2
+#define MAX_NCOMPS 32 /* coarsen_conserve.cpp:11 */
+#define MAX_PTS 16    /* qr.hpp:6 */
+
+/* element_sizes / measure_entity (size.cpp:14-55, size.hpp:48-57) */
+static double* element_sizes(orc_mesh* m)
+{
+  u32 d = m->dim, n = mesh_count(m, d), vpe = d + 1;
+  u32 const* ev = ask_down(m, d, 0);
+  double const* x = coords_of(m);
+  double* out = NEW(double, n);
+  for (u32 i = 0; i < n; ++i) {
+    double ex[4][3];
+    for (u32 j = 0; j < vpe; ++j)
+      memcpy(ex[j], x + 3 * ev[i * vpe + j], sizeof(double) * 3);
+    if (d == 3)
+      out[i] = tet_volume((double const (*)[3]) ex);
+    else if (d == 2)
+      out[i] = tri_area((double const (*)[3]) ex);
+    else
+      out[i] = sqrt(vsqdist(ex[0], ex[1]));
+  }
+  return out;
+}
+
+/* interp_to_elems / average_element_field (element_field.cpp:10-45,
+ * element_field.hpp:8-25): the mean of the vertex values */
+static double* interp_to_elems(orc_mesh* m, double const* in, u32 nc)
+{
+  u32 d = m->dim, n = mesh_count(m, d), vpe = d + 1;
+  u32 const* ev = ask_down(m, d, 0);
+  double* out = NEW(double, (size_t) n * nc);
+  for (u32 i = 0; i < n; ++i) {
+    double* o = out + (size_t) i * nc;
+    for (u32 j = 0; j < nc; ++j)
+      o[j] = 0;
+    for (u32 k = 0; k < vpe; ++k)
+      for (u32 j = 0; j < nc; ++j)
+        o[j] += in[(size_t) ev[i * vpe + k] * nc + j];
+    for (u32 j = 0; j < nc; ++j)
+      o[j] /= vpe;
+  }
+  return out;
+}
+
+static int has_elem_tags(orc_mesh* m, int transfer)
+{
+  otags* ts = &m->tags[m->dim];
+  for (u32 i = 0; i < ts->n; ++i)
+    if (ts->at[i]->type == T_F64 && (transfer < 0 || ts->at[i]->transfer == transfer))
+      return 1;
+  return 0;
+}
+
+/* the element tag of the output: the same elements' values (offs_same over
+ * n old elements), then ngen generated values */
+static double* same_then_gen(otag const* t, u32 const* soff, u32 n, u32 ngen)
+{
+  u32 w = t->ncomps, nsame = soff[n];
+  double* out = NEW(double, (size_t)(nsame + ngen) * w);
+  double const* in = (double const*) t->data;
+  for (u32 i = 0; i < n; ++i)
+    if (soff[i] != soff[i + 1])
+      memcpy(out + (size_t) soff[i] * w, in + (size_t) i * w, sizeof(double) * w);
+  return out;
+}
+
+/* refine_conserve / refine_fit (refine_conserve.cpp:10-71,
+ * refine_fit.cpp:10-70), in that order (refine_common.cpp:110-113): the
+ * same elements keep their values; the children of a split element share
+ * its value equally (conserve) or copy it (pointwise) */
+static void refine_transfer(orc_mesh* m, orc_mesh* mo, prods_t const* p)
+{
+  u32 dim = m->dim;
+  u32 go[5];
+  ngen_offsets(p, go);
+  static int const kinds[2] = {TR_CONSERVE, TR_POINTWISE};
+  for (u32 kk = 0; kk < 2; ++kk)
+    for (u32 ti = 0; ti < m->tags[dim].n; ++ti) {
+      otag* t = m->tags[dim].at[ti];
+      if (t->type != T_F64 || t->transfer != kinds[kk])
+        continue;
+      u32 w = t->ncomps;
+      double const* in = (double const*) t->data;
+      double* out = NEW(double, (size_t) go[4] * w);
+      double* same = (double*) expand(in, sizeof(double), p->offs[0], p->ndoms[0], w);
+      memcpy(out + (size_t) go[0] * w, same, sizeof(double) * (go[1] - go[0]) * w);
+      free(same);
+      u32 const* po = p->offs[dim];
+      for (u32 i = 0; p->ndoms[dim] && po && i < p->ndoms[dim]; ++i) {
+        u32 f = po[i], e = po[i + 1];
+        double denom = e - f;
+        for (u32 j = f; j < e; ++j)
+          for (u32 k = 0; k < w; ++k) {
+            double v = in[(size_t) i * w + k];
+            out[(size_t)(go[dim] + j) * w + k] = (kinds[kk] == TR_CONSERVE) ? v / denom : v;
+          }
+      }
+      tag_add2(&mo->tags[dim], T_F64, t->name, w, kinds[kk], out);
+    }
+}
+
+static double sign_of(double x) { return (x < 0) ? -1 : 1; } /* qr.cpp:10-13 */
+
+/* get_reflector2 (qr.cpp:40-62) */
+static u32 get_reflector2(double a[MAX_PTS][4], double v[MAX_PTS], u32 k, u32 npts)
+{
+  double cnorm = 0;
+  for (u32 i = k; i < npts; ++i)
+    cnorm += a[i][k] * a[i][k];
+  cnorm = sqrt(cnorm);
+  if (cnorm < 1e-6)
+    return 0;
+  for (u32 i = 0; i < k; ++i)
+    v[i] = 0;
+  for (u32 i = k; i < npts; ++i)
+    v[i] = a[i][k];
+  v[k] += sign_of(a[k][k]) * cnorm;
+  double rnorm = 0;
+  for (u32 i = k; i < npts; ++i)
+    rnorm += v[i] * v[i];
+  rnorm = sqrt(rnorm);
+  for (u32 i = k; i < npts; ++i)
+    v[i] /= rnorm;
+  return 1;
+}
+
+/* qr_decomp2 (qr.cpp:151-168) with reflect_columns2 / reflect_rows2
+ * (qr.cpp:76-86, 100-110): Householder QR of the npts x 4 matrix a; Q is
+ * accumulated explicitly; returns the number of reflections taken */
+static u32 qr_decomp2(double a[MAX_PTS][4], double q[MAX_PTS][MAX_PTS], double r[MAX_PTS][4],
+    u32 npts)
+{
+  for (u32 i = 0; i < npts; ++i)
+    for (u32 j = 0; j < 4; ++j)
+      r[i][j] = a[i][j];
+  for (u32 i = 0; i < npts; ++i)
+    for (u32 j = 0; j < npts; ++j)
+      q[i][j] = (double) (i == j);
+  double v[MAX_PTS] = {0};
+  u32 rank = 0;
+  for (u32 k = 0; k < 4; ++k)
+    if (get_reflector2(r, v, k, npts)) {
+      for (u32 j = 0; j < 4; ++j) {
+        double dot = 0;
+        for (u32 i = k; i < npts; ++i)
+          dot += r[i][j] * v[i];
+        for (u32 i = k; i < npts; ++i)
+          r[i][j] -= 2 * dot * v[i];
+      }
+      for (u32 i = 0; i < npts; ++i) {
+        double dot = 0;
+        for (u32 j = k; j < npts; ++j)
+          dot += q[i][j] * v[j];
+        for (u32 j = k; j < npts; ++j)
+          q[i][j] -= 2 * dot * v[j];
+      }
+      ++rank;
+    }
+  return rank;
+}
+
+/* qr_solve2 (qr.cpp:170-200): y = Q^T b over the first 4 columns, then
+ * back substitution with R */
+static void qr_solve2(double q[MAX_PTS][MAX_PTS], double r[MAX_PTS][4], double const* b,
+    u32 npts, double x[4])
+{
+  double y[4];
+  for (u32 i = 0; i < 4; ++i) {
+    y[i] = 0;
+    for (u32 j = 0; j < npts; ++j)
+      y[i] += q[j][i] * b[j];
+  }
+  for (u32 ii = 0; ii < 4; ++ii) {
+    u32 i = 4 - ii - 1;
+    x[i] = y[i];
+    for (u32 j = i + 1; j < 4; ++j)
+      x[i] -= r[i][j] * x[j];
+    x[i] /= r[i][i];
+  }
+}
+
+/* coarsen_conserve (coarsen_conserve.cpp:15-139) and coarsen_fit
+ * (coarsen_fit.cpp:17-187) for the elements of a coarsen: cavity = the
+ * old elements around a collapsing vertex.  Conserve: the cavity's total
+ * is split over its new elements by new size.  Pointwise (3D only, the
+ * reference asserts): a least-squares linear fit over the first MAX_PTS
+ * old element centroids, evaluated at each new centroid; fewer than 4
+ * points or a rank-deficient fit give the cavity average.  The fit leaves
+ * an element "coordinates" tag (the centroids) on the old mesh, computed
+ * only if absent, and on the new mesh, before the pointwise tags. */
+static void coarsen_transfer(orc_mesh* m, orc_mesh* mo, u32 const* gen_off_v, u32 const* goff,
+    u32 const* soff)
+{
+  u32 dim = m->dim, nv = mesh_count(m, 0), n = mesh_count(m, dim);
+  u32 nsame = soff[n], ngen = goff[n];
+  oup const* up = ask_up(m, 0, dim);
+  if (has_elem_tags(m, -1)) {
+    double* sizes = element_sizes(mo);
+    for (u32 ti = 0; ti < m->tags[dim].n; ++ti) {
+      otag* t = m->tags[dim].at[ti];
+      if (t->type != T_F64 || t->transfer != TR_CONSERVE)
+        continue;
+      u32 w = t->ncomps;
+      if (w > MAX_NCOMPS)
+        die("coarsen_conserve: too many components");
+      double const* in = (double const*) t->data;
+      double* out = same_then_gen(t, soff, n, ngen);
+      double* gen = out + (size_t) nsame * w;
+      for (u32 v = 0; v < nv; ++v) {
+        if (gen_off_v[v] == gen_off_v[v + 1])
+          continue;
+        u32 f = up->offsets[v], e = up->offsets[v + 1];
+        double sum[MAX_NCOMPS] = {0};
+        for (u32 j = f; j < e; ++j)
+          for (u32 k = 0; k < w; ++k)
+            sum[k] = sum[k] + in[(size_t) up->adj[j] * w + k];
+        double cav = 0;
+        for (u32 j = f; j < e; ++j) {
+          u32 el = up->adj[j];
+          if (goff[el] != goff[el + 1])
+            cav += sizes[goff[el] + nsame];
+        }
+        for (u32 j = f; j < e; ++j) {
+          u32 el = up->adj[j];
+          if (goff[el] == goff[el + 1])
+            continue;
+          double s = sizes[goff[el] + nsame] / cav;
+          for (u32 k = 0; k < w; ++k)
+            gen[(size_t) goff[el] * w + k] = sum[k] * s;
+        }
+      }
+      tag_add2(&mo->tags[dim], T_F64, t->name, w, TR_CONSERVE, out);
+    }
+    free(sizes);
+  }
+  if (!has_elem_tags(m, TR_POINTWISE))
+    return;
+  if (dim != 3)
+    die("coarsen_fit: pointwise element fields need a 3D mesh");
+  if (!find_tag(m, dim, "coordinates"))
+    tag_add(&m->tags[dim], T_F64, "coordinates", 3, interp_to_elems(m, coords_of(m), 3));
+  tag_add(&mo->tags[dim], T_F64, "coordinates", 3, interp_to_elems(mo, coords_of(mo), 3));
+  double const* oc = (double const*) find_tag(m, dim, "coordinates")->data;
+  double const* nc = (double const*) find_tag(mo, dim, "coordinates")->data;
+  for (u32 ti = 0; ti < m->tags[dim].n; ++ti) {
+    otag* t = m->tags[dim].at[ti];
+    if (t->type != T_F64 || t->transfer != TR_POINTWISE)
+      continue;
+    u32 w = t->ncomps;
+    if (w > MAX_NCOMPS)
+      die("coarsen_fit: too many components");
+    double const* in = (double const*) t->data;
+    double* out = same_then_gen(t, soff, n, ngen);
+    double* gen = out + (size_t) nsame * w;
+    for (u32 v = 0; v < nv; ++v) {
+      if (gen_off_v[v] == gen_off_v[v + 1])
+        continue;
+      u32 f = up->offsets[v], e = up->offsets[v + 1];
+      u32 npts = e - f;
+      if (npts > MAX_PTS)
+        npts = MAX_PTS;
+      int fit = npts >= 4;
+      double q[MAX_PTS][MAX_PTS], r[MAX_PTS][4];
+      if (fit) {
+        double a[MAX_PTS][4];
+        for (u32 j = 0; j < npts; ++j) {
+          u32 el = up->adj[f + j];
+          a[j][0] = 1.0;
+          memcpy(&a[j][1], oc + 3 * (size_t) el, sizeof(double) * 3);
+        }
+        if (qr_decomp2(a, q, r, npts) < 4)
+          fit = 0;
+      }
+      for (u32 k = 0; k < w; ++k) {
+        double c[4], avg = 0;
+        if (fit) {
+          double b[MAX_PTS];
+          for (u32 j = 0; j < npts; ++j)
+            b[j] = in[(size_t) up->adj[f + j] * w + k];
+          qr_solve2(q, r, b, npts, c);
+        } else {
+          for (u32 j = f; j < e; ++j)
+            avg += in[(size_t) up->adj[j] * w + k];
+          avg /= (e - f);
+        }
+        for (u32 j = f; j < e; ++j) {
+          u32 el = up->adj[j];
+          if (goff[el] == goff[el + 1])
+            continue;
+          u32 ge = goff[el];
+          if (fit) {
+            double const* xc = nc + 3 * (size_t)(ge + nsame);
+            double d = 0;
+            for (u32 l = 0; l < 3; ++l)
+              d += c[1 + l] * xc[l];
+            gen[(size_t) ge * w + k] = c[0] + d;
+          } else {
+            gen[(size_t) ge * w + k] = avg;
+          }
+        }
+      }
+    }
+    tag_add2(&mo->tags[dim], T_F64, t->name, w, TR_POINTWISE, out);
+  }
+}
+
+/* swap_conserve (swap_conserve.cpp:15-131) and swap_fit (swap_fit.cpp:
+ * 16-127): cavity = the old elements around a swapped edge; conserve
+ * splits the cavity total by new size, pointwise takes the cavity average
+ * (3D only, the reference asserts) */
+static void swap_transfer(orc_mesh* m, orc_mesh* mo, u32 const* goff_e, u32 const* soff)
+{
+  u32 dim = m->dim, ne = mesh_count(m, 1), n = mesh_count(m, dim);
+  u32 nsame = soff[n], ngen = goff_e[ne];
+  oup const* up = ask_up(m, 1, dim);
+  if (has_elem_tags(m, -1)) {
+    double* sizes = element_sizes(mo);
+    for (u32 ti = 0; ti < m->tags[dim].n; ++ti) {
+      otag* t = m->tags[dim].at[ti];
+      if (t->type != T_F64 || t->transfer != TR_CONSERVE)
+        continue;
+      u32 w = t->ncomps;
+      if (w > MAX_NCOMPS)
+        die("swap_conserve: too many components");
+      double const* in = (double const*) t->data;
+      double* out = same_then_gen(t, soff, n, ngen);
+      double* gen = out + (size_t) nsame * w;
+      for (u32 i = 0; i < ne; ++i) {
+        if (goff_e[i] == goff_e[i + 1])
+          continue;
+        double sum[MAX_NCOMPS] = {0};
+        for (u32 j = up->offsets[i]; j < up->offsets[i + 1]; ++j)
+          for (u32 k = 0; k < w; ++k)
+            sum[k] = sum[k] + in[(size_t) up->adj[j] * w + k];
+        double cav = 0;
+        for (u32 j = goff_e[i]; j < goff_e[i + 1]; ++j)
+          cav += sizes[nsame + j];
+        for (u32 j = goff_e[i]; j < goff_e[i + 1]; ++j) {
+          double s = sizes[nsame + j] / cav;
+          for (u32 k = 0; k < w; ++k)
+            gen[(size_t) j * w + k] = sum[k] * s;
+        }
+      }
+      tag_add2(&mo->tags[dim], T_F64, t->name, w, TR_CONSERVE, out);
+    }
+    free(sizes);
+  }
+  if (!has_elem_tags(m, TR_POINTWISE))
+    return;
+  if (dim != 3)
+    die("swap_fit: pointwise element fields need a 3D mesh");
+  for (u32 ti = 0; ti < m->tags[dim].n; ++ti) {
+    otag* t = m->tags[dim].at[ti];
+    if (t->type != T_F64 || t->transfer != TR_POINTWISE)
+      continue;
+    u32 w = t->ncomps;
+    double const* in = (double const*) t->data;
+    double* out = same_then_gen(t, soff, n, ngen);
+    double* gen = out + (size_t) nsame * w;
+    for (u32 i = 0; i < ne; ++i) {
+      if (goff_e[i] == goff_e[i + 1])
+        continue;
+      u32 f = up->offsets[i], e = up->offsets[i + 1];
+      for (u32 k = 0; k < w; ++k) {
+        double avg = 0;
+        for (u32 j = f; j < e; ++j)
+          avg += in[(size_t) up->adj[j] * w + k];
+        avg /= (e - f);
+        for (u32 g = goff_e[i]; g < goff_e[i + 1]; ++g)
+          gen[(size_t) g * w + k] = avg;
+      }
+    }
+    tag_add2(&mo->tags[dim], T_F64, t->name, w, TR_POINTWISE, out);
+  }
+}
+
 /* ------------------------------------------------------------ refine */
 
 /* refine_quality (refine_qualities.cpp:16-82): worst child quality of
@@ -1852,6 +2243,8 @@ static void refine_ents(orc_mesh* m, orc_mesh* mo, u32 src_dim, u32 const* gen_o
       mesh_set_ents(mo, pd, np, vo);
     }
     inherit_class(m, mo, pd, &p);
+    if (pd == dim)
+      refine_transfer(m, mo, &p);
     prods_free(&p);
   }
   for (u32 d = 0; d <= dim; ++d) {
@@ -2201,6 +2594,8 @@ static void coarsen_ents(orc_mesh* m, orc_mesh* mo, u32 ed, u32 const* gen_off_v
   p.ndoms[ed] = n;
   p.offs[ed] = goff;
   inherit_class(m, mo, ed, &p);
+  if (ed == m->dim)
+    coarsen_transfer(m, mo, gen_off_v, goff, soff);
   free(goff);
   free(soff);
 }
@@ -2690,6 +3085,8 @@ static void swap_ents(orc_mesh* m, orc_mesh* mo, u32 d, u32 const* indset,
     p.offs[1] = goff;
     inherit_class(m, mo, d, &p);
   }
+  if (d == m->dim)
+    swap_transfer(m, mo, goff, soff);
   free(goff);
   free(soff);
 }
@@ -2994,6 +3391,30 @@ void orc_add_field(orc_mesh* m, unsigned dim, char const* name, unsigned ncomps,
 {
   u32 n = mesh_count(m, dim) * ncomps;
   tag_add(&m->tags[dim], T_F64, name, ncomps, f64_copy(data, n));
+}
+
+/* the fit of coarsen_fit on its own: rank of the npts x 4 row-major a,
+ * and the least-squares solution when the rank is 4 */
+unsigned orc_qr_fit(unsigned npts, double const* a, double const* b, double* x)
+{
+  double aa[MAX_PTS][4], q[MAX_PTS][MAX_PTS], r[MAX_PTS][4];
+  if (npts < 4 || npts > MAX_PTS)
+    die("orc_qr_fit: 4 to 16 points");
+  for (u32 i = 0; i < npts; ++i)
+    for (u32 j = 0; j < 4; ++j)
+      aa[i][j] = a[i * 4 + j];
+  u32 rank = qr_decomp2(aa, q, r, npts);
+  if (rank == 4)
+    qr_solve2(q, r, b, npts, x);
+  return rank;
+}
+
+/* osh_new_field with a transfer type (omega_h.cpp:613-622) */
+void orc_add_field_tt(orc_mesh* m, unsigned dim, char const* name, unsigned ncomps,
+    double const* data, int transfer)
+{
+  u32 n = mesh_count(m, dim) * ncomps;
+  tag_add2(&m->tags[dim], T_F64, name, ncomps, transfer, f64_copy(data, n));
 }
 
 void orc_free_field(orc_mesh* m, unsigned dim, char const* name)

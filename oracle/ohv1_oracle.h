@@ -46,6 +46,11 @@ unsigned orc_ntags(orc_mesh* m, unsigned dim);
 char const* orc_tag_name(orc_mesh* m, unsigned dim, unsigned i);
 void orc_add_field(orc_mesh* m, unsigned dim, char const* name,
     unsigned ncomps, double const* data);
+/* Householder least-squares fit of coarsen_fit (4 to 16 rows of 4) */
+unsigned orc_qr_fit(unsigned npts, double const* a, double const* b, double* x);
+/* a field with an enum osh_transfer type (0 not, 1 pointwise, 2 conserve) */
+void orc_add_field_tt(orc_mesh* m, unsigned dim, char const* name,
+    unsigned ncomps, double const* data, int transfer);
 void orc_free_field(orc_mesh* m, unsigned dim, char const* name);
 
 double orc_min_quality(orc_mesh* m);

@@ -29,6 +29,7 @@
 #include "loop.hpp"
 #include "mesh.hpp"
 #include "mesh_diff.hpp"
+#include "qr.hpp"
 #include "quality.hpp"
 #include "refine.hpp"
 #include "size.hpp"
@@ -162,6 +163,15 @@ void ref_add_field(void* m, unsigned dim, char const* name, unsigned ncomps,
 {
   unsigned n = mesh_count(M(m), dim) * ncomps;
   mesh_add_tag(M(m), dim, TAG_F64, name, ncomps, array_to_device(data, n));
+}
+
+/* osh_new_field (omega_h.cpp:613-622): a field with a transfer type */
+void ref_add_field_tt(void* m, unsigned dim, char const* name, unsigned ncomps,
+    double const* data, int transfer)
+{
+  unsigned n = mesh_count(M(m), dim) * ncomps;
+  add_tag2(mesh_tags(M(m), dim), TAG_F64, name, ncomps,
+      static_cast<enum osh_transfer>(transfer), array_to_device(data, n));
 }
 
 void ref_free_field(void* m, unsigned dim, char const* name)
@@ -358,6 +368,22 @@ void* ref_read_vtk(char const* path)
 void ref_write_vtk(void* m, char const* path)
 {
   write_mesh_vtk(static_cast<struct mesh*>(m), path);
+}
+
+/* qr_decomp2 + qr_solve2 (qr.cpp:151-200) on an npts x 4 row-major matrix:
+ * the rank found, and the solution when it is 4 */
+unsigned ref_qr_fit(unsigned npts, double const* a, double const* b, double* x)
+{
+  double aa[MAX_PTS][4], q[MAX_PTS][MAX_PTS], r[MAX_PTS][4], bb[MAX_PTS];
+  for (unsigned i = 0; i < npts; ++i) {
+    for (unsigned j = 0; j < 4; ++j)
+      aa[i][j] = a[i * 4 + j];
+    bb[i] = b[i];
+  }
+  unsigned rank = qr_decomp2(aa, q, r, npts);
+  if (rank == 4)
+    qr_solve2(q, r, bb, npts, x);
+  return rank;
 }
 
 /* mesh_diff (mesh_diff.cpp:141-178), stdout flushed so a caller sees the

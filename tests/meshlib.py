@@ -48,6 +48,7 @@ _SIG = {
     "tag_name": (_s, [_p, _u, _u]),
     "add_field": (None, [_p, _u, _s, _u, _p]),
     "free_field": (None, [_p, _u, _s]),
+    "add_field_tt": (None, [_p, _u, _s, _u, _p, C.c_int]),
     "min_quality": (_d, [_p]),
     "qualities": (None, [_p, _p]),
     "measure_edges": (None, [_p, _p]),
@@ -218,10 +219,15 @@ class Mesh:
     def tag_names(self, dim):
         return [self.call("tag_name", dim, i).decode() for i in range(self.call("ntags", dim))]
 
-    def add_field(self, dim, name, data):
+    def add_field(self, dim, name, data, transfer=None):
+        """A float64 field; transfer (enum osh_transfer: 1 pointwise,
+        2 conserve) makes an element field follow the mesh operations."""
         data = np.ascontiguousarray(data, dtype=np.float64)
         ncomps = data.size // self.count(dim)
-        self.call("add_field", dim, name.encode(), ncomps, _ptr(data))
+        if transfer is None:
+            self.call("add_field", dim, name.encode(), ncomps, _ptr(data))
+        else:
+            self.call("add_field_tt", dim, name.encode(), ncomps, _ptr(data), int(transfer))
 
     def free_field(self, dim, name):
         self.call("free_field", dim, name.encode())

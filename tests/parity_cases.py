@@ -109,6 +109,76 @@ ANISO_CASES = {
 }
 
 
+POINTWISE, CONSERVE = 1, 2
+
+
+def add_element_fields(m, pointwise=True):
+    """Element fields of every transfer kind: conserved 1- and 3-component
+    fields, pointwise linear and nonlinear fields (3D only: the reference
+    fits pointwise fields in 3D alone), and one without a transfer type,
+    which the first operation drops."""
+    dim = m.dim
+    x = m.coords().reshape(-1, 3)
+    ev = m.down(dim, 0).reshape(-1, dim + 1).astype(np.int64)
+    cen = x[ev].sum(1) / (dim + 1)
+    n = len(ev)
+    m.add_field(dim, "mass", 1.0 + cen[:, 0] + 0.5 * cen[:, 1] ** 2, transfer=CONSERVE)
+    mom = np.stack([cen[:, 1], -cen[:, 0], 0.25 + 0 * cen[:, 2]], 1)
+    m.add_field(dim, "momentum", mom.ravel(), transfer=CONSERVE)
+    m.add_field(dim, "junk", np.arange(n, dtype=np.float64))
+    if pointwise and dim == 3:
+        lin = np.stack([1 + 2 * cen[:, 0] - cen[:, 2], 0.5 * cen[:, 1], 3 - cen[:, 0] + cen[:, 1]], 1)
+        m.add_field(dim, "velocity", lin.ravel(), transfer=POINTWISE)
+        m.add_field(dim, "temperature", np.sin(3 * cen[:, 0]) * np.cos(2 * cen[:, 2]), transfer=POINTWISE)
+
+
+def transfer_c1_steps(lib, on_device=False):
+    """The C1 steps (3D Kuhn box 8^3, h = 0.05: refine, coarsen, swap) with
+    element fields of every transfer kind."""
+    m = make_mesh(lib, 3, 8, on_device)
+    add_element_fields(m)
+    yield "fields", m
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), 0.05))
+    qf = min(m.min_quality(), 0.3)
+    m.last = m.refine_by_size(qf)
+    yield "refine", m
+    m.last = m.coarsen_by_size(qf, 1.0 / 3.0)
+    yield "coarsen", m
+    m.last = m.swap_slivers(0.3, 4)
+    yield "swap", m
+
+
+def transfer_warp_steps(lib, dim, n, h, warp, cycles, on_device=False):
+    """warp_steps with element fields attached before the first cycle."""
+    steps = warp_steps(lib, dim, n, h, warp, cycles=cycles, on_device=on_device)
+    label, m = next(steps)
+    add_element_fields(m)
+    yield label, m
+    for label, m in steps:
+        yield label, m
+
+
+def transfer_coarsen_steps(lib, on_device=False):
+    """3D box 6^3 with a large size: collapses only, so the pointwise fit
+    runs on cavities of every size (and leaves its element centroids)."""
+    m = make_mesh(lib, 3, 6, on_device)
+    add_element_fields(m)
+    yield "fields", m
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), 0.6))
+    for i in range(4):
+        m.last = m.coarsen_by_size(0.2, 1.0 / 3.0)
+        yield "coarsen%d" % i, m
+
+
+TRANSFER_CASES = {
+    "transfer_c1": lambda lib, dev=False: transfer_c1_steps(lib, dev),
+    "transfer_coarsen3d_n6": lambda lib, dev=False: transfer_coarsen_steps(lib, dev),
+    "transfer_warp3d_rot_n4": lambda lib, dev=False: transfer_warp_steps(lib, 3, 4, 0.15, "rot", 2, on_device=dev),
+    "transfer_warp2d_sine_n16": lambda lib, dev=False: transfer_warp_steps(lib, 2, 16, 1 / 12.0, "sine", 3,
+                                                                           on_device=dev),
+}
+
+
 def vtu_case(case, lib):
     """Final meshes of the VTU file tests: a freshly derived box, the last
     state of a workload above, a 3D box after one mesh_adapt, a 3D box

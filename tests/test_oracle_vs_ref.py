@@ -22,3 +22,15 @@ def test_oracle_equals_reference(case, libs):
         assert getattr(ma, "last", None) == getattr(mb, "last", None)
         d = P.diff_states(ma, mb)
         assert d is None, "%s after %s: %s" % (case, la, d)
+
+
+@pytest.mark.parametrize("case", sorted(P.TRANSFER_CASES))
+def test_oracle_transfer_equals_reference(case, libs):
+    """Element fields of every transfer kind follow refine, coarsen and swap
+    exactly as in the reference (every array, every step)."""
+    orc, ref = libs
+    for (la, ma), (lb, mb) in zip(P.TRANSFER_CASES[case](orc), P.TRANSFER_CASES[case](ref)):
+        assert la == lb
+        assert getattr(ma, "last", None) == getattr(mb, "last", None)
+        d = P.diff_states(ma, mb)
+        assert d is None, "%s after %s: %s" % (case, la, d)
