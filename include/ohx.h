@@ -122,6 +122,21 @@ void ohx_mesh_add_field(ohx_mesh* m, unsigned dim, char const* name,
 void ohx_mesh_add_field_tt(ohx_mesh* m, unsigned dim, char const* name,
     unsigned ncomps, double const* data, int transfer) OSH_PUBLIC;
 void ohx_mesh_free_field(ohx_mesh* m, unsigned dim, char const* name) OSH_PUBLIC;
+/* the size-field producers of the reference's warp loop (test_warp.cpp:
+ * 70-83): mesh_element_gradients (element_gradients.cpp:66-89) adds the
+ * element field "grad_<name>" (3 components per component of the vertex
+ * field name); mesh_recover_by_volume (recover_by_volume.cpp:52-66) adds
+ * the vertex field name, the size-weighted mean of the element field name;
+ * mesh_size_from_hessian (size_from_hessian.cpp:44-72) adds the vertex
+ * field "adapt_size" = max_h - sum_j weights[j] |H_j|_F, at least min_h
+ * (weights: one per 9 components, or NULL for all 1); and
+ * mesh_identity_size_field (size.cpp:90-135), the longest edge at each
+ * vertex, as osh_identity_size */
+void ohx_mesh_element_gradients(ohx_mesh* m, char const* name) OSH_PUBLIC;
+void ohx_mesh_recover_by_volume(ohx_mesh* m, char const* name) OSH_PUBLIC;
+void ohx_mesh_size_from_hessian(ohx_mesh* m, char const* hess_name,
+    double const* weights, double min_h, double max_h) OSH_PUBLIC;
+void ohx_mesh_identity_size(ohx_mesh* m, char const* name) OSH_PUBLIC;
 /* uniform field on every vertex, set on the device */
 void ohx_mesh_set_uniform_field(ohx_mesh* m, char const* name, double value) OSH_PUBLIC;
 

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "lib", "libomega_h_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_transfer.hip", "ohx_batch.hip", "ohx_vtk.hip",
+SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_transfer.hip", "ohx_fields.hip", "ohx_batch.hip", "ohx_vtk.hip",
            "ohx_diff.hip", "ohx_api.hip"]
 # host-only command-line tools linked against the library
 TOOLS = {"vtkdiff": "ohx_vtkdiff.cpp"}

@@ -149,27 +149,6 @@ static ohx_mesh* clone_mesh(ohx_mesh* src) {
   return mm;
 }
 
-// identity size (size.cpp:90-135): longest adjacent edge per vertex
-static void identity_size(Mesh& m, char const* name) {
-  Graph const& g = ask_star(m, 0, 1);
-  u32 nv = count(m, 0);
-  double const* x = coords(m);
-  Buf<double> out(nv);
-  double* o = out.get();
-  u32 const* go = g.offsets.get();
-  u32 const* ga = g.adj.get();
-  parallel_for(nv, OHX_LAMBDA(u32 i) {
-    V3 a = ld3(x + 3 * (size_t) i);
-    double mx = 0;
-    for (u32 j = go[i]; j < go[i + 1]; ++j) {
-      double l = sqrt(sqdist(ld3(x + 3 * (size_t) ga[j]), a));
-      if (l > mx) mx = l;
-    }
-    o[i] = mx;
-  });
-  add_tag_f64(m, 0, name, 1, std::move(out));
-}
-
 extern "C" {
 
 char const* ohx_last_error(void) { return t_has_err ? t_err.c_str() : nullptr; }
@@ -414,6 +393,23 @@ void ohx_mesh_add_field_tt(ohx_mesh* m, unsigned d, char const* name, unsigned n
     size_t n = (size_t) count(m->m, d) * nc;
     add_tag_f64(m->m, d, name, nc, to_device(data, n), transfer);
   });
+}
+
+void ohx_mesh_element_gradients(ohx_mesh* m, char const* name) {
+  guard_void([&]() { element_gradients(m->m, name); });
+}
+
+void ohx_mesh_recover_by_volume(ohx_mesh* m, char const* name) {
+  guard_void([&]() { recover_by_volume(m->m, name); });
+}
+
+void ohx_mesh_size_from_hessian(ohx_mesh* m, char const* hess_name, double const* weights, double min_h,
+                                double max_h) {
+  guard_void([&]() { size_from_hessian(m->m, hess_name, weights, min_h, max_h); });
+}
+
+void ohx_mesh_identity_size(ohx_mesh* m, char const* name) {
+  guard_void([&]() { identity_size(m->m, name); });
 }
 
 void ohx_mesh_free_field(ohx_mesh* m, unsigned d, char const* name) {

@@ -98,6 +98,15 @@ void bridge_graph(u32 n, u32 const* offs, u32 const* adj, u32* p_nedges, Buf<u32
 // setup
 void derive_model(Mesh& m, double crease_angle);
 
+// fields computed on the mesh (ohx_fields.hip): element sizes; the size-field
+// producers element_gradients.cpp, recover_by_volume.cpp,
+// size_from_hessian.cpp; the identity size (size.cpp:90-135)
+Buf<double> element_sizes(Mesh& m);
+void element_gradients(Mesh& m, char const* name);
+void recover_by_volume(Mesh& m, char const* name);
+void size_from_hessian(Mesh& m, char const* hess_name, double const* weights, double min_h, double max_h);
+void identity_size(Mesh& m, char const* name);
+
 // the VTU format of osh_read_vtk / osh_write_vtk (zlib-compressed arrays
 // when g_vtk_compress is set; read either way)
 extern bool g_vtk_compress;

@@ -46,12 +46,6 @@ std::vector<Tag*> fields_of(Mesh& m, int kind) {
   return out;
 }
 
-bool any_f64(Mesh& m) {
-  for (auto& tp : m.tags[m.dim])
-    if (tp->type == TagType::F64) return true;
-  return false;
-}
-
 // the output tags (in the input's order), as lists of at most 8
 std::vector<FieldList> make_outputs(Mesh& m, Mesh& mo, std::vector<Tag*> const& ts, int kind, u32 nout) {
   std::vector<FieldList> out;
@@ -107,30 +101,6 @@ __global__ __launch_bounds__(BLOCK) void copy_same(u32 n, u32 const* __restrict_
     u32 w = L.ncomps[t];
     for (u32 c = 0; c < w; ++c) L.dst[t][(size_t) o * w + c] = L.src[t][(size_t) i * w + c];
   }
-}
-
-// element_sizes (size.cpp:14-55): volume or area
-__global__ __launch_bounds__(BLOCK) void element_sizes_k(u32 n, u32 dim, u32 const* __restrict__ ev,
-                                                         double const* __restrict__ x, double* out) {
-  u32 i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  u32 const* v = ev + (size_t) i * (dim + 1);
-  if (dim == 3)
-    out[i] = tet_volume(ld3(x + 3 * (size_t) v[0]), ld3(x + 3 * (size_t) v[1]), ld3(x + 3 * (size_t) v[2]),
-                        ld3(x + 3 * (size_t) v[3]));
-  else
-    out[i] = tri_area(ld3(x + 3 * (size_t) v[0]), ld3(x + 3 * (size_t) v[1]), ld3(x + 3 * (size_t) v[2]));
-}
-
-Buf<double> element_sizes(Mesh& m) {
-  u32 n = count(m, m.dim);
-  Buf<double> out(n);
-  if (n) {
-    hipLaunchKernelGGL(element_sizes_k, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, m.dim, ask_down(m, m.dim, 0),
-                       coords(m), out.get());
-    check_launch("element_sizes");
-  }
-  return out;
 }
 
 // average_element_field (element_field.hpp:8-25): ((0 + x0) + x1 ...) / vpe

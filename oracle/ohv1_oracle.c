@@ -3547,3 +3547,188 @@ double orc_metric_length(double const* a6, double const* b6, double const* x0, d
   sub3(x1, x0, e);
   return sqrt(metric_len2(a6, b6, e));
 }
+
+/* ------------------------------------------------ size-field producers
+ * The warp loop's size field (test_warp.cpp:70-83): element gradients of a
+ * vertex field, recovered to vertices by volume weighting, twice, then a
+ * size from the Hessian's norm; and the identity size field. */
+
+/* element_jacobian (jacobian.cpp:7-17): column i = edge from vertex 0 to
+ * vertex i + 1 */
+static void element_jacobian(u32 nv, double (*x)[3], double jac[3][3])
+{
+  for (u32 i = 0; i < 3; ++i)
+    for (u32 j = 0; j < 3; ++j)
+      jac[i][j] = 0;
+  for (u32 i = 0; i < nv; ++i)
+    for (u32 j = 0; j < 3; ++j)
+      jac[j][i] = x[i + 1][j] - x[0][j];
+}
+
+/* invert_jacobian (jacobian.cpp:19-66, algebra.hpp:143-196) for 2 and 3 */
+static void invert_jacobian(u32 dim, double in[3][3], double out[3][3])
+{
+  if (dim == 3) {
+    double tmp[3][3];
+    double c[3];
+    cross(in[0], in[1], c);
+    double d = dot3(in[2], c);
+    cross(in[1], in[2], tmp[0]);
+    cross(in[2], in[0], tmp[1]);
+    cross(in[0], in[1], tmp[2]);
+    double s = 1.0 / d;
+    for (u32 i = 0; i < 3; ++i)
+      for (u32 j = 0; j < 3; ++j)
+        out[i][j] = tmp[j][i] * s;
+    return;
+  }
+  double nrm[2][2];
+  for (u32 i = 0; i < 2; ++i)
+    for (u32 j = 0; j < 2; ++j) {
+      nrm[i][j] = 0;
+      for (u32 k = 0; k < 3; ++k)
+        nrm[i][j] += in[i][k] * in[j][k];
+    }
+  double det = nrm[0][0] * nrm[1][1] - nrm[0][1] * nrm[1][0];
+  double inv[2][2];
+  inv[0][0] = nrm[1][1] / det;
+  inv[0][1] = -nrm[0][1] / det;
+  inv[1][0] = -nrm[1][0] / det;
+  inv[1][1] = nrm[0][0] / det;
+  for (u32 i = 0; i < 3; ++i)
+    for (u32 j = 0; j < 3; ++j)
+      out[i][j] = 0;
+  for (u32 i = 0; i < 3; ++i)
+    for (u32 j = 0; j < 2; ++j)
+      for (u32 k = 0; k < 2; ++k)
+        out[i][j] += in[k][i] * inv[k][j];
+}
+
+/* mesh_element_gradients (element_gradients.cpp:15-91): element tag
+ * "grad_<name>" with 3 components per input component */
+void orc_element_gradients(orc_mesh* m, char const* name)
+{
+  u32 dim = m->dim, n = mesh_count(m, dim), vpe = dim + 1;
+  otag* t = find_tag(m, 0, name);
+  if (!t || t->type != T_F64)
+    die("element_gradients: no such vertex field");
+  u32 nc = t->ncomps, nco = 3 * nc;
+  double const* f = (double const*) t->data;
+  double const* x = coords_of(m);
+  u32 const* ev = ask_down(m, dim, 0);
+  double* out = NEW(double, (size_t) n * nco);
+  for (u32 i = 0; i < n; ++i) {
+    double ex[4][3];
+    double const* ec[4];
+    for (u32 j = 0; j < vpe; ++j) {
+      u32 v = ev[i * vpe + j];
+      ec[j] = f + (size_t) v * nc;
+      memcpy(ex[j], x + 3 * (size_t) v, sizeof(double) * 3);
+    }
+    double jac[3][3], jaci[3][3];
+    element_jacobian(dim, ex, jac);
+    invert_jacobian(dim, jac, jaci);
+    double* g = out + (size_t) i * nco;
+    for (u32 j = 0; j < nco; ++j)
+      g[j] = 0;
+    for (u32 j = 0; j < dim; ++j)
+      for (u32 k = 0; k < nc; ++k)
+        for (u32 l = 0; l < 3; ++l)
+          g[k * 3 + l] += jaci[j][l] * (ec[j + 1][k] - ec[0][k]);
+  }
+  char gname[80];
+  snprintf(gname, sizeof(gname), "grad_%s", name);
+  tag_add(&m->tags[dim], T_F64, gname, nco, out);
+}
+
+/* mesh_recover_by_volume (recover_by_volume.cpp:10-68): the vertex field
+ * name = the size-weighted mean of the element field name around it */
+void orc_recover_by_volume(orc_mesh* m, char const* name)
+{
+  u32 dim = m->dim, nv = mesh_count(m, 0);
+  otag* t = find_tag(m, dim, name);
+  if (!t || t->type != T_F64)
+    die("recover_by_volume: no such element field");
+  u32 nc = t->ncomps;
+  double const* f = (double const*) t->data;
+  double* sizes = element_sizes(m);
+  oup const* up = ask_up(m, 0, dim);
+  double* out = NEW(double, (size_t) nv * nc);
+  for (u32 i = 0; i < nv; ++i) {
+    double* o = out + (size_t) i * nc;
+    for (u32 j = 0; j < nc; ++j)
+      o[j] = 0;
+    double ssum = 0;
+    for (u32 j = up->offsets[i]; j < up->offsets[i + 1]; ++j) {
+      u32 el = up->adj[j];
+      double es = sizes[el];
+      ssum += es;
+      for (u32 k = 0; k < nc; ++k)
+        o[k] += es * f[(size_t) el * nc + k];
+    }
+    for (u32 j = 0; j < nc; ++j)
+      o[j] /= ssum;
+  }
+  free(sizes);
+  tag_add(&m->tags[0], T_F64, name, nc, out);
+}
+
+/* mesh_size_from_hessian (size_from_hessian.cpp:11-72): per vertex the
+ * weighted sum of the Frobenius norms of the 3x3 Hessians (one per
+ * solution component), then max_h - that, floored at min_h, as the vertex
+ * field "adapt_size" */
+void orc_size_from_hessian(orc_mesh* m, char const* hname, double const* weights, double min_h,
+    double max_h)
+{
+  u32 nv = mesh_count(m, 0);
+  otag* t = find_tag(m, 0, hname);
+  if (!t || t->type != T_F64 || t->ncomps % 9)
+    die("size_from_hessian: needs a vertex field of 9 components per solution component");
+  if (!(max_h > min_h) || !(min_h > 0))
+    die("size_from_hessian: needs 0 < min_h < max_h");
+  u32 nh = t->ncomps, ns = nh / 9;
+  double const* h = (double const*) t->data;
+  double* out = NEW(double, nv);
+  for (u32 i = 0; i < nv; ++i) {
+    double const* hv = h + (size_t) i * nh;
+    double total = 0;
+    for (u32 j = 0; j < ns; ++j) {
+      double d = 0;
+      for (u32 k = 0; k < 9; ++k)
+        d += hv[k] * hv[k];
+      double nrm = sqrt(d);
+      double w = weights ? weights[j] : 1;
+      if (!(w >= 0))
+        die("size_from_hessian: negative weight");
+      total += w * nrm;
+      hv += 9;
+    }
+    out[i] = total;
+  }
+  for (u32 i = 0; i < nv; ++i) {
+    out[i] = max_h - out[i];
+    if (out[i] < min_h)
+      out[i] = min_h;
+  }
+  tag_add(&m->tags[0], T_F64, "adapt_size", 1, out);
+}
+
+/* mesh_identity_size_field (size.cpp:90-135): the longest edge at each
+ * vertex */
+void orc_identity_size(orc_mesh* m, char const* name)
+{
+  u32 nv = mesh_count(m, 0);
+  ograph const* g = ask_star(m, 0, 1);
+  double const* x = coords_of(m);
+  double* out = NEW(double, nv);
+  for (u32 i = 0; i < nv; ++i) {
+    double mx = 0;
+    for (u32 j = g->offsets[i]; j < g->offsets[i + 1]; ++j) {
+      double l = sqrt(vsqdist(x + 3 * (size_t) g->adj[j], x + 3 * (size_t) i));
+      if (l > mx)
+        mx = l;
+    }
+    out[i] = mx;
+  }
+  tag_add(&m->tags[0], T_F64, name, 1, out);
+}

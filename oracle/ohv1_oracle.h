@@ -46,6 +46,13 @@ unsigned orc_ntags(orc_mesh* m, unsigned dim);
 char const* orc_tag_name(orc_mesh* m, unsigned dim, unsigned i);
 void orc_add_field(orc_mesh* m, unsigned dim, char const* name,
     unsigned ncomps, double const* data);
+/* size-field producers (element_gradients.cpp, recover_by_volume.cpp,
+ * size_from_hessian.cpp, size.cpp:90-135); weights may be null (all 1) */
+void orc_element_gradients(orc_mesh* m, char const* name);
+void orc_recover_by_volume(orc_mesh* m, char const* name);
+void orc_size_from_hessian(orc_mesh* m, char const* hess_name, double const* weights,
+    double min_h, double max_h);
+void orc_identity_size(orc_mesh* m, char const* name);
 /* Householder least-squares fit of coarsen_fit (4 to 16 rows of 4) */
 unsigned orc_qr_fit(unsigned npts, double const* a, double const* b, double* x);
 /* a field with an enum osh_transfer type (0 not, 1 pointwise, 2 conserve) */

@@ -23,6 +23,7 @@
 #include "coarsen.hpp"
 #include "derive_model.hpp"
 #include "edge_swap.hpp"
+#include "element_gradients.hpp"
 #include "doubles.hpp"
 #include "graph.hpp"
 #include "indset.hpp"
@@ -31,8 +32,10 @@
 #include "mesh_diff.hpp"
 #include "qr.hpp"
 #include "quality.hpp"
+#include "recover_by_volume.hpp"
 #include "refine.hpp"
 #include "size.hpp"
+#include "size_from_hessian.hpp"
 #include "swap.hpp"
 #include "vtk_io.hpp"
 #include "tables.hpp"
@@ -369,6 +372,17 @@ void ref_write_vtk(void* m, char const* path)
 {
   write_mesh_vtk(static_cast<struct mesh*>(m), path);
 }
+
+/* the size-field producers of the warp loop (test_warp.cpp:70-83) and
+ * the identity size (size.cpp:90-135) */
+void ref_element_gradients(void* m, char const* name) { mesh_element_gradients(M(m), name); }
+void ref_recover_by_volume(void* m, char const* name) { mesh_recover_by_volume(M(m), name); }
+void ref_size_from_hessian(void* m, char const* hess_name, double const* weights,
+    double min_h, double max_h)
+{
+  mesh_size_from_hessian(M(m), hess_name, weights, min_h, max_h);
+}
+void ref_identity_size(void* m, char const* name) { mesh_identity_size_field(M(m), name); }
 
 /* qr_decomp2 + qr_solve2 (qr.cpp:151-200) on an npts x 4 row-major matrix:
  * the rank found, and the solution when it is 4 */

@@ -49,6 +49,10 @@ _SIG = {
     "add_field": (None, [_p, _u, _s, _u, _p]),
     "free_field": (None, [_p, _u, _s]),
     "add_field_tt": (None, [_p, _u, _s, _u, _p, C.c_int]),
+    "element_gradients": (None, [_p, _s]),
+    "recover_by_volume": (None, [_p, _s]),
+    "size_from_hessian": (None, [_p, _s, _p, _d, _d]),
+    "identity_size": (None, [_p, _s]),
     "min_quality": (_d, [_p]),
     "qualities": (None, [_p, _p]),
     "measure_edges": (None, [_p, _p]),
@@ -381,6 +385,20 @@ class Mesh:
             f = d.ref_write_vtk
             f.restype, f.argtypes = None, [_p, _s]
             f(self.h, path.encode())
+
+    # ---- size-field producers
+    def element_gradients(self, name):
+        self.call("element_gradients", name.encode())
+
+    def recover_by_volume(self, name):
+        self.call("recover_by_volume", name.encode())
+
+    def size_from_hessian(self, hess_name, weights, min_h, max_h):
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        self.call("size_from_hessian", hess_name.encode(), None if w is None else _ptr(w), min_h, max_h)
+
+    def identity_size(self, name):
+        self.call("identity_size", name.encode())
 
     def metric_from_hessian(self, hessian, eps, hmin, hmax, out="metric"):
         self.call("metric_from_hessian", hessian.encode(), eps, hmin, hmax, out.encode())

@@ -179,6 +179,76 @@ TRANSFER_CASES = {
 }
 
 
+def dye_field(coords, dim):
+    """dye_fun (test_warp.cpp:52-68): a cone of height 1 and radius 1/4
+    around (1/4, 1/2), a negative one around (3/4, 1/2); 3D: balls, centred
+    at z = 1/2."""
+    zc = 0.5 if dim == 3 else 0.0
+    l = np.array([0.25, 0.5, zc])
+    r = np.array([0.75, 0.5, zc])
+    xl = coords - l
+    nl = np.sqrt(np.sum(xl * xl, axis=1))
+    xr = coords - r
+    nr = np.sqrt(np.sum(xr * xr, axis=1))
+    v = np.where(nl <= 0.25, 4 * (0.25 - nl), np.where(nr <= 0.25, -4 * (0.25 - nr), 0.0))
+    return v
+
+
+def set_size_field(m, weight, min_h, max_h):
+    """set_size_field (test_warp.cpp:70-83): gradients of the dye recovered
+    to the vertices twice, then the size from the Hessian's norm."""
+    d = m.dim
+    m.free_field(0, "adapt_size")
+    m.element_gradients("dye")
+    m.recover_by_volume("grad_dye")
+    m.free_field(d, "grad_dye")
+    m.element_gradients("grad_dye")
+    m.free_field(0, "grad_dye")
+    m.recover_by_volume("grad_grad_dye")
+    m.free_field(d, "grad_grad_dye")
+    m.size_from_hessian("grad_grad_dye", [weight], min_h, max_h)
+    m.free_field(0, "grad_grad_dye")
+
+
+def dye_warp_steps(lib, dim, n, h0, weight, min_h, max_h, cycles, warps, good=0.4, on_device=False):
+    """test_warp.cpp:99-126: refine to a uniform size, then a dye field whose
+    recovered Hessian sets the size; per warp, up to three rounds of
+    warp_to_limit, a new size field and mesh_adapt (no sliver layers)."""
+    m = make_mesh(lib, dim, n, on_device)
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), h0))
+    while m.refine_by_size(0.0):
+        pass
+    yield "refined", m
+    m.add_field(0, "dye", dye_field(m.coords().reshape(-1, 3), dim))
+    set_size_field(m, weight, min_h, max_h)
+    yield "size", m
+    m.last = m.adapt(1.0 / 3.0, good, 0, 50)
+    yield "adapt", m
+    rot = W.PI_REF / 4
+    for i in range(cycles):
+        for j in range(warps):
+            m.add_field(0, "warp", W.rotation_warp(m.coords().reshape(-1, 3), rot).ravel())
+            for it in range(3):
+                done = m.warp_to_limit(0.1)
+                yield "warp%d.%d.%d" % (i, j, it), m
+                set_size_field(m, weight, min_h, max_h)
+                m.last = m.adapt(1.0 / 3.0, good, 0, 50)
+                yield "adapt%d.%d.%d" % (i, j, it), m
+                if done:
+                    break
+            m.free_field(0, "warp")
+        rot = -rot
+
+
+SIZE_CASES = {
+    # test_warp.cpp itself: 2 x 4 warps of the unit square
+    "dye_warp2d": lambda lib, dev=False: dye_warp_steps(lib, 2, 1, 0.1, 0.075 / 100.0, 0.025, 0.1, 2, 4,
+                                                        on_device=dev),
+    "dye_warp3d": lambda lib, dev=False: dye_warp_steps(lib, 3, 2, 0.25, 0.004, 0.12, 0.25, 1, 2,
+                                                        on_device=dev),
+}
+
+
 def vtu_case(case, lib):
     """Final meshes of the VTU file tests: a freshly derived box, the last
     state of a workload above, a 3D box after one mesh_adapt, a 3D box
