@@ -439,59 +439,99 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   return out;
 }
 
+// the owning low entity of every use of an up adjacency (offsets over n
+// lows): each low writes its index over its range
+static Buf<u32> up_owner(u32 const* offs, u32 n, u32 nuses) {
+  Buf<u32> own(nuses);
+  u32* o = own.get();
+  parallel_for(n, OHX_LAMBDA(u32 i) {
+    for (u32 j = offs[i]; j < offs[i + 1]; ++j) o[j] = i;
+  });
+  return own;
+}
+
 // ----------------------------------------------------------------- stars
 // get_star_general (star.cpp:13-111): lows of the highs around each low,
-// first occurrence kept, in up-adjacency order
+// first occurrence kept, in up-adjacency order.  One wave per low entity:
+// the lows of its highs are staged in LDS and lane p keeps entry p if it is
+// not the entity itself and no earlier entry equals it; a ballot prefix
+// gives each kept entry its place.  Entities with more than STAR_CAP
+// entries (never in a simplicial mesh of sane valence) take a serial path.
+constexpr u32 STAR_CAP = 1024;
+constexpr u32 STAR_WAVES = BLOCK / 64;
+
+template <bool FILL>
+__global__ __launch_bounds__(BLOCK) void star_general_k(u32 nl, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
+                                                        u32 const* __restrict__ lh, u32 lph, u32* __restrict__ deg,
+                                                        u32 const* __restrict__ go, u32* __restrict__ ga) {
+  __shared__ u32 buf[STAR_WAVES][STAR_CAP];
+  u32 w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  u32 i = blockIdx.x * STAR_WAVES + w;
+  bool live = i < nl;
+  u32 f = live ? uo[i] : 0, e = live ? uo[i + 1] : 0;
+  u32 n = (e - f) * lph;
+  bool staged = live && n <= STAR_CAP;
+  u32* E = buf[w];
+  if (staged)
+    for (u32 p = lane; p < n; p += 64) E[p] = lh[(size_t) ua[f + p / lph] * lph + p % lph];
+  __syncthreads();
+  if (staged) {
+    u32 cnt = 0;
+    u32 base_out = FILL ? go[i] : 0;
+    u64 below = (1ull << lane) - 1ull;
+    for (u32 b = 0; b < n; b += 64) {
+      u32 p = b + lane;
+      bool keep = false;
+      u32 s = 0;
+      if (p < n) {
+        s = E[p];
+        keep = s != i;
+        for (u32 q = 0; q < p && keep; ++q)
+          if (E[q] == s) keep = false;
+      }
+      u64 m = __ballot(keep);
+      if (FILL && keep) ga[base_out + cnt + (u32) __popcll(m & below)] = s;
+      cnt += (u32) __popcll(m);
+    }
+    if (!FILL && lane == 0) deg[i] = cnt;
+  } else if (live && lane == 0) {
+    // serial: uniqueness by scanning what came before
+    u32 cnt = 0;
+    u32 base_out = FILL ? go[i] : 0;
+    for (u32 p = 0; p < n; ++p) {
+      u32 s = lh[(size_t) ua[f + p / lph] * lph + p % lph];
+      if (s == i) continue;
+      bool seen = false;
+      for (u32 q = 0; q < p && !seen; ++q) seen = lh[(size_t) ua[f + q / lph] * lph + q % lph] == s;
+      if (seen) continue;
+      if (FILL) ga[base_out + cnt] = s;
+      ++cnt;
+    }
+    if (!FILL) deg[i] = cnt;
+  }
+}
+
 static Graph star_general(Mesh& m, u32 ld, u32 hd) {
   u32 nl = count(m, ld);
   Up const& u = ask_up(m, ld, hd);
   u32 const* lh = ask_down(m, hd, ld);
   u32 lph = h_down_deg[hd][ld];
-  u32 const* uo = u.offsets.get();
-  u32 const* ua = u.adj.get();
   Buf<u32> deg(nl);
-  u32* dg = deg.get();
-  // count pass: uniqueness by scanning what came before (no buffer needed)
-  parallel_for(nl, OHX_LAMBDA(u32 i) {
-    u32 size = 0;
-    for (u32 j = uo[i]; j < uo[i + 1]; ++j)
-      for (u32 k = 0; k < lph; ++k) {
-        u32 s = lh[(size_t) ua[j] * lph + k];
-        if (s == i) continue;
-        bool seen = false;
-        for (u32 j2 = uo[i]; j2 <= j && !seen; ++j2)
-          for (u32 k2 = 0; k2 < lph; ++k2) {
-            if (j2 == j && k2 == k) break;
-            if (lh[(size_t) ua[j2] * lph + k2] == s) {
-              seen = true;
-              break;
-            }
-          }
-        if (!seen) ++size;
-      }
-    dg[i] = size;
-  });
+  u32 nb = (nl + STAR_WAVES - 1) / STAR_WAVES;
   Graph g;
   g.offsets = Buf<u32>(nl + 1);
-  u32 nadj = exscan_total(dg, nl, g.offsets.get());
+  if (nl) {
+    hipLaunchKernelGGL(star_general_k<false>, dim3(nb), dim3(BLOCK), 0, stream(), nl, u.offsets.get(), u.adj.get(), lh,
+                       lph, deg.get(), (u32 const*) nullptr, (u32*) nullptr);
+    check_launch("star_general");
+  }
+  u32 nadj = exscan_total(deg.get(), nl, g.offsets.get());
   g.adj = Buf<u32>(nadj);
-  u32* ga = g.adj.get();
-  u32 const* go = g.offsets.get();
-  parallel_for(nl, OHX_LAMBDA(u32 i) {
-    u32 base = go[i], size = 0;
-    for (u32 j = uo[i]; j < uo[i + 1]; ++j)
-      for (u32 k = 0; k < lph; ++k) {
-        u32 s = lh[(size_t) ua[j] * lph + k];
-        if (s == i) continue;
-        bool seen = false;
-        for (u32 t = 0; t < size; ++t)
-          if (ga[base + t] == s) {
-            seen = true;
-            break;
-          }
-        if (!seen) ga[base + size++] = s;
-      }
-  });
+  if (nl) {
+    hipLaunchKernelGGL(star_general_k<true>, dim3(nb), dim3(BLOCK), 0, stream(), nl, u.offsets.get(), u.adj.get(), lh,
+                       lph, (u32*) nullptr, g.offsets.get(), g.adj.get());
+    check_launch("star_general");
+  }
   return g;
 }
 
@@ -559,15 +599,23 @@ static Graph star_edge_tet(Mesh& m) {
   g.adj = Buf<u32>(nadj);
   u32* ga = g.adj.get();
   u32 const* go = g.offsets.get();
-  parallel_for(ne, OHX_LAMBDA(u32 i) {
-    u32 o = go[i];
-    for (u32 j = to[i]; j < to[i + 1]; ++j) {
-      u32 t = ta[j], d = td[j];
-      ga[o++] = et[3 * (size_t) t + (d + 1) % 3];
-      ga[o++] = et[3 * (size_t) t + (d + 2) % 3];
-    }
-    u32 const* oe = opp(3, 1);
-    for (u32 j = ro[i]; j < ro[i + 1]; ++j) ga[o++] = er[6 * (size_t) ra[j] + oe[rd[j]]];
+  // one thread per use: the owning edge's offsets come from the up lists'
+  // own offsets (edge i's triangle uses fill go[i] + 2 (j - to[i]), its tet
+  // uses follow them), so every read and write is coalesced
+  u32 ntu = ut.adj.size(), nru = ur.adj.size();
+  Buf<u32> owner_t = up_owner(to, ne, ntu), owner_r = up_owner(ro, ne, nru);
+  u32 const* ot = owner_t.get();
+  u32 const* orr = owner_r.get();
+  parallel_for(ntu, OHX_LAMBDA(u32 j) {
+    u32 i = ot[j], t = ta[j], d = td[j];
+    u32 o = go[i] + 2 * (j - to[i]);
+    ga[o] = et[3 * (size_t) t + (d + 1) % 3];
+    ga[o + 1] = et[3 * (size_t) t + (d + 2) % 3];
+  });
+  parallel_for(nru, OHX_LAMBDA(u32 j) {
+    u32 i = orr[j];
+    u32 o = go[i] + 2 * (to[i + 1] - to[i]) + (j - ro[i]);
+    ga[o] = er[6 * (size_t) ra[j] + opp(3, 1)[rd[j]]];
   });
   return g;
 }
@@ -582,7 +630,11 @@ static Graph get_star(Mesh& m, u32 ld, u32 hd) {
 }
 
 // ------------------------------------------------------------------ dual
-// element_dual_from_verts (dual.cpp:40-98)
+// element_dual_from_verts (dual.cpp:40-98): across side j of element i,
+// the element of the side's first vertex (other than i, first in up order)
+// that also contains every other side vertex.  The up lists are sorted by
+// element, so this is an intersection of sorted lists, walked in the first
+// vertex's order.
 static Buf<u32> dual_from_verts(Mesh& m) {
   u32 dim = m.dim, sd = dim - 1;
   u32 ne = count(m, dim);
@@ -597,25 +649,27 @@ static Buf<u32> dual_from_verts(Mesh& m) {
   parallel_for(ne, OHX_LAMBDA(u32 i) {
     for (u32 j = 0; j < spe; ++j) {
       u32 const* sc = canon(dim, sd, 0, j);
-      // elements of the first side vertex other than i that also contain
-      // every other side vertex
-      u32 v0 = ev[(size_t) i * vpe + sc[0]];
+      u32 pos[3], end[3];
+      for (u32 k = 0; k < vps; ++k) {
+        u32 v = ev[(size_t) i * vpe + sc[k]];
+        pos[k] = uo[v];
+        end[k] = uo[v + 1];
+      }
       u32 found = INVALID, nfound = 0;
-      for (u32 a = uo[v0]; a < uo[v0 + 1]; ++a) {
+      bool done = false;
+      for (u32 a = pos[0]; a < end[0] && !done; ++a) {
         u32 c = ua[a];
-        if (c == i) continue;
         bool all = true;
-        for (u32 k = 1; k < vps && all; ++k) {
-          u32 v = ev[(size_t) i * vpe + sc[k]];
-          bool has = false;
-          for (u32 b = uo[v]; b < uo[v + 1]; ++b)
-            if (ua[b] == c) {
-              has = true;
-              break;
-            }
-          all = has;
+        for (u32 k = 1; k < vps; ++k) {
+          while (pos[k] < end[k] && ua[pos[k]] < c) ++pos[k];
+          if (pos[k] == end[k]) {
+            done = true;
+            all = false;
+            break;
+          }
+          if (ua[pos[k]] != c) all = false;
         }
-        if (all) {
+        if (all && c != i) {
           if (!nfound) found = c;
           ++nfound;
         }

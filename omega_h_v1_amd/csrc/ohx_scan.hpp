@@ -4,7 +4,9 @@
 // where f is a __device__ functor; optionally the total goes to a host slot.
 // Small inputs (one tile) use one block; larger ones a decoupled look-back
 // scan (fetch-add tile tickets, 64-bit (epoch, flag, value) tile words, a
-// wave-wide look-back, bounded spin).
+// wave-wide look-back, bounded spin).  Tiles hold 256 x I items; I (4 or
+// 16) is chosen per call site and size class by the online tuner: small
+// scans want more, smaller tiles to reach every CU, large ones fewer.
 #pragma once
 
 #include "ohx_common.hpp"
@@ -12,8 +14,6 @@
 namespace ohx {
 
 constexpr u32 SCAN_T = 256;
-constexpr u32 SCAN_I = 16;
-constexpr u32 SCAN_TILE = SCAN_T * SCAN_I;  // 4096 items per block
 
 // tile words and ticket shared by every scan of the process
 struct ScanTiles {
@@ -58,11 +58,12 @@ __device__ inline u64 pack(u32 epoch, u32 inclusive, u32 v) {
 
 __device__ inline u32 pad(u32 i) { return i + (i >> 5); }
 
-template <class F>
+template <u32 SCAN_I, class F>
 __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles, u32* __restrict__ out,
                                                          u64* __restrict__ tiles, u64* __restrict__ ticket,
                                                          u32 epoch, u32* __restrict__ host_total,
                                                          u32* __restrict__ err) {
+  constexpr u32 SCAN_TILE = SCAN_T * SCAN_I;
   __shared__ u32 tile[SCAN_TILE + SCAN_TILE / 32];
   __shared__ u32 lds_w[SCAN_T / 64];
   __shared__ u32 s_tile, s_prefix;
@@ -167,15 +168,34 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
 
 }  // namespace scan_detail
 
-template <class F>
-void exscan_fn(u32 n, F f, u32* out, u32* host_total_dev = nullptr) {
-  u32 nt = n ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+template <u32 I, class F>
+void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hipEvent_t eb) {
+  constexpr u32 tile = SCAN_T * I;
+  u32 nt = n ? (n + tile - 1) / tile : 1;
   ScanTiles st = scan_tiles_for(nt);
+  if (ea) {
+    hipExtLaunchKernelGGL(scan_detail::scan_fn_kernel<I, F>, dim3(nt), dim3(SCAN_T), 0, stream(), ea, eb, 0, n, f, nt,
+                          out, st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
+    check_launch("scan");
+    return;
+  }
   // compulsory bytes: one 4-byte input per item (a lower bound for functors
   // that read more) and the n + 1 offsets out
   double prof_bytes = (double) n * 4 + (double) (n + 1) * 4;
-  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
-                     st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
+  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<I, F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
+               st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
+}
+
+template <class F>
+void exscan_fn(u32 n, F f, u32* out, u32* host_total_dev = nullptr) {
+  // one tuner per call site (functor type) and host thread
+  static thread_local Tuner tuner("scan", {4, 16});
+  tuned(tuner, n, [&](u32 items, hipEvent_t ea, hipEvent_t eb) {
+    if (items == 4)
+      exscan_launch<4>(n, f, out, host_total_dev, ea, eb);
+    else
+      exscan_launch<16>(n, f, out, host_total_dev, ea, eb);
+  });
 }
 
 // as exscan_fn and returns the total (one synchronisation)

@@ -1,0 +1,68 @@
+"""Adjacency derivations on the GPU against the oracle: every star and the
+element dual, on reduced meshes (where the general star and the dual from
+vertices run) and full ones, including adapted meshes of uneven valence."""
+import numpy as np
+import pytest
+
+import meshlib as ML
+import parity_cases as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def product():
+    lib = ML.load("product")
+    if lib is None:
+        pytest.fail("product library not built: run __graft_entry__.build() first")
+    return lib
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    lib = ML.load("oracle")
+    if lib is None:
+        pytest.fail("oracle library not built: make -C oracle")
+    return lib
+
+
+def _adapted(oracle, case):
+    m = None
+    for _, m in dict(P.CASES, **P.SIZE_CASES)[case](oracle):
+        pass
+    return m.dim, m.coords().reshape(-1, 3).copy(), m.down(m.dim, 0).reshape(-1, m.dim + 1).copy()
+
+
+def _meshes(oracle):
+    out = []
+    for dim, n in ((2, 7), (3, 4)):
+        x, c = ML.box(dim, n)
+        out.append(("box%dd" % dim, dim, x, c))
+    for case in ("warp3d_rot_n4", "dye_warp2d"):
+        dim, x, c = _adapted(oracle, case)
+        out.append((case, dim, x, c))
+    return out
+
+
+def _compare(a, b, what):
+    assert a[0].shape == b[0].shape and np.array_equal(a[0], b[0]), what + " offsets"
+    assert np.array_equal(a[1], b[1]), what + " adjacency"
+
+
+def test_reduced_mesh_stars_and_dual(product, oracle):
+    for name, dim, x, c in _meshes(oracle):
+        mp, mo = ML.Mesh(product, dim, x, c), ML.Mesh(oracle, dim, x, c)
+        # the general star (no edges derived yet) and the dual from vertices
+        _compare(mp.star(0, dim), mo.star(0, dim), "%s star(0,%d)" % (name, dim))
+        assert np.array_equal(mp.dual(), mo.dual()), name + " dual"
+
+
+def test_full_mesh_stars_and_dual(product, oracle):
+    for name, dim, x, c in _meshes(oracle):
+        mp, mo = ML.Mesh(product, dim, x, c), ML.Mesh(oracle, dim, x, c)
+        mp.derive_model_full()
+        mo.derive_model_full()
+        for l in range(dim):
+            for h in range(l + 1, dim + 1):
+                _compare(mp.star(l, h), mo.star(l, h), "%s star(%d,%d)" % (name, l, h))
+        assert np.array_equal(mp.dual(), mo.dual()), name + " dual"
