@@ -69,30 +69,34 @@ __global__ __launch_bounds__(BLOCK) void indset_round(u32 n, u32 const* __restri
                                                       u32 const* __restrict__ old, u32* __restrict__ state,
                                                       u32* __restrict__ flag) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  u32 s = old[i];
-  if (s != UNKNOWN) {
+  bool undecided = false;
+  if (i < n) {
+    u32 s = old[i];
+    if (s == UNKNOWN) {
+      s = IN_SET;
+      u32 f = offs[i], e = offs[i + 1];
+      for (u32 j = f; j < e; ++j)
+        if (old[adj[j]] == IN_SET) {
+          s = NOT_IN_SET;
+          break;
+        }
+      if (s == IN_SET) {
+        double myg = good[i];
+        for (u32 j = f; j < e; ++j) {
+          u32 o = adj[j];
+          if (old[o] == NOT_IN_SET) continue;
+          double og = good[o];
+          if ((myg == og && o < i) || myg < og) {
+            s = UNKNOWN;
+            undecided = true;
+            break;
+          }
+        }
+      }
+    }
     state[i] = s;
-    return;
   }
-  u32 f = offs[i], e = offs[i + 1];
-  for (u32 j = f; j < e; ++j)
-    if (old[adj[j]] == IN_SET) {
-      state[i] = NOT_IN_SET;
-      return;
-    }
-  double myg = good[i];
-  for (u32 j = f; j < e; ++j) {
-    u32 o = adj[j];
-    if (old[o] == NOT_IN_SET) continue;
-    double og = good[o];
-    if ((myg == og && o < i) || myg < og) {
-      state[i] = UNKNOWN;
-      *flag = 1u;
-      return;
-    }
-  }
-  state[i] = IN_SET;
+  wave_flag(flag, undecided);
 }
 
 // rounds == 0: rounds in batches with a host check after each, up to the
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     cand[i] = 0;
   else {
     out[i] = minq;
-    words[W_ANY] = 1u;
+    wave_flag(words + W_ANY, true);
   }
 }
 
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void refine_candidate_f64(u32 n, u32 const* 
   if (i >= n) return;
   u32 c = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt) > 1.0 ? 1u : 0u;
   cand[i] = c;
-  if (c) words[W_ANY] = 1u;
+  wave_flag(words + W_ANY, c != 0);
 }
 
 static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool require_better,
@@ -854,7 +858,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
   }
   if (lane == 0) {
     codes[i] = c;
-    if (c != DONT_COLLAPSE) words[W_ANY] = 1u;
+    wave_flag(words + W_ANY, c != DONT_COLLAPSE);
   }
 }
 
@@ -869,7 +873,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const*
   double sz = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
   u32 c = (sz < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
   codes[i] = c;
-  if (c) words[W_ANY] = 1u;
+  wave_flag(words + W_ANY, c != 0);
 }
 
 // tags_subset (subset.cpp:15-43): every tag of the surviving vertices,
@@ -1507,7 +1511,7 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
   if (sq > old_minq) {
     out[i] = sq;
     rsz[i] = rs;
-    words[W_ANY] = 1u;
+    wave_flag(words + W_ANY, true);
   } else {
     cand[i] = 0;
   }

@@ -74,8 +74,6 @@ struct StreamScope {
   }
 };
 void sync();
-// after a failed scan: return the scan's tile ticket to zero (stream drained)
-void scan_recover();
 
 // Host-visible slots (8 bytes each) in mapped pinned memory: kernels write
 // the few scalars the host needs (scan totals, reductions, error codes)
@@ -221,6 +219,14 @@ void parallel_for(u32 n, F f) {
   check_launch("parallel_for");
 }
 #define OHX_LAMBDA [=] __device__
+
+// *w = 1 once per wave if any active lane has pred: stores of many lanes to
+// one word serialise in L2 and hold up the kernel's completion
+__device__ inline void wave_flag(u32* w, bool pred) {
+  unsigned long long m = __ballot(pred);
+  unsigned long long act = __ballot(1);
+  if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) *w = 1u;
+}
 
 // ---------------------------------------------------------------- timing
 // Per-kernel-name timers, enabled by ohx_profile_enable().  A timed launch

@@ -77,11 +77,11 @@ T reduce(T const* a, u32 n, u32 begin, T init) {
 }  // namespace
 
 namespace {
-// per thread (the thread's stream): tile words never cleared, a ticket
+// per thread (the thread's stream): tile words, never cleared between scans
+// (each scan tags its words with a new epoch)
 struct ScanState {
   u64* tiles = nullptr;
   u32 cap = 0;
-  u64* ticket = nullptr;
   u32 epoch = 0;
 };
 ScanState& scan_state() {
@@ -92,10 +92,6 @@ ScanState& scan_state() {
 
 ScanTiles scan_tiles_for(u32 nt) {
   ScanState& st = scan_state();
-  if (!st.ticket) {
-    st.ticket = static_cast<u64*>(dev_alloc(sizeof(u64)));
-    OHX_CHECK(hipMemsetAsync(st.ticket, 0, sizeof(u64), stream()));
-  }
   if (nt > st.cap) {
     // a larger tile array; the old one is released once the stream drains
     if (st.tiles) {
@@ -108,12 +104,7 @@ ScanTiles scan_tiles_for(u32 nt) {
   }
   // epochs stay below 2^30 - 1, so fresh all-ones words never match one
   st.epoch = st.epoch + 1 >= 0x3fffffffu ? 1u : st.epoch + 1;
-  return ScanTiles{st.tiles, st.ticket, st.epoch};
-}
-
-void scan_recover() {
-  ScanState& st = scan_state();
-  if (st.ticket) OHX_CHECK(hipMemset(st.ticket, 0, sizeof(u64)));
+  return ScanTiles{st.tiles, st.epoch};
 }
 
 void exscan(u32 const* a, u32 n, u32* out) {
@@ -176,8 +167,11 @@ Buf<double> copy_f64(double const* a, u32 n) {
 }
 
 Buf<u32> negate_offsets(u32 const* offs, u32 n) {
+  // offsets of 0/1 flags: the items before i without a product number
+  // i - offs[i], so no scan is needed
   Buf<u32> o(n + 1);
-  exscan_fn(n, OHX_LAMBDA(u32 i) { return (offs[i + 1] == offs[i]) ? 1u : 0u; }, o.get());
+  u32* op = o.get();
+  parallel_for(n + 1, OHX_LAMBDA(u32 i) { op[i] = i - offs[i]; });
   return o;
 }
 

@@ -26,7 +26,8 @@ void fill(u32* a, u32 n, u32 v);
 Buf<u32> iota(u32 n);
 Buf<u32> copy(u32 const* a, u32 n);
 Buf<double> copy_f64(double const* a, u32 n);
-// uints_negate_offsets (ints.cpp:224-232)
+// uints_negate_offsets (ints.cpp:224-232) for offsets whose steps are 0 or 1
+// (the exclusive scan of 0/1 flags): out[i] = i - offs[i]
 Buf<u32> negate_offsets(u32 const* offs, u32 n);
 // uints_scale
 Buf<u32> scale(u32 const* a, u32 n, u32 s);

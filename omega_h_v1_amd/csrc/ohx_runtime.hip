@@ -84,7 +84,6 @@ void sync() {
   if (*e) {
     unsigned code = *e;
     *e = 0;
-    if (code == ERR_SCAN_SPIN) scan_recover();
     fail(dev_error_text(code), __FILE__, __LINE__);
   }
 }

@@ -3,7 +3,7 @@
 // exscan_fn(n, f, out, host_total): out[0] = 0, out[i+1] = f(0) + ... + f(i),
 // where f is a __device__ functor; optionally the total goes to a host slot.
 // Small inputs (one tile) use one block; larger ones a decoupled look-back
-// scan (fetch-add tile tickets, 64-bit (epoch, flag, value) tile words, a
+// scan (tiles in block order, 64-bit (epoch, flag, value) tile words, a
 // wave-wide look-back, bounded spin).  Tiles hold 256 x I items; I (4 or
 // 16) is chosen per call site and size class by the online tuner: small
 // scans want more, smaller tiles to reach every CU, large ones fewer.
@@ -15,10 +15,9 @@ namespace ohx {
 
 constexpr u32 SCAN_T = 256;
 
-// tile words and ticket shared by every scan of the process
+// tile words shared by every scan of a host thread (its stream)
 struct ScanTiles {
   u64* tiles;
-  u64* ticket;
   u32 epoch;
 };
 ScanTiles scan_tiles_for(u32 ntiles);
@@ -60,30 +59,21 @@ __device__ inline u32 pad(u32 i) { return i + (i >> 5); }
 
 template <u32 SCAN_I, class F>
 __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles, u32* __restrict__ out,
-                                                         u64* __restrict__ tiles, u64* __restrict__ ticket,
+                                                         u64* __restrict__ tiles,
                                                          u32 epoch, u32* __restrict__ host_total,
                                                          u32* __restrict__ err) {
   constexpr u32 SCAN_TILE = SCAN_T * SCAN_I;
   __shared__ u32 tile[SCAN_TILE + SCAN_TILE / 32];
   __shared__ u32 lds_w[SCAN_T / 64];
-  __shared__ u32 s_tile, s_prefix;
-  if (threadIdx.x == 0) {
-    u32 t = 0;
-    if (ntiles > 1) {
-      // tile ticket in dispatch order (a block only ever waits on tiles that
-      // earlier-started blocks own); the holder of the last ticket returns
-      // the counter to zero for the next scan on the stream
-      t = (u32) __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (t == ntiles - 1) __hip_atomic_store(ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    s_tile = t;
-  }
-  __syncthreads();
-  u32 t = s_tile;
-  if (t >= ntiles) {  // impossible with a consistent ticket; never spin on it
-    if (threadIdx.x == 0) *err = ERR_SCAN_SPIN;
-    return;
-  }
+  __shared__ u32 s_prefix;
+  // tile = block index.  Workgroups are dispatched in index order within
+  // each XCD, so the lowest-numbered resident block always finds every
+  // earlier tile finished or on an XCD with free slots: the look-back
+  // cannot wait on a block that is never scheduled.  (A fetch-add ticket
+  // would give the same guarantee without that argument, but same-address
+  // atomics retire one per ~12 ns on this part, which serialises large
+  // scans.)  The spin stays bounded regardless.
+  u32 t = blockIdx.x;
   u32 base = t * SCAN_TILE;
 #pragma unroll
   for (u32 k = 0; k < SCAN_I; ++k) {
@@ -102,9 +92,13 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
   u32 agg;
   u32 run = block_excl_scan(s, lds_w, &agg);
   if (threadIdx.x < 64) {
-    // wave 0 publishes this tile and looks back over up to 64 predecessor
-    // tiles per step: lane k reads tile j - k; the closest inclusive prefix
-    // ends the walk once every tile before it in the window is published
+    // wave 0 publishes this tile and looks back over up to 64 x LB
+    // predecessor tiles per step (lane l reads tiles j - LB l .. j - LB l -
+    // LB + 1): the closest inclusive prefix ends the walk once every tile
+    // before it in the window is published.  LB = 1: wider windows (LB = 4)
+    // measured slower on large scans, the extra polling loads competing with
+    // the tiles' own traffic in L2.
+    constexpr u32 LB = 1;
     u32 lane = threadIdx.x;
     u32 prefix = 0;
     if (ntiles > 1) {
@@ -115,17 +109,31 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
         int j = (int) t - 1;
         u32 spins = 0;
         while (true) {
-          int idx = j - (int) lane;
-          u64 v = idx >= 0 ? __hip_atomic_load(&tiles[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                           : pack(epoch, 1, 0);
-          u32 hi = (u32) (v >> 32);
-          bool ready = (hi >> 1) == epoch;
-          bool incl = ready && (hi & 1u);
-          u64 pmask = __ballot(incl);
-          u64 nrmask = __ballot(!ready);
+          u64 v[LB];
+#pragma unroll
+          for (u32 q = 0; q < LB; ++q) {
+            int idx = j - (int) (LB * lane + q);
+            v[q] = idx >= 0 ? __hip_atomic_load(&tiles[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : pack(epoch, 1, 0);
+          }
+          // per lane: the first inclusive entry (LB if none) and whether an
+          // entry before it (or any entry, if none) is still unpublished
+          u32 qi = LB;
+          bool waiting = false;
+#pragma unroll
+          for (u32 q = 0; q < LB; ++q) {
+            u32 hi = (u32) (v[q] >> 32);
+            bool ready = (hi >> 1) == epoch;
+            if (qi == LB) {
+              if (!ready) waiting = true;
+              else if (hi & 1u) qi = q;
+            }
+          }
+          u64 pmask = __ballot(qi < LB);
           u32 first = pmask ? (u32) (__ffsll((long long) pmask) - 1) : 64u;
-          u64 need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
-          if (nrmask & need) {
+          // lanes up to the first with an inclusive entry must be complete
+          bool need = lane <= first;
+          if (__ballot(need && waiting)) {
             if (++spins > (1u << 22)) {
               if (lane == 0) *err = ERR_SCAN_SPIN;
               break;
@@ -133,12 +141,18 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
             __builtin_amdgcn_s_sleep(1);
             continue;
           }
-          u32 val = lane <= first ? (u32) v : 0u;
+          u32 val = 0;
+          if (need) {
+            u32 upto = lane < first ? LB : qi + 1;
+#pragma unroll
+            for (u32 q = 0; q < LB; ++q)
+              if (q < upto) val += (u32) v[q];
+          }
 #pragma unroll
           for (u32 d = 32; d >= 1; d >>= 1) val += __shfl_xor(val, d, 64);
           prefix += val;
           if (first < 64) break;
-          j -= 64;
+          j -= 64 * (int) LB;
         }
         if (lane == 0)
           __hip_atomic_store(&tiles[t], pack(epoch, 1, prefix + agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -175,7 +189,7 @@ void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hip
   ScanTiles st = scan_tiles_for(nt);
   if (ea) {
     hipExtLaunchKernelGGL(scan_detail::scan_fn_kernel<I, F>, dim3(nt), dim3(SCAN_T), 0, stream(), ea, eb, 0, n, f, nt,
-                          out, st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
+                          out, st.tiles, st.epoch, host_total_dev, dev_error_word());
     check_launch("scan");
     return;
   }
@@ -183,7 +197,7 @@ void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hip
   // that read more) and the n + 1 offsets out
   double prof_bytes = (double) n * 4 + (double) (n + 1) * 4;
   launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<I, F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
-               st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
+               st.tiles, st.epoch, host_total_dev, dev_error_word());
 }
 
 template <class F>
