@@ -26,32 +26,49 @@ static double now_s() {
 
 // scratch word slots (zeroed by a fill kernel before each use)
 enum { W_ANY = 0, W_FLAGS = 8, W_STATS = 40 };
-// FP32-mixed statistics: u64 at words 40-41 (decisions settled in FP32),
-// u64 at 42-43 (FP64 re-evaluations), float bits at 44 (max |q32 - q64|)
+// FP32-mixed statistics: two slotted u64 counters (decisions settled in
+// FP32, FP64 re-evaluations) in a buffer of their own, float bits of the
+// max |q32 - q64| at scratch word 40
 struct ScreenStats {
   unsigned long long* screened;
   unsigned long long* reeval;
   u32* max_err;
 };
+constexpr size_t STAT_WORDS = (size_t) STAT_SLOTS * STAT_STRIDE;
+static unsigned long long* stat_counters() {
+  // one buffer per thread, for the life of the thread, zeroed once
+  static thread_local unsigned long long* c = nullptr;
+  if (!c) {
+    c = static_cast<unsigned long long*>(dev_alloc(2 * STAT_WORDS * sizeof(unsigned long long)));
+    OHX_CHECK(hipMemsetAsync(c, 0, 2 * STAT_WORDS * sizeof(unsigned long long), stream()));
+  }
+  return c;
+}
 static ScreenStats screen_stats() {
-  u32* w = scratch_words() + W_STATS;
-  return ScreenStats{reinterpret_cast<unsigned long long*>(w), reinterpret_cast<unsigned long long*>(w + 2), w + 4};
+  unsigned long long* c = stat_counters();
+  return ScreenStats{c, c + STAT_WORDS, scratch_words() + W_STATS};
 }
 
 void mixed_stats(double* max_err, unsigned long long* screened, unsigned long long* reeval) {
-  u32 h[5];
-  to_host(h, scratch_words() + W_STATS, 5);
-  unsigned long long a, b;
-  std::memcpy(&a, h, 8);
-  std::memcpy(&b, h + 2, 8);
+  std::vector<unsigned long long> c(2 * STAT_WORDS);
+  to_host(c.data(), stat_counters(), c.size());
+  unsigned long long a = 0, b = 0;
+  for (u32 k = 0; k < STAT_SLOTS; ++k) {
+    a += c[(size_t) k * STAT_STRIDE];
+    b += c[STAT_WORDS + (size_t) k * STAT_STRIDE];
+  }
+  u32 h = read_at(scratch_words(), W_STATS);
   float e;
-  std::memcpy(&e, h + 4, 4);
+  std::memcpy(&e, &h, 4);
   if (max_err) *max_err = e;
   if (screened) *screened = a;
   if (reeval) *reeval = b;
 }
 
-void mixed_stats_reset() { fill(scratch_words() + W_STATS, 5, 0); }
+void mixed_stats_reset() {
+  OHX_CHECK(hipMemsetAsync(stat_counters(), 0, 2 * STAT_WORDS * sizeof(unsigned long long), stream()));
+  fill(scratch_words() + W_STATS, 1, 0);
+}
 
 static u32 read_word(u32 k) { return read_at(scratch_words(), k); }
 static void zero_words(u32 first, u32 n) { fill(scratch_words() + first, n, 0); }

@@ -88,7 +88,8 @@ void batch_note_changed(Mesh& m, u32 d, u32 const* indset) {
   CompOf co = comp_of(m, b, d);
   u32* ch = b.changed.get();
   parallel_for(count(m, d), OHX_LAMBDA(u32 i) {
-    if (indset[i]) ch[co(i)] = 1u;  // every writer stores the same value
+    // neighbours share a component: one store per component and wave
+    wave_flag_at(ch + co(i), indset[i] != 0);
   });
 }
 

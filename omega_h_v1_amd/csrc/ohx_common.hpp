@@ -227,6 +227,17 @@ __device__ inline void wave_flag(u32* w, bool pred) {
   unsigned long long act = __ballot(1);
   if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) *w = 1u;
 }
+// *w = 1 for the lanes with pred, one store per distinct address in the wave
+__device__ inline void wave_flag_at(u32* w, bool pred) {
+  unsigned long long todo = __ballot(pred);
+  while (todo) {
+    int leader = __ffsll((long long) todo) - 1;
+    unsigned long long addr = (unsigned long long) __shfl((long long) (size_t) w, leader);
+    bool same = pred && (unsigned long long) (size_t) w == addr;
+    if ((int) __lane_id() == leader) *w = 1u;
+    todo &= ~__ballot(same);
+  }
+}
 
 // ---------------------------------------------------------------- timing
 // Per-kernel-name timers, enabled by ohx_profile_enable().  A timed launch

@@ -352,11 +352,17 @@ __device__ inline float quality32(V3 const* p, Sym const* pm, bool signed_z) {
 
 // screening statistics in device words: decisions settled in FP32, FP64
 // re-evaluations, and the largest |q32 - q64| among re-evaluated values
-// (float bits, so unsigned order is float order)
+// (float bits, so unsigned order is float order).  The counters are spread
+// over STAT_SLOTS words 128 bytes apart, by block: atomics on one address
+// retire one at a time in L2, so one shared counter would serialise every
+// wave of the launch.
+constexpr u32 STAT_SLOTS = 64;
+constexpr u32 STAT_STRIDE = 16;  // u64 words between slots (128 bytes)
 __device__ inline void wave_count(unsigned long long* w, bool pred) {
   unsigned long long m = __ballot(pred);
   unsigned long long act = __ballot(1);
-  if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) atomicAdd(w, (unsigned long long) __popcll(m));
+  if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1))
+    atomicAdd(w + (size_t) (blockIdx.x % STAT_SLOTS) * STAT_STRIDE, (unsigned long long) __popcll(m));
 }
 __device__ inline void note_max_err(u32* w, float err) {
   // err >= 0: the atomic is skipped once a larger value is recorded
