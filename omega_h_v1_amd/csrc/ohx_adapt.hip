@@ -71,7 +71,12 @@ void mixed_stats_reset() {
 }
 
 static u32 read_word(u32 k) { return read_at(scratch_words(), k); }
-static void zero_words(u32 first, u32 n) { fill(scratch_words() + first, n, 0); }
+// a new value for the tagged "any" word W_ANY (never 0, the cleared value)
+static u32 next_any_tag() {
+  static thread_local u32 t = 0;
+  t = t + 1 ? t + 1 : 1;
+  return t;
+}
 
 // ------------------------------------------------------- independent set
 // One Jacobi round of the deterministic Luby variant (indset.cpp:55-84):
@@ -125,14 +130,20 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
   u32 n = count(m, ent_dim);
   Graph const& g = ask_star(m, ent_dim, m.dim);
   Buf<u32> a(n), b(n);
+  constexpr u32 BATCH = 8;
+  u32* flags = scratch_words() + W_FLAGS;
   {
+    // the states, and the first batch's round flags cleared on the way
     u32* s = a.get();
-    parallel_for(n, OHX_LAMBDA(u32 i) { s[i] = filter[i] ? UNKNOWN : NOT_IN_SET; });
+    parallel_for(n, OHX_LAMBDA(u32 i) {
+      s[i] = filter[i] ? UNKNOWN : NOT_IN_SET;
+      if (i < BATCH) flags[i] = 0;
+    });
   }
   if (!n) {
     if (rounds) {
-      fill(scratch_words() + W_FLAGS, 1, 0);
-      *last_flag = scratch_words() + W_FLAGS;
+      fill(flags, 1, 0);
+      *last_flag = flags;
     }
     return a;
   }
@@ -143,12 +154,10 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
   // rounds are launched in batches; a round after convergence copies the
   // state unchanged, so running past the converged round is harmless
   // rounds per host check: most sets settle within 8 rounds
-  constexpr u32 BATCH = 8;
   u32 const MAX_ROUNDS = g_indset_max_rounds;
-  u32* flags = scratch_words() + W_FLAGS;
   if (rounds) {
     u32 nr = rounds < BATCH ? rounds : BATCH;
-    fill(flags, nr, 0);
+    if (n < nr) fill(flags, nr, 0);  // the set-up kernel cleared flags [0, n)
     for (u32 r = 0; r < nr; ++r) {
       launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
                    g.adj.get(), good, a.get(), b.get(), flags + r);
@@ -160,7 +169,7 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
   for (u32 done = 0; done < MAX_ROUNDS; done += BATCH) {
     // the last batch stops at the limit, as the reference's loop does
     u32 nr = MAX_ROUNDS - done < BATCH ? MAX_ROUNDS - done : BATCH;
-    fill(flags, nr, 0);
+    if (done || n < nr) fill(flags, nr, 0);
     for (u32 r = 0; r < nr; ++r) {
       launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
                          g.adj.get(), good, a.get(), b.get(), flags + r);
@@ -198,7 +207,7 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     u32 ns, u32 const* __restrict__ vs, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ ve, double const* __restrict__ x,
     double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor_all,
-    double const* __restrict__ qfe, u32* __restrict__ cand, double* __restrict__ out, u32* __restrict__ words,
+    double const* __restrict__ qfe, u32* __restrict__ cand, double* __restrict__ out, u32* __restrict__ words, u32 tag,
     u32* __restrict__ err, ScreenStats st) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ns || !cand[i]) return;
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(BLOCK) void refine_quality_f64(
     cand[i] = 0;
   else {
     out[i] = minq;
-    wave_flag(words + W_ANY, true);
+    wave_flag_value(words + W_ANY, true, tag);
   }
 }
 
@@ -305,12 +314,13 @@ __global__ __launch_bounds__(BLOCK) void refine_candidate_f64(u32 n, u32 const* 
                                                               double const* __restrict__ x,
                                                               double const* __restrict__ h,
                                                               double const* __restrict__ mt,
-                                                              u32* __restrict__ cand, u32* __restrict__ words) {
+                                                              u32* __restrict__ cand, u32* __restrict__ words,
+                                                              u32 tag) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   u32 c = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt) > 1.0 ? 1u : 0u;
   cand[i] = c;
-  wave_flag(words + W_ANY, c != 0);
+  wave_flag_value(words + W_ANY, c != 0, tag);
 }
 
 static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool require_better,
@@ -323,7 +333,7 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
   Buf<double> eq;
   if (require_better) eq = mesh_qualities(m);
   Buf<double> out(ns);
-  zero_words(0, 1);
+  u32 tag = next_any_tag();
   double prof_bytes = (double) ns * 20 + (double) u.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 24;
   ScreenStats sst = screen_stats();
   // in the mixed mode the FP32 screen is a variant chosen online (it pays
@@ -333,12 +343,12 @@ static Buf<double> refine_qualities(Mesh& m, u32* cand, double qual_floor, bool 
     if (ea) {
       hipExtLaunchKernelGGL(kernel, dim3(nblocks(ns)), dim3(BLOCK), 0, stream(), ea, eb, 0, ns, vs, u.offsets.get(),
                             u.adj.get(), u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, qfe,
-                            cand, out.get(), scratch_words(), dev_error_word(), sst);
+                            cand, out.get(), scratch_words(), tag, dev_error_word(), sst);
       check_launch(kn);
     } else {
       launch_timed(kn, prof_bytes, kernel, dim3(nblocks(ns)), dim3(BLOCK), ns, vs, u.offsets.get(), u.adj.get(),
                    u.dirs.get(), ve, coords(m), metric_of(m), eq.get(), qual_floor, qfe, cand, out.get(),
-                   scratch_words(), dev_error_word(), sst);
+                   scratch_words(), tag, dev_error_word(), sst);
     }
   };
   auto run = [&](u32 screen, hipEvent_t ea, hipEvent_t eb) {
@@ -671,13 +681,13 @@ u32 refine_by_size(Mesh& m, double qual_floor) {
   double const* h = size_of(m);
   double const* mt = metric_of(m);
   Buf<u32> cand(ne);
-  zero_words(0, 1);
+  u32 tag = next_any_tag();
   if (ne) {
     double prof_bytes = (double) ne * 12 + (double) count(m, 0) * 32;
     launch_timed("refine_candidate_f64", prof_bytes, refine_candidate_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
-                       ask_down(m, 1, 0), coords(m), h, mt, cand.get(), scratch_words());
+                       ask_down(m, 1, 0), coords(m), h, mt, cand.get(), scratch_words(), tag);
   }
-  if (!read_word(W_ANY)) return 0;
+  if (read_word(W_ANY) != tag) return 0;
   if (g_batch) {
     batch_mask(m, 1, cand.get());
     Buf<double> qfe = batch_floors(m, 1);
@@ -774,7 +784,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
     u32 ne, u32 const* __restrict__ ev, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
     u32 const* __restrict__ ud, u32 const* __restrict__ vel, double const* __restrict__ x,
     double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor_all,
-    double const* __restrict__ qfe, u32* __restrict__ codes, double* __restrict__ out, u32* __restrict__ words,
+    double const* __restrict__ qfe, u32* __restrict__ codes, double* __restrict__ out, u32* __restrict__ words, u32 tag,
     ScreenStats st) {
   size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
   size_t i = gid / G;
@@ -875,7 +885,7 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
   }
   if (lane == 0) {
     codes[i] = c;
-    wave_flag(words + W_ANY, c != DONT_COLLAPSE);
+    wave_flag_value(words + W_ANY, c != DONT_COLLAPSE, tag);
   }
 }
 
@@ -884,13 +894,14 @@ __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const*
                                                                double const* __restrict__ x,
                                                                double const* __restrict__ h,
                                                                double const* __restrict__ mt, double floor_,
-                                                               u32* __restrict__ codes, u32* __restrict__ words) {
+                                                               u32* __restrict__ codes, u32* __restrict__ words,
+                                                               u32 tag) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= n) return;
   double sz = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
   u32 c = (sz < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
   codes[i] = c;
-  wave_flag(words + W_ANY, c != 0);
+  wave_flag_value(words + W_ANY, c != 0, tag);
 }
 
 // tags_subset (subset.cpp:15-43): every tag of the surviving vertices,
@@ -970,7 +981,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   Up const& ue = ask_up(m, 0, dim);
   if (require_better) eq = element_qualities(dim, count(m, dim), vel, coords(m), count(m, 0), metric_of(m));
   Buf<double> cq((size_t) 2 * ne);
-  zero_words(0, 1);
+  u32 tag = next_any_tag();
   {
     double prof_bytes = (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28;
     // variants, chosen online per size class: lanes per edge (1, 2, 4 or 8)
@@ -990,11 +1001,11 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       dim3 grid(nblocks((size_t) ne * g));
       if (ea) {
         hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ne, ev, uoff, uadj, udir, vel, xs,
-                              mts, eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), sst);
+                              mts, eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
         check_launch(kn);
       } else {
         launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ne, ev, uoff, uadj, udir, vel, xs, mts, eq.get(),
-                     qual_floor, qfe, codes.get(), cq.get(), scratch_words(), sst);
+                     qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
       }
     };
     tuned(tuner, ne, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
@@ -1026,7 +1037,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     });
   }
   eq.release();
-  if (!read_word(W_ANY)) return 0;
+  if (read_word(W_ANY) != tag) return 0;
   // collapse_to_vert (collapses_to_verts.cpp:13-70)
   u32 nv = count(m, 0);
   Up const& uv = ask_up(m, 0, 1);
@@ -1247,13 +1258,13 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
   double const* h = size_of(m);
   double const* mt = metric_of(m);
   Buf<u32> codes(ne);
-  zero_words(0, 1);
+  u32 tag = next_any_tag();
   if (ne) {
     double prof_bytes = (double) ne * 12 + (double) count(m, 0) * 32;
     launch_timed("coarsen_size_code_f64", prof_bytes, coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
-                       ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words());
+                       ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words(), tag);
   }
-  if (!read_word(W_ANY)) return 0;
+  if (read_word(W_ANY) != tag) return 0;
   if (g_batch) {
     batch_mask(m, 1, codes.get());
     Buf<double> qfe = batch_floors(m, 1);
@@ -1494,7 +1505,7 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
     u32 ne, u32 const* __restrict__ uo, u32 const* __restrict__ ua, u32 const* __restrict__ ud,
     u32 const* __restrict__ ev, u32 const* __restrict__ vt, double const* __restrict__ x,
     double const* __restrict__ mt, double const* __restrict__ eq, u32* __restrict__ cand,
-    double* __restrict__ out, u32* __restrict__ rsz, u32* __restrict__ words, u32* __restrict__ err) {
+    double* __restrict__ out, u32* __restrict__ rsz, u32* __restrict__ words, u32 tag, u32* __restrict__ err) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ne || !cand[i]) return;
   double old_minq = 1;
@@ -1528,7 +1539,7 @@ __global__ __launch_bounds__(BLOCK) void swap_quality_f64(
   if (sq > old_minq) {
     out[i] = sq;
     rsz[i] = rs;
-    wave_flag(words + W_ANY, true);
+    wave_flag_value(words + W_ANY, true, tag);
   } else {
     cand[i] = 0;
   }
@@ -1666,15 +1677,15 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   Buf<double> eq = mesh_qualities(m);
   Buf<double> q(ne);
   Buf<u32> rsz = filled(ne, 0);
-  zero_words(0, 1);
+  u32 tag = next_any_tag();
   {
     double prof_bytes = (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24;
     launch_timed("swap_quality_f64", prof_bytes, swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), ne, u.offsets.get(),
                        u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m), eq.get(), cand.get(), q.get(),
-                       rsz.get(), scratch_words(), dev_error_word());
+                       rsz.get(), scratch_words(), tag, dev_error_word());
   }
   eq.release();
-  if (!read_word(W_ANY)) return 0;
+  if (read_word(W_ANY) != tag) return 0;
   Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
   if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();

@@ -227,6 +227,13 @@ __device__ inline void wave_flag(u32* w, bool pred) {
   unsigned long long act = __ballot(1);
   if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) *w = 1u;
 }
+// *w = v once per wave if any active lane has pred (a tagged "any" word: the
+// host compares it with the v of this launch, so it needs no clearing)
+__device__ inline void wave_flag_value(u32* w, bool pred, u32 v) {
+  unsigned long long m = __ballot(pred);
+  unsigned long long act = __ballot(1);
+  if (m && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) *w = v;
+}
 // *w = 1 for the lanes with pred, one store per distinct address in the wave
 __device__ inline void wave_flag_at(u32* w, bool pred) {
   unsigned long long todo = __ballot(pred);

@@ -254,24 +254,58 @@ __global__ __launch_bounds__(BLOCK) void upfd_rank_split(u32 nlows, u32 deg, u32
   }
 }
 
+// The bucket counters: one buffer per host thread, all zero between uses
+// (upfd_fill counts every bucket back down to zero), so a call needs no
+// clearing launch.  dirty marks a call that did not get as far as the fill.
+namespace {
+struct ZeroCounters {
+  u32* p = nullptr;
+  size_t cap = 0;
+  bool dirty = false;
+};
+ZeroCounters& zero_counters() {
+  static thread_local ZeroCounters z;
+  return z;
+}
+u32* take_zeroed_counters(u32 n) {
+  ZeroCounters& z = zero_counters();
+  if (n > z.cap) {
+    // stream-ordered release: the arena reuses the block only after the
+    // stream's earlier work
+    if (z.p) dev_free(z.p);
+    size_t cap = z.cap * 2 > n ? z.cap * 2 : n;
+    if (cap > 0xffffffffull) cap = 0xffffffffull;
+    z.cap = cap < ((size_t) 1 << 16) ? ((size_t) 1 << 16) : cap;
+    z.p = static_cast<u32*>(dev_alloc(z.cap * sizeof(u32)));
+    z.dirty = true;
+  }
+  if (z.dirty) {
+    fill(z.p, (u32) z.cap, 0);
+    z.dirty = false;
+  }
+  return z.p;
+}
+}  // namespace
+
 void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_highs, Up& u) {
   u32 deg = h_down_deg[hd][ld];
   if ((size_t) nhighs * deg >= ((size_t) 1 << 32)) fail("up_from_down: more than 2^32 uses", __FILE__, __LINE__);
   u32 nuses = nhighs * deg;
-  Buf<u32> counts = filled(nlows, 0);
+  u32* counts = take_zeroed_counters(nlows);
   if (nuses) {
+    zero_counters().dirty = true;
     // compulsory bytes: the down map in, each counter read and written
     launch_timed("upfd_count", (double) nuses * 4 + (double) nlows * 8, upfd_count, dim3(nblocks(nuses)),
-                 dim3(BLOCK), nuses, lows_of_highs, counts.get());
+                 dim3(BLOCK), nuses, lows_of_highs, counts);
   }
-  u.offsets = exscan(counts.get(), nlows);
+  u.offsets = exscan(counts, nlows);
   Buf<u32> tmp(nuses);
   if (nuses) {
     // down map and uses out, offsets and counters
     launch_timed("upfd_fill", (double) nuses * 8 + (double) nlows * 12, upfd_fill, dim3(nblocks(nuses)),
-                 dim3(BLOCK), nuses, lows_of_highs, u.offsets.get(), counts.get(), tmp.get());
+                 dim3(BLOCK), nuses, lows_of_highs, u.offsets.get(), counts, tmp.get());
+    zero_counters().dirty = false;
   }
-  counts.release();
   u.adj = Buf<u32>(nuses);
   u.dirs = Buf<u32>(nuses);
   if (nlows && nuses) {
