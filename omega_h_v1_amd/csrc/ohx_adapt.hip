@@ -1148,10 +1148,18 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       check_launch("subset_tags");
     }
   }
-  // coarsen_all_ents (coarsen_common.cpp:91-107), top dimension first
+  // coarsen_all_ents (coarsen_common.cpp:91-107), top dimension first.
+  // Phase 1 decides every dimension's survivors and products on the device
+  // (a dimension's fused sides feed the next lower one, no host decision
+  // needed); one read then brings all the totals; phase 2 writes the
+  // outputs.
   bool has_class = find_tag(m, 0, "class_dim") != nullptr;
   bool has_id = find_tag(m, 0, "class_id") != nullptr;
   Buf<u32> fused[4];
+  struct DimPlan {
+    Buf<u32> gv, gd, egoff, esoff;
+  };
+  DimPlan plan[4];
   for (u32 dd = 0; dd < dim; ++dd) {
     u32 d = dim - dd;
     u32 n = count(m, d);
@@ -1211,22 +1219,36 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
         return 1u;
       }, esoff.get());
     }
-    will.release();
-    u32 tots[2];
-    u32 const* pt[2] = {egoff.get() + n, esoff.get() + n};
-    read_many(pt, 2, tots);
-    u32 ngen = tots[0], nsame = tots[1];
+    plan[d] = DimPlan{std::move(gv), std::move(gd), std::move(egoff), std::move(esoff)};
+  }
+  u32 tots[8];
+  {
+    u32 const* pt[8];
+    u32 k = 0;
+    for (u32 d = 1; d <= dim; ++d) {
+      u32 n = count(m, d);
+      pt[k++] = plan[d].egoff.get() + n;
+      pt[k++] = plan[d].esoff.get() + n;
+    }
+    read_many(pt, k, tots);
+  }
+  for (u32 dd = 0; dd < dim; ++dd) {
+    u32 d = dim - dd;
+    u32 n = count(m, d);
+    u32 vpe = d + 1;
+    u32 ngen = tots[2 * (d - 1)], nsame = tots[2 * (d - 1) + 1];
+    DimPlan& pl = plan[d];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     Buf<u32> cdo, cio;
     CoarsenArgs a{};
     a.d = d;
     a.n = n;
     a.nsame = nsame;
-    a.vent = vent;
-    a.esoff = esoff.get();
-    a.egoff = egoff.get();
-    a.gv = gv.get();
-    a.gd = gd.get();
+    a.vent = ask_down(m, d, 0);
+    a.esoff = pl.esoff.get();
+    a.egoff = pl.egoff.get();
+    a.gv = pl.gv.get();
+    a.gd = pl.gd.get();
     a.vsoff = soff.get();
     a.out = vout.get();
     if (has_class) {
@@ -1246,7 +1268,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
       if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
     }
-    if (d == dim) coarsen_transfer(m, mo, goff.get(), total, egoff.get(), esoff.get(), nsame, ngen);
+    if (d == dim) coarsen_transfer(m, mo, goff.get(), total, pl.egoff.get(), pl.esoff.get(), nsame, ngen);
   }
   say("collapsed", total, 1);
   overwrite(m, std::move(mo));
