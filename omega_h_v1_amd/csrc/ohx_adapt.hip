@@ -1037,7 +1037,10 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     });
   }
   eq.release();
-  if (read_word(W_ANY) != tag) return 0;
+  // no host check here: if the quality check left no collapse, the
+  // candidates below are empty, the independent set settles at once and the
+  // one read of its total returns 0 (check_coarsen_quality's early return)
+  (void) tag;
   // collapse_to_vert (collapses_to_verts.cpp:13-70)
   u32 nv = count(m, 0);
   Up const& uv = ask_up(m, 0, 1);
@@ -1126,6 +1129,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   cq.release();
   vq.release();
   u32 total = got[1];
+  if (!total) return 0;
   indset.release();
   Buf<u32> soff = negate_offsets(goff.get(), nv);
   u32 nv_out = nv - total;
