@@ -12,6 +12,7 @@
 #include <thread>
 #include <memory>
 #include <string>
+#include <type_traits>
 
 #include "../../include/ohx.h"
 #include "ohx_adapt.hpp"
@@ -64,11 +65,21 @@ static void guard_void(F&& f) {
   }
 }
 
-// the public layer mirrors the reference's assert/abort behaviour
+// the public layer mirrors the reference's assert/abort behaviour, and
+// returns only once the library's stream has finished: the caller reads the
+// returned arrays and fills the new ones from its own stream (the
+// reference's arrays are ready when its calls return)
 template <class F>
 static auto osh_guard(F&& f) -> decltype(f()) {
   try {
-    return f();
+    if constexpr (std::is_void_v<decltype(f())>) {
+      f();
+      sync();
+    } else {
+      auto r = f();
+      sync();
+      return r;
+    }
   } catch (std::exception const& e) {
     std::fprintf(stderr, "omega_h (MI355X): %s\n", e.what());
     std::abort();
