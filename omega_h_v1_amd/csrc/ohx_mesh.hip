@@ -210,24 +210,26 @@ Buf<double> measure_edges(Mesh& m) {
 // --------------------------------------------------------- up_from_down
 // Upward adjacency as a stable counting sort of uses by low entity
 // (invert_map.cpp:72-136 + up_from_down.cpp:9-47): per low, the uses in
-// increasing use index (= high * deg + direction).  Counts and slots come
-// from atomics; each bucket is then sorted, which makes the result exact.
+// increasing use index (= high * deg + direction).  The counting atomics
+// also hand each use a slot in its bucket; each bucket is then sorted, which
+// makes the result exact whatever order the atomics ran in.
 __global__ __launch_bounds__(BLOCK) void upfd_count(u32 nuses, u32 const* __restrict__ in,
-                                                    u32* __restrict__ counts) {
+                                                    u32* __restrict__ counts, u32* __restrict__ slot) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
-  if (i < nuses) atomicAdd(&counts[in[i]], 1u);
+  if (i < nuses) slot[i] = atomicAdd(&counts[in[i]], 1u);
 }
 
-// slots are taken by counting the bucket back down (the bucket is sorted
-// afterwards, so the slot order does not matter)
+// every use to its slot (no atomics); the use that took slot 0 of a bucket
+// clears that bucket's counter, so all counters are zero again afterwards
 __global__ __launch_bounds__(BLOCK) void upfd_fill(u32 nuses, u32 const* __restrict__ in,
-                                                   u32 const* __restrict__ offs, u32* __restrict__ counts,
-                                                   u32* __restrict__ out) {
+                                                   u32 const* __restrict__ slot, u32 const* __restrict__ offs,
+                                                   u32* __restrict__ counts, u32* __restrict__ out) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= nuses) return;
   u32 d = in[i];
-  u32 j = atomicSub(&counts[d], 1u) - 1u;
+  u32 j = slot[i];
   out[offs[d] + j] = i;
+  if (j == 0) counts[d] = 0;
 }
 
 // G lanes per low entity: each use's final slot is its rank among the
@@ -255,8 +257,8 @@ __global__ __launch_bounds__(BLOCK) void upfd_rank_split(u32 nlows, u32 deg, u32
 }
 
 // The bucket counters: one buffer per host thread, all zero between uses
-// (upfd_fill counts every bucket back down to zero), so a call needs no
-// clearing launch.  dirty marks a call that did not get as far as the fill.
+// (upfd_fill clears every bucket it fills), so a call needs no clearing
+// launch.  dirty marks a call that did not get as far as the fill.
 namespace {
 struct ZeroCounters {
   u32* p = nullptr;
@@ -292,20 +294,24 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
   if ((size_t) nhighs * deg >= ((size_t) 1 << 32)) fail("up_from_down: more than 2^32 uses", __FILE__, __LINE__);
   u32 nuses = nhighs * deg;
   u32* counts = take_zeroed_counters(nlows);
+  Buf<u32> slot(nuses);
   if (nuses) {
     zero_counters().dirty = true;
     // compulsory bytes: the down map in, each counter read and written
-    launch_timed("upfd_count", (double) nuses * 4 + (double) nlows * 8, upfd_count, dim3(nblocks(nuses)),
-                 dim3(BLOCK), nuses, lows_of_highs, counts);
+    // compulsory bytes: the down map in, slots out, each counter read and
+    // written
+    launch_timed("upfd_count", (double) nuses * 8 + (double) nlows * 8, upfd_count, dim3(nblocks(nuses)),
+                 dim3(BLOCK), nuses, lows_of_highs, counts, slot.get());
   }
   u.offsets = exscan(counts, nlows);
   Buf<u32> tmp(nuses);
   if (nuses) {
-    // down map and uses out, offsets and counters
-    launch_timed("upfd_fill", (double) nuses * 8 + (double) nlows * 12, upfd_fill, dim3(nblocks(nuses)),
-                 dim3(BLOCK), nuses, lows_of_highs, u.offsets.get(), counts, tmp.get());
+    // down map and slots in, uses out, offsets read and counters cleared
+    launch_timed("upfd_fill", (double) nuses * 12 + (double) nlows * 8, upfd_fill, dim3(nblocks(nuses)),
+                 dim3(BLOCK), nuses, lows_of_highs, slot.get(), u.offsets.get(), counts, tmp.get());
     zero_counters().dirty = false;
   }
+  slot.release();
   u.adj = Buf<u32>(nuses);
   u.dirs = Buf<u32>(nuses);
   if (nlows && nuses) {
@@ -401,33 +407,56 @@ __global__ __launch_bounds__(BLOCK) void reflect_down_hash(u32 nh, u32 const* __
   u32 hv[VPH];
 #pragma unroll
   for (u32 k = 0; k < VPH; ++k) hv[k] = vh[(size_t) i * VPH + k];
+  // the LPH lookups advance together one probe at a time, so each probe
+  // level issues all its table reads, then all its vertex reads, before
+  // waiting (two memory round trips per level instead of per lookup)
+  u32 w[LPH][VPL], slot[LPH], found[LPH];
+  bool open[LPH];
 #pragma unroll
   for (u32 j = 0; j < LPH; ++j) {
     u32 const* lc = canon(HD, LD, 0, j);
-    u32 w[VPL];
 #pragma unroll
-    for (u32 k = 0; k < VPL; ++k) w[k] = hv[lc[k]];
-    sort_small(w);
-    u32 slot = tuple_hash(w, mask);
-    u32 found = INVALID;
-    for (u32 probe = 0; probe <= mask; ++probe) {
-      u32 e = table[slot];
-      if (e == INVALID) break;
-      u32 t[VPL];
+    for (u32 k = 0; k < VPL; ++k) w[j][k] = hv[lc[k]];
+    sort_small(w[j]);
+    slot[j] = tuple_hash(w[j], mask);
+    found[j] = INVALID;
+    open[j] = true;
+  }
+  for (u32 probe = 0; probe <= mask; ++probe) {
+    u32 e[LPH];
 #pragma unroll
-      for (u32 k = 0; k < VPL; ++k) t[k] = vl[(size_t) e * VPL + k];
-      sort_small(t);
+    for (u32 j = 0; j < LPH; ++j) e[j] = open[j] ? table[slot[j]] : INVALID;
+    u32 t[LPH][VPL];
+#pragma unroll
+    for (u32 j = 0; j < LPH; ++j)
+#pragma unroll
+      for (u32 k = 0; k < VPL; ++k) t[j][k] = e[j] != INVALID ? vl[(size_t) e[j] * VPL + k] : INVALID;
+    bool more = false;
+#pragma unroll
+    for (u32 j = 0; j < LPH; ++j) {
+      if (!open[j]) continue;
+      if (e[j] == INVALID) {  // an empty slot ends the search: not present
+        open[j] = false;
+        continue;
+      }
+      sort_small(t[j]);
       bool same = true;
 #pragma unroll
-      for (u32 k = 0; k < VPL; ++k) same = same && (t[k] == w[k]);
+      for (u32 k = 0; k < VPL; ++k) same = same && (t[j][k] == w[j][k]);
       if (same) {
-        found = e;
-        break;
+        found[j] = e[j];
+        open[j] = false;
+      } else {
+        slot[j] = (slot[j] + 1) & mask;
+        more = true;
       }
-      slot = (slot + 1) & mask;
     }
-    if (found == INVALID) *err = ERR_REFLECT_NOT_FOUND;
-    out[(size_t) i * LPH + j] = found;
+    if (!more) break;
+  }
+#pragma unroll
+  for (u32 j = 0; j < LPH; ++j) {
+    if (found[j] == INVALID) *err = ERR_REFLECT_NOT_FOUND;
+    out[(size_t) i * LPH + j] = found[j];
   }
 }
 
