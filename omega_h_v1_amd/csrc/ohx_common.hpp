@@ -245,6 +245,33 @@ __device__ inline void wave_flag_at(u32* w, bool pred) {
     todo &= ~__ballot(same);
   }
 }
+// atomicAdd(&counts[key], 1) for the lanes with pred, returning each lane's
+// old value as if the adds ran one after another in some order: the lanes
+// with the same key are combined, one atomic per distinct key in the wave,
+// and a lane's value is the leader's returned base plus its rank among them.
+// Every lane of the wave must call it (it ballots).
+__device__ inline u32 wave_slot_add(u32* counts, u32 key, bool pred) {
+  unsigned long long todo = __ballot(pred);
+  u32 const lane = (u32) __lane_id();
+  unsigned long long const below = (1ull << lane) - 1ull;
+  u32 leader_of = lane, rank = 0, n = 0;
+  while (todo) {
+    int leader = __ffsll((long long) todo) - 1;
+    u32 lk = (u32) __builtin_amdgcn_readlane((int) key, leader);
+    bool same = pred && key == lk;
+    unsigned long long mm = __ballot(same);
+    if (same) {
+      leader_of = (u32) leader;
+      rank = (u32) __popcll(mm & below);
+    }
+    if (lane == (u32) leader) n = (u32) __popcll(mm);
+    todo &= ~mm;
+  }
+  u32 base = 0;
+  if (pred && leader_of == lane) base = atomicAdd(&counts[key], n);
+  base = (u32) __shfl((int) base, (int) leader_of);
+  return base + rank;
+}
 
 // ---------------------------------------------------------------- timing
 // Per-kernel-name timers, enabled by ohx_profile_enable().  A timed launch

@@ -213,10 +213,21 @@ Buf<double> measure_edges(Mesh& m) {
 // increasing use index (= high * deg + direction).  The counting atomics
 // also hand each use a slot in its bucket; each bucket is then sorted, which
 // makes the result exact whatever order the atomics ran in.
+// COMBINE: a wave combines its adds per distinct low (one atomic each),
+// which pays where the uses of a low sit close together in the down map (a
+// freshly generated mesh) and costs a loop over the wave's distinct lows
+// where they do not (a mesh after many adapt passes)
+template <bool COMBINE>
 __global__ __launch_bounds__(BLOCK) void upfd_count(u32 nuses, u32 const* __restrict__ in,
                                                     u32* __restrict__ counts, u32* __restrict__ slot) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
-  if (i < nuses) slot[i] = atomicAdd(&counts[in[i]], 1u);
+  if (COMBINE) {
+    bool live = i < nuses;
+    u32 s = wave_slot_add(counts, live ? in[i] : 0u, live);
+    if (live) slot[i] = s;
+  } else if (i < nuses) {
+    slot[i] = atomicAdd(&counts[in[i]], 1u);
+  }
 }
 
 // every use to its slot (no atomics); the use that took slot 0 of a bucket
@@ -299,9 +310,29 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
     zero_counters().dirty = true;
     // compulsory bytes: the down map in, each counter read and written
     // compulsory bytes: the down map in, slots out, each counter read and
-    // written
-    launch_timed("upfd_count", (double) nuses * 8 + (double) nlows * 8, upfd_count, dim3(nblocks(nuses)),
-                 dim3(BLOCK), nuses, lows_of_highs, counts, slot.get());
+    // written; plain or wave-combined adds, chosen online per (high, low)
+    // dimension pair and size class
+    static thread_local Tuner tuners[4][4] = {
+        {Tuner("-", {0}), Tuner("-", {0}), Tuner("-", {0}), Tuner("-", {0})},
+        {Tuner("upfd_count_1_0", {0, 1}), Tuner("-", {0}), Tuner("-", {0}), Tuner("-", {0})},
+        {Tuner("upfd_count_2_0", {0, 1}), Tuner("upfd_count_2_1", {0, 1}), Tuner("-", {0}), Tuner("-", {0})},
+        {Tuner("upfd_count_3_0", {0, 1}), Tuner("upfd_count_3_1", {0, 1}), Tuner("upfd_count_3_2", {0, 1}),
+         Tuner("-", {0})}};
+    double bytes = (double) nuses * 8 + (double) nlows * 8;
+    auto go = [&](auto kernel, hipEvent_t ea, hipEvent_t eb) {
+      if (ea) {
+        hipExtLaunchKernelGGL(kernel, dim3(nblocks(nuses)), dim3(BLOCK), 0, stream(), ea, eb, 0, nuses,
+                              lows_of_highs, counts, slot.get());
+        check_launch("upfd_count");
+      } else {
+        launch_timed("upfd_count", bytes, kernel, dim3(nblocks(nuses)), dim3(BLOCK), nuses, lows_of_highs, counts,
+                     slot.get());
+      }
+    };
+    tuned(tuners[hd][ld], nuses, [&](u32 v, hipEvent_t ea, hipEvent_t eb) {
+      if (v) go(upfd_count<true>, ea, eb);
+      else go(upfd_count<false>, ea, eb);
+    });
   }
   u.offsets = exscan(counts, nlows);
   Buf<u32> tmp(nuses);
@@ -351,8 +382,9 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
 // vertices (reflect_down.cpp:13-81, find_by_verts.hpp:14-76).  The answer is
 // unique in a conforming mesh, so any search finds the reference's: here an
 // open-addressing hash table keyed by the sorted vertex tuple (lows inserted
-// once, two to four probes per lookup) replaces the scan of the first
-// vertex's upward adjacency, whose random reads grow with vertex valence.
+// once, mostly one probe per lookup at the table's load) replaces the scan
+// of the first vertex's upward adjacency, whose random reads grow with
+// vertex valence.
 template <u32 N>
 __device__ inline void sort_small(u32 (&v)[N]) {
 #pragma unroll
@@ -469,9 +501,14 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   Buf<u32> out((size_t) nh * lph);
   if (!nh) return out;
   u32* err = dev_error_word();
-  // table of at least twice as many slots as lows (load factor <= 1/2)
+  // table of at least four times as many slots as lows (load factor <= 1/4:
+  // most lookups end at their first probe)
+#ifndef OHX_HASH_SLOTS_PER_LOW
+#define OHX_HASH_SLOTS_PER_LOW 4
+#endif
+  OHX_REQUIRE((size_t) OHX_HASH_SLOTS_PER_LOW * nl <= ((size_t) 1 << 31), "reflect_down: too many lows for the table");
   u32 cap = 1024;
-  while (cap < 2 * (size_t) nl) cap *= 2;
+  while (cap < (size_t) OHX_HASH_SLOTS_PER_LOW * nl) cap *= 2;
   u32 mask = cap - 1;
   Buf<u32> table = filled(cap, INVALID);
   // compulsory bytes: low vertices in, the table written once
