@@ -830,6 +830,8 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
         }
         if (!(q == q)) nan = true;
         minq32 = q < minq32 ? q : minq32;
+        // one element certainly below the floor settles the rejection
+        if (minq32 < (float) qual_floor - SCREEN_MARGIN) break;
       }
 #pragma unroll
       for (u32 d = G / 2; d >= 1; d >>= 1) {
@@ -871,6 +873,10 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
         q = element_quality(DIM, p);
       }
       if (q < minq) minq = q;
+      // below the floor the direction is rejected whatever the remaining
+      // elements give (the minimum only falls), so the lane stops; the
+      // mixed kernel keeps the full minimum for its error statistic
+      if (!MIXED && minq < qual_floor) break;
     }
     minq = group_min<G>(minq);
     if (eq) old_minq = group_min<G>(old_minq);
@@ -1515,6 +1521,10 @@ __device__ inline u32 choose_edge_swap(u32 rs, V3 const* ex, V3 const* rx, doubl
       double tq = cache[tri];
       if (tq < mesh_minq) mesh_minq = tq;
       if (mesh_minq < 0) break;
+      // a triangulation at or below the best so far cannot replace it (a
+      // later one must be strictly better), so its remaining triangles are
+      // not needed: the choice and its quality are unchanged
+      if (!(mesh_minq > best)) break;
     }
     if (mesh_minq > best) {
       code = i;
