@@ -251,6 +251,12 @@ unsigned ohx_tuning_report(char* buf, unsigned cap) OSH_PUBLIC;
 /* pin the tuner of that name (as in the report) to one of its variants;
  * variant < 0 lets it tune again.  Every variant gives identical results. */
 void ohx_tuning_force(char const* name, int variant) OSH_PUBLIC;
+/* add earlier decisions, lines in the report's format, to the table the
+ * tuners consult before timing a size class (the library loads
+ * tuning_gfx950.txt from its package directory, or the file OHX_TUNING_FILE
+ * names, at its first tuned launch); returns the entries read, OHX_ERROR if
+ * the file cannot be read.  Call before adapting. */
+unsigned ohx_tuning_load(char const* path) OSH_PUBLIC;
 /* per-kernel timing: while enabled, each timed launch carries a start and a
  * stop event in its own dispatch (hipExtLaunchKernel), so the interval is
  * the kernel's execution alone */

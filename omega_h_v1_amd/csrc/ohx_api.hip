@@ -712,6 +712,9 @@ void ohx_mixed_stats_reset(void) {
 void ohx_tuning_force(char const* name, int variant) {
   guard_void([&]() { tuning_force(name, variant); });
 }
+unsigned ohx_tuning_load(char const* path) {
+  return guard(OHX_ERROR, [&]() { return tuning_load_file(path); });
+}
 unsigned ohx_tuning_report(char* buf, unsigned cap) {
   return guard(0u, [&]() {
     std::string r = tuning_report();

@@ -343,5 +343,9 @@ void tuned(Tuner& t, u32 n, L launch) {
 std::string tuning_report();
 // pin a tuner to one of its variants (variant < 0: tune again)
 void tuning_force(char const* name, int variant);
+// add decisions ("name:2^class=variant" lines) to the table the tuners
+// consult before timing; returns the number of entries read
+u32 tuning_load_text(std::string const& text);
+u32 tuning_load_file(char const* path);
 
 }  // namespace ohx

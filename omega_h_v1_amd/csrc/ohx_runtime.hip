@@ -1,9 +1,13 @@
 // Runtime pieces: stream, pinned staging, caching HBM arena, event timers.
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
-#include <cstring>
 
 #include "ohx_common.hpp"
 
@@ -349,6 +353,78 @@ void tuning_force(char const* name, int variant) {
     forced_variants()[name] = (u32) variant;
 }
 
+// Decisions measured earlier: tuner name -> size class -> variant.  A tuner
+// takes the table's entry for a size class it has not timed yet instead of
+// timing every variant there (each timing runs the whole kernel, which at
+// large sizes costs as much as several passes save).  The table ships next
+// to the library (tuning_gfx950.txt in the package directory, written from
+// tuning reports of MI355X runs); OHX_TUNING_FILE names another file, or no
+// file when empty; ohx_tuning_load adds entries.  Load before adapting.
+static std::map<std::string, std::map<u32, u32>>& tuning_table() {
+  static std::map<std::string, std::map<u32, u32>> t;
+  return t;
+}
+
+// lines "name:2^class=variant" (the report's format; "?" entries skipped)
+u32 tuning_load_text(std::string const& text) {
+  u32 n = 0;
+  size_t pos = 0;
+  while (pos < text.size()) {
+    size_t end = text.find('\n', pos);
+    if (end == std::string::npos) end = text.size();
+    std::string line = text.substr(pos, end - pos);
+    pos = end + 1;
+    size_t colon = line.rfind(":2^"), eq = line.rfind('=');
+    if (colon == std::string::npos || eq == std::string::npos || eq < colon) continue;
+    char* stop = nullptr;
+    unsigned long cls = std::strtoul(line.c_str() + colon + 3, &stop, 10);
+    if (stop != line.c_str() + eq || cls > 32) continue;
+    char const* vs = line.c_str() + eq + 1;
+    if (*vs < '0' || *vs > '9') continue;
+    unsigned long v = std::strtoul(vs, &stop, 10);
+    tuning_table()[line.substr(0, colon)][(u32) cls] = (u32) v;
+    ++n;
+  }
+  return n;
+}
+
+u32 tuning_load_file(char const* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) fail("tuning table: cannot open the file", __FILE__, __LINE__);
+  std::string text;
+  char buf[4096];
+  size_t k;
+  while ((k = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, k);
+  std::fclose(f);
+  return tuning_load_text(text);
+}
+
+static void load_default_table() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::string path;
+    if (char const* env = std::getenv("OHX_TUNING_FILE")) {
+      path = env;
+    } else {
+      // <package>/lib/libomega_h_amd.so -> <package>/tuning_gfx950.txt
+      Dl_info info;
+      if (dladdr((void*) &load_default_table, &info) && info.dli_fname) {
+        path = info.dli_fname;
+        for (int up = 0; up < 2; ++up) {
+          size_t slash = path.rfind('/');
+          path = slash == std::string::npos ? std::string(".") : path.substr(0, slash);
+        }
+        path += "/tuning_gfx950.txt";
+      }
+    }
+    if (path.empty()) return;
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return;  // no table: every size class is timed online
+    std::fclose(f);
+    tuning_load_file(path.c_str());
+  });
+}
+
 static u32 size_class(u32 n) {
   u32 c = 0;
   while ((1u << c) < n && c < 32) ++c;
@@ -393,7 +469,17 @@ u32 Tuner::pick(u32 n, hipEvent_t* a, hipEvent_t* b) {
   }
   u32 cls = size_class(n);
   Cls& c = classes[cls];
-  if (c.best.empty()) c.best.assign(variants.size(), -1.0);
+  if (c.best.empty()) {
+    c.best.assign(variants.size(), -1.0);
+    load_default_table();
+    auto t = tuning_table().find(name);
+    if (t != tuning_table().end()) {
+      auto e = t->second.find(cls);
+      if (e != t->second.end())
+        for (u32 k = 0; k < variants.size(); ++k)
+          if (variants[k] == e->second) c.chosen = (int) k;
+    }
+  }
   if (c.chosen < 0 && !pending.empty()) harvest();
   if (c.chosen >= 0) return variants[c.chosen];
   // the next variant without a timing (round robin while results are due)
