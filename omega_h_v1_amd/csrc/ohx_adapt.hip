@@ -739,26 +739,57 @@ static void check_exposed_sides(Mesh& m, u32 ed, u32* codes) {
   u32 const* ua = u.adj.get();
   u32 const* ud = u.dirs.get();
   u32* err = dev_error_word();
+  // the edge's adjacent entities are taken CH at a time with their reads
+  // issued level by level (entities, then their vertices and sides, then the
+  // class dimensions), so a thread waits on three round trips per chunk
+  // rather than on a chain per entity; the decisions then run in the
+  // reference's order on the loaded values
   parallel_for(count(m, 1), OHX_LAMBDA(u32 i) {
     u32 c = codes[i];
     if (c == DONT_COLLAPSE) return;
     u32 const* so = opp(ed, 0);
-    for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
-      u32 ent = ua[j];
-      u32 const* eve = canon(ed, 1, 0, ud[j]);
-      for (u32 k = 0; k < 2; ++k) {
-        if (!collapses(c, k)) continue;
-        u32 va = ev[2 * i + (1 - k)];
-        u32 l;
-        for (l = 0; l < 2; ++l) {
-          u32 vb = vent[(size_t) ent * vpe + eve[l]];
-          if (va == vb) {
-            u32 side = sent[(size_t) ent * spe + so[eve[l]]];
-            if (scd[side] != ecd[ent]) c = dont_collapse(c, k);
-            break;
+    u32 const ve[2] = {ev[2 * (size_t) i], ev[2 * (size_t) i + 1]};
+    u32 const f = uo[i], e = uo[i + 1];
+    constexpr u32 CH = 8;
+    for (u32 j0 = f; j0 < e; j0 += CH) {
+      u32 ent[CH], lv[CH][2], vb[CH][2], sd[CH][2], sc[CH][2], ec[CH];
+#pragma unroll
+      for (u32 q = 0; q < CH; ++q) {
+        ent[q] = 0;
+        lv[q][0] = lv[q][1] = 0;
+        if (j0 + q < e) {
+          ent[q] = ua[j0 + q];
+          u32 const* eve = canon(ed, 1, 0, ud[j0 + q]);
+          lv[q][0] = eve[0];
+          lv[q][1] = eve[1];
+        }
+      }
+#pragma unroll
+      for (u32 q = 0; q < CH; ++q)
+#pragma unroll
+        for (u32 l = 0; l < 2; ++l) {
+          vb[q][l] = j0 + q < e ? vent[(size_t) ent[q] * vpe + lv[q][l]] : INVALID;
+          sd[q][l] = j0 + q < e ? sent[(size_t) ent[q] * spe + so[lv[q][l]]] : 0;
+        }
+#pragma unroll
+      for (u32 q = 0; q < CH; ++q) {
+        sc[q][0] = j0 + q < e ? scd[sd[q][0]] : 0;
+        sc[q][1] = j0 + q < e ? scd[sd[q][1]] : 0;
+        ec[q] = j0 + q < e ? ecd[ent[q]] : 0;
+      }
+#pragma unroll
+      for (u32 q = 0; q < CH; ++q) {
+        if (j0 + q >= e) break;
+        for (u32 k = 0; k < 2; ++k) {
+          if (!collapses(c, k)) continue;
+          u32 va = ve[1 - k];
+          u32 l = vb[q][0] == va ? 0u : (vb[q][1] == va ? 1u : 2u);
+          if (l == 2) {
+            *err = ERR_EXPOSED_SIDE;
+          } else if (sc[q][l] != ec[q]) {
+            c = dont_collapse(c, k);
           }
         }
-        if (l == 2) *err = ERR_EXPOSED_SIDE;
       }
     }
     codes[i] = c;
