@@ -320,6 +320,7 @@ struct Tuner {
   };
   struct Cls {
     std::vector<double> best;  // ms per item per variant, < 0 if untimed
+    std::vector<u32> timed;    // timings harvested per variant
     int chosen = -1;
     u32 next = 0;
   };

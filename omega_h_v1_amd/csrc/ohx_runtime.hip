@@ -425,6 +425,18 @@ static void load_default_table() {
   });
 }
 
+// timings per variant before a size class is decided (OHX_TUNING_TRIALS,
+// default 1; more makes the choice robust to run-to-run noise on small
+// kernels, for writing a tuning table)
+static u32 tuning_trials() {
+  static u32 const n = [] {
+    char const* e = std::getenv("OHX_TUNING_TRIALS");
+    long v = e ? std::strtol(e, nullptr, 10) : 1;
+    return (u32) (v < 1 ? 1 : v > 64 ? 64 : v);
+  }();
+  return n;
+}
+
 static u32 size_class(u32 n) {
   u32 c = 0;
   while ((1u << c) < n && c < 32) ++c;
@@ -443,6 +455,7 @@ void Tuner::harvest() {
     Cls& c = classes[tr.cls];
     double per = ms / (tr.items > 0 ? tr.items : 1);
     if (c.best[tr.v] < 0 || per < c.best[tr.v]) c.best[tr.v] = per;
+    ++c.timed[tr.v];
     pool.push_back(tr.a);
     pool.push_back(tr.b);
     pending[k] = pending.back();
@@ -451,7 +464,7 @@ void Tuner::harvest() {
   for (Cls& c : classes) {
     if (c.chosen >= 0 || c.best.empty()) continue;
     bool all = true;
-    for (double b : c.best) all = all && b >= 0;
+    for (u32 t : c.timed) all = all && t >= tuning_trials();
     if (!all) continue;
     u32 arg = 0;
     for (u32 v = 1; v < c.best.size(); ++v)
@@ -471,6 +484,7 @@ u32 Tuner::pick(u32 n, hipEvent_t* a, hipEvent_t* b) {
   Cls& c = classes[cls];
   if (c.best.empty()) {
     c.best.assign(variants.size(), -1.0);
+    c.timed.assign(variants.size(), 0);
     load_default_table();
     auto t = tuning_table().find(name);
     if (t != tuning_table().end()) {
@@ -482,11 +496,12 @@ u32 Tuner::pick(u32 n, hipEvent_t* a, hipEvent_t* b) {
   }
   if (c.chosen < 0 && !pending.empty()) harvest();
   if (c.chosen >= 0) return variants[c.chosen];
-  // the next variant without a timing (round robin while results are due)
+  // the next variant short of its timings (round robin while results are
+  // due: a launch in flight counts once its events are read)
   u32 v = c.next % variants.size();
   for (u32 k = 0; k < variants.size(); ++k) {
     u32 cand = (c.next + k) % variants.size();
-    if (c.best[cand] < 0) {
+    if (c.timed[cand] < tuning_trials()) {
       v = cand;
       break;
     }
