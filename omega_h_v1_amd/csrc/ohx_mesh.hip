@@ -38,6 +38,10 @@ void overwrite(Mesh& m, Mesh&& w) {
     }
   }
   m.dual = std::move(w.dual);
+  for (u32 i = 0; i < 4; ++i) {
+    m.low_table[i] = std::move(w.low_table[i]);
+    m.low_mask[i] = w.low_mask[i];
+  }
 }
 
 Tag* find_tag(Mesh& m, u32 d, char const* name) {
@@ -502,25 +506,33 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
   if (!nh) return out;
   u32* err = dev_error_word();
   // table of at least four times as many slots as lows (load factor <= 1/4:
-  // most lookups end at their first probe)
+  // most lookups end at their first probe); a 3D mesh's edge table serves
+  // both reflections onto edges
 #ifndef OHX_HASH_SLOTS_PER_LOW
 #define OHX_HASH_SLOTS_PER_LOW 4
 #endif
-  OHX_REQUIRE((size_t) OHX_HASH_SLOTS_PER_LOW * nl <= ((size_t) 1 << 31), "reflect_down: too many lows for the table");
-  u32 cap = 1024;
-  while (cap < (size_t) OHX_HASH_SLOTS_PER_LOW * nl) cap *= 2;
-  u32 mask = cap - 1;
-  Buf<u32> table = filled(cap, INVALID);
-  // compulsory bytes: low vertices in, the table written once
-  double ins_bytes = (double) nl * (ld + 1) * 4 + (double) cap * 4;
-  if (nl) {
-    if (ld == 1)
-      launch_timed("lows_hash_insert", ins_bytes, lows_hash_insert<2>, dim3(nblocks(nl)), dim3(BLOCK), nl, vl, mask,
-                   table.get(), err);
-    else
-      launch_timed("lows_hash_insert", ins_bytes, lows_hash_insert<3>, dim3(nblocks(nl)), dim3(BLOCK), nl, vl, mask,
-                   table.get(), err);
+  if (!m.low_table[ld]) {
+    OHX_REQUIRE((size_t) OHX_HASH_SLOTS_PER_LOW * nl <= ((size_t) 1 << 31),
+                "reflect_down: too many lows for the table");
+    u32 cap = 1024;
+    while (cap < (size_t) OHX_HASH_SLOTS_PER_LOW * nl) cap *= 2;
+    u32 mask = cap - 1;
+    Buf<u32> table = filled(cap, INVALID);
+    // compulsory bytes: low vertices in, the table written once
+    double ins_bytes = (double) nl * (ld + 1) * 4 + (double) cap * 4;
+    if (nl) {
+      if (ld == 1)
+        launch_timed("lows_hash_insert", ins_bytes, lows_hash_insert<2>, dim3(nblocks(nl)), dim3(BLOCK), nl, vl,
+                     mask, table.get(), err);
+      else
+        launch_timed("lows_hash_insert", ins_bytes, lows_hash_insert<3>, dim3(nblocks(nl)), dim3(BLOCK), nl, vl,
+                     mask, table.get(), err);
+    }
+    m.low_table[ld] = std::move(table);
+    m.low_mask[ld] = mask;
   }
+  u32 const mask = m.low_mask[ld], cap = mask + 1;
+  Buf<u32> const& table = m.low_table[ld];
   // compulsory bytes: high vertices in, low ids out, the table and the low
   // vertices read once
   double prof_bytes = (double) nh * ((hd + 1) * 4 + lph * 4) + (double) cap * 4 + (double) nl * (ld + 1) * 4;
@@ -536,6 +548,12 @@ static Buf<u32> reflect_down(Mesh& m, u32 hd, u32 ld) {
                  err);
   else
     fail("reflect_down: unsupported dimensions", __FILE__, __LINE__);
+  // only a 3D mesh's edges are found twice (from tets and from faces); any
+  // other table goes back to the arena at once (stream-ordered)
+  if (!(m.dim == 3 && ld == 1 && !m.down[hd == 3 ? 2 : 3][1])) {
+    m.low_table[ld].release();
+    m.low_mask[ld] = 0;
+  }
   return out;
 }
 

@@ -43,6 +43,12 @@ struct Mesh {
   std::unique_ptr<Up> up[4][4];
   std::unique_ptr<Graph> star[4][4];
   Buf<u32> dual;
+  // reflect_down's hash table of a low dimension's entities (slots of
+  // entity ids keyed by their sorted vertices, mask = slots - 1), kept while
+  // another reflection onto the same lows is due (3D: tets and faces onto
+  // edges)
+  Buf<u32> low_table[4];
+  u32 low_mask[4] = {0, 0, 0, 0};
   std::vector<std::unique_ptr<Tag>> tags[4];
 
   Mesh(u32 d, bool f) : dim(d), full(f) {}
