@@ -5,8 +5,10 @@
 // Small inputs (one tile) use one block; larger ones a decoupled look-back
 // scan (tiles in block order, 64-bit (epoch, flag, value) tile words, a
 // wave-wide look-back, bounded spin).  Tiles hold 256 x I items; I (4 or
-// 16) is chosen per call site and size class by the online tuner: small
-// scans want more, smaller tiles to reach every CU, large ones fewer.
+// 16) and the look-back window (64 or 256 tiles per step) are chosen per
+// call site and size class by the online tuner: small scans want more,
+// smaller tiles to reach every CU and a window that spans them, large ones
+// fewer tiles and a narrow window.
 #pragma once
 
 #include "ohx_common.hpp"
@@ -57,7 +59,7 @@ __device__ inline u64 pack(u32 epoch, u32 inclusive, u32 v) {
 
 __device__ inline u32 pad(u32 i) { return i + (i >> 5); }
 
-template <u32 SCAN_I, class F>
+template <u32 SCAN_I, u32 LB, class F>
 __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles, u32* __restrict__ out,
                                                          u64* __restrict__ tiles,
                                                          u32 epoch, u32* __restrict__ host_total,
@@ -95,10 +97,10 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
     // wave 0 publishes this tile and looks back over up to 64 x LB
     // predecessor tiles per step (lane l reads tiles j - LB l .. j - LB l -
     // LB + 1): the closest inclusive prefix ends the walk once every tile
-    // before it in the window is published.  LB = 1: wider windows (LB = 4)
-    // measured slower on large scans, the extra polling loads competing with
-    // the tiles' own traffic in L2.
-    constexpr u32 LB = 1;
+    // before it in the window is published.  A wide window (LB = 4) reaches
+    // an inclusive prefix in one step on scans of up to 256 tiles; on large
+    // scans its extra polling loads compete with the tiles' own traffic in
+    // L2, so the tuner picks LB per size class.
     u32 lane = threadIdx.x;
     u32 prefix = 0;
     if (ntiles > 1) {
@@ -182,13 +184,13 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
 
 }  // namespace scan_detail
 
-template <u32 I, class F>
+template <u32 I, u32 LB, class F>
 void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hipEvent_t eb) {
   constexpr u32 tile = SCAN_T * I;
   u32 nt = n ? (n + tile - 1) / tile : 1;
   ScanTiles st = scan_tiles_for(nt);
   if (ea) {
-    hipExtLaunchKernelGGL(scan_detail::scan_fn_kernel<I, F>, dim3(nt), dim3(SCAN_T), 0, stream(), ea, eb, 0, n, f, nt,
+    hipExtLaunchKernelGGL(scan_detail::scan_fn_kernel<I, LB, F>, dim3(nt), dim3(SCAN_T), 0, stream(), ea, eb, 0, n, f, nt,
                           out, st.tiles, st.epoch, host_total_dev, dev_error_word());
     check_launch("scan");
     return;
@@ -196,19 +198,23 @@ void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hip
   // compulsory bytes: one 4-byte input per item (a lower bound for functors
   // that read more) and the n + 1 offsets out
   double prof_bytes = (double) n * 4 + (double) (n + 1) * 4;
-  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<I, F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
+  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<I, LB, F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
                st.tiles, st.epoch, host_total_dev, dev_error_word());
 }
 
 template <class F>
 void exscan_fn(u32 n, F f, u32* out, u32* host_total_dev = nullptr) {
-  // one tuner per call site (functor type) and host thread
-  static thread_local Tuner tuner("scan", {4, 16});
-  tuned(tuner, n, [&](u32 items, hipEvent_t ea, hipEvent_t eb) {
-    if (items == 4)
-      exscan_launch<4>(n, f, out, host_total_dev, ea, eb);
+  // one tuner per call site (functor type) and host thread; variants: 4 or
+  // 16 items per thread with a 64-tile look-back window, or (44) 4 items
+  // with a 256-tile window
+  static thread_local Tuner tuner("scan", {4, 16, 44});
+  tuned(tuner, n, [&](u32 v, hipEvent_t ea, hipEvent_t eb) {
+    if (v == 4)
+      exscan_launch<4, 1>(n, f, out, host_total_dev, ea, eb);
+    else if (v == 44)
+      exscan_launch<4, 4>(n, f, out, host_total_dev, ea, eb);
     else
-      exscan_launch<16>(n, f, out, host_total_dev, ea, eb);
+      exscan_launch<16, 1>(n, f, out, host_total_dev, ea, eb);
   });
 }
 
