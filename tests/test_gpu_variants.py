@@ -1,10 +1,10 @@
 """Every tuned variant of a kernel gives the same result.
 
 The online tuner picks among interchangeable variants of some kernels (lanes
-per edge, plain or wave-combined counting, scan tiles and look-back
-window); which one runs depends on
-timings, so each is pinned here in turn and the adapt runs of the parity
-cases are compared with the oracle's after every step.
+per edge, plain or wave-combined counting, scan tiles and look-back window);
+which one runs depends on timings, so each is pinned here in turn and the
+adapt runs of the parity cases are compared with the oracle's after every
+step.
 """
 import ctypes as C
 
