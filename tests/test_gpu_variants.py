@@ -1,10 +1,10 @@
 """Every tuned variant of a kernel gives the same result.
 
 The online tuner picks among interchangeable variants of some kernels (lanes
-per edge, plain or wave-combined counting, scan tiles and look-back window);
-which one runs depends on timings, so each is pinned here in turn and the
-adapt runs of the parity cases are compared with the oracle's after every
-step.
+per edge, plain or wave-combined counting, hash insertion, scan tiles and
+look-back window); which one runs depends on timings, so each is pinned here
+in turn and the adapt runs of the parity cases are compared with the
+oracle's after every step.
 """
 import ctypes as C
 
@@ -22,6 +22,8 @@ PINS = [
     ({"coarsen_quality_2d_f64": 8, "coarsen_quality_3d_f64": 8}, "collapse qualities, eight lanes per edge"),
     ({"upfd_count_%d_%d" % (h, l): 1 for h in (1, 2, 3) for l in range(h)}, "up_from_down, wave-combined counting"),
     ({"upfd_count_%d_%d" % (h, l): 0 for h in (1, 2, 3) for l in range(h)}, "up_from_down, plain counting"),
+    ({"lows_hash_insert_1": 1, "lows_hash_insert_2": 1}, "hash insertion, home slots first"),
+    ({"lows_hash_insert_1": 0, "lows_hash_insert_2": 0}, "hash insertion, compare-and-swap only"),
     ({"scan": 44}, "scans, 256-tile look-back window"),
     ({"scan": 16}, "scans, 16 items per thread"),
 ]
