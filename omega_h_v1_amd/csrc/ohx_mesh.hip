@@ -271,6 +271,53 @@ __global__ __launch_bounds__(BLOCK) void upfd_rank_split(u32 nlows, u32 deg, u32
   }
 }
 
+// S lanes per low entity, the bucket held in the lanes: lane j of a segment
+// holds the bucket's j-th use, and each use's rank is counted against the
+// whole bucket read across the segment with lane permutes (no memory reads
+// in the loop).  Buckets longer than S fall back to the loop over the fill
+// buffer.  Every lane of the wave takes part in the permutes.
+template <u32 S>
+__global__ __launch_bounds__(BLOCK) void upfd_rank_seg(u32 nlows, u32 deg, u32 const* __restrict__ offs,
+                                                       u32 const* __restrict__ tmp, u32* __restrict__ adj,
+                                                       u32* __restrict__ dirs) {
+  size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
+  size_t grp = gid / S;
+  u32 const lane = (u32) (gid % S);
+  u32 const seg0 = (threadIdx.x & 63u) & ~(S - 1);
+  bool const live = grp < nlows;
+  u32 f = 0, e = 0;
+  if (live) {
+    f = offs[grp];
+    e = offs[grp + 1];
+  }
+  u32 const k = e - f;
+  // use indices are below 2^32 - 1, so the padding never counts as smaller
+  u32 const v = (live && lane < k) ? tmp[f + lane] : INVALID;
+  u32 r = 0;
+#pragma unroll 8
+  for (u32 t = 0; t < S; ++t) {
+    u32 o = (u32) __shfl((int) v, (int) (seg0 + t), 64);
+    r += o < v ? 1u : 0u;
+  }
+  if (!live) return;
+  if (k <= S) {
+    if (lane < k) {
+      u32 h = v / deg;
+      adj[f + r] = h;
+      dirs[f + r] = v - h * deg;
+    }
+    return;
+  }
+  for (u32 j = f + lane; j < e; j += S) {
+    u32 w = tmp[j];
+    u32 rr = 0;
+    for (u32 q = f; q < e; ++q) rr += tmp[q] < w ? 1u : 0u;
+    u32 h = w / deg;
+    adj[f + rr] = h;
+    dirs[f + rr] = w - h * deg;
+  }
+}
+
 // The bucket counters: one buffer per host thread, all zero between uses
 // (upfd_fill clears every bucket it fills), so a call needs no clearing
 // launch.  dirty marks a call that did not get as far as the fill.
@@ -352,13 +399,15 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
   if (nlows && nuses) {
     // offsets, uses in, (high, direction) out; lanes per bucket (4 to 32)
     // chosen online per (high, low) dimension pair and size class
+    // variants: G lanes per bucket reading the fill buffer (4 to 32), or
+    // (200 + S) the bucket held in S lanes (8 to 64)
+    static std::vector<u32> const V = {4, 8, 16, 32, 208, 216, 232, 264};
     static thread_local Tuner tuners[4][4] = {
         {Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
-        {Tuner("upfd_rank_split_1_0", {4, 8, 16, 32}), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
-        {Tuner("upfd_rank_split_2_0", {4, 8, 16, 32}), Tuner("upfd_rank_split_2_1", {4, 8, 16, 32}), Tuner("-", {4}),
-         Tuner("-", {4})},
-        {Tuner("upfd_rank_split_3_0", {4, 8, 16, 32}), Tuner("upfd_rank_split_3_1", {4, 8, 16, 32}),
-         Tuner("upfd_rank_split_3_2", {4, 8, 16, 32}), Tuner("-", {4})}};
+        {Tuner("upfd_rank_split_1_0", V), Tuner("-", {4}), Tuner("-", {4}), Tuner("-", {4})},
+        {Tuner("upfd_rank_split_2_0", V), Tuner("upfd_rank_split_2_1", V), Tuner("-", {4}), Tuner("-", {4})},
+        {Tuner("upfd_rank_split_3_0", V), Tuner("upfd_rank_split_3_1", V), Tuner("upfd_rank_split_3_2", V),
+         Tuner("-", {4})}};
     double bytes = (double) (nlows + 1) * 4 + (double) nuses * 12;
     u32 const* o = u.offsets.get();
     auto go = [&](auto kernel, u32 g, hipEvent_t ea, hipEvent_t eb) {
@@ -376,7 +425,11 @@ void up_from_down(u32 hd, u32 ld, u32 nhighs, u32 nlows, u32 const* lows_of_high
       if (g == 4) go(upfd_rank_split<4>, g, ea, eb);
       else if (g == 8) go(upfd_rank_split<8>, g, ea, eb);
       else if (g == 16) go(upfd_rank_split<16>, g, ea, eb);
-      else go(upfd_rank_split<32>, g, ea, eb);
+      else if (g == 32) go(upfd_rank_split<32>, g, ea, eb);
+      else if (g == 208) go(upfd_rank_seg<8>, 8, ea, eb);
+      else if (g == 216) go(upfd_rank_seg<16>, 16, ea, eb);
+      else if (g == 232) go(upfd_rank_seg<32>, 32, ea, eb);
+      else go(upfd_rank_seg<64>, 64, ea, eb);
     });
   }
 }

@@ -22,6 +22,12 @@ PINS = [
     ({"coarsen_quality_2d_f64": 8, "coarsen_quality_3d_f64": 8}, "collapse qualities, eight lanes per edge"),
     ({"upfd_count_%d_%d" % (h, l): 1 for h in (1, 2, 3) for l in range(h)}, "up_from_down, wave-combined counting"),
     ({"upfd_count_%d_%d" % (h, l): 0 for h in (1, 2, 3) for l in range(h)}, "up_from_down, plain counting"),
+    ({"upfd_rank_split_%d_%d" % (h, l): 208 for h in (1, 2, 3) for l in range(h)},
+     "up_from_down ranks, buckets held in 8 lanes (longer ones read from memory)"),
+    ({"upfd_rank_split_%d_%d" % (h, l): 264 for h in (1, 2, 3) for l in range(h)},
+     "up_from_down ranks, buckets held in 64 lanes"),
+    ({"upfd_rank_split_%d_%d" % (h, l): 4 for h in (1, 2, 3) for l in range(h)},
+     "up_from_down ranks, 4 lanes reading memory"),
     ({"lows_hash_insert_1": 1, "lows_hash_insert_2": 1}, "hash insertion, home slots first"),
     ({"lows_hash_insert_1": 0, "lows_hash_insert_2": 0}, "hash insertion, compare-and-swap only"),
     ({"scan": 44}, "scans, 256-tile look-back window"),
