@@ -159,6 +159,7 @@ void* dev_alloc(size_t bytes) {
     }
   }
   if (!p) {
+    auto t0 = std::chrono::steady_clock::now();
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
       // give this stream's cached blocks back and retry once
@@ -166,6 +167,10 @@ void* dev_alloc(size_t bytes) {
       dev_trim();
       OHX_CHECK(hipMalloc(&p, c));
     }
+    // host-side count and wall time of the arena's misses
+    if (prof_on())
+      prof_host("host_malloc",
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
   std::lock_guard<std::mutex> lock(a.mu);
   a.live[p] = c;
