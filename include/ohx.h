@@ -246,7 +246,8 @@ void ohx_mixed_stats(double* max_err, unsigned long long* screened,
     unsigned long long* reeval) OSH_PUBLIC;
 void ohx_mixed_stats_reset(void) OSH_PUBLIC;
 /* the online autotuner's choices so far, "kernel:2^class=variant" per line
- * (variant: lanes per item); returns the full length */
+ * (variant: the kernel's own code, e.g. lanes per item, items per thread,
+ * 0/1 for a plain or combined form); returns the full length */
 unsigned ohx_tuning_report(char* buf, unsigned cap) OSH_PUBLIC;
 /* pin the tuner of that name (as in the report) to one of its variants;
  * variant < 0 lets it tune again.  Every variant gives identical results. */
