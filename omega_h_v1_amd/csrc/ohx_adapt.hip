@@ -95,26 +95,26 @@ __global__ __launch_bounds__(BLOCK) void indset_round(u32 n, u32 const* __restri
   if (i < n) {
     u32 s = old[i];
     if (s == UNKNOWN) {
-      s = IN_SET;
+      // one walk over the neighbours: a neighbour in the set decides at
+      // once; otherwise a live neighbour with larger goodness (or equal
+      // goodness and a lower index) makes this one wait (goodness is read
+      // only until the first such neighbour)
       u32 f = offs[i], e = offs[i + 1];
-      for (u32 j = f; j < e; ++j)
-        if (old[adj[j]] == IN_SET) {
-          s = NOT_IN_SET;
+      double const myg = good[i];
+      bool wait = false, out = false;
+      for (u32 j = f; j < e; ++j) {
+        u32 o = adj[j];
+        u32 so = old[o];
+        if (so == IN_SET) {
+          out = true;
           break;
         }
-      if (s == IN_SET) {
-        double myg = good[i];
-        for (u32 j = f; j < e; ++j) {
-          u32 o = adj[j];
-          if (old[o] == NOT_IN_SET) continue;
-          double og = good[o];
-          if ((myg == og && o < i) || myg < og) {
-            s = UNKNOWN;
-            undecided = true;
-            break;
-          }
-        }
+        if (so == NOT_IN_SET || wait) continue;
+        double og = good[o];
+        if ((myg == og && o < i) || myg < og) wait = true;
       }
+      s = out ? (u32) NOT_IN_SET : (wait ? (u32) UNKNOWN : (u32) IN_SET);
+      undecided = !out && wait;
     }
     state[i] = s;
   }
