@@ -1209,9 +1209,10 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32 nsd = count(m, d - 1);
     fused[d - 1] = filled(nsd, 0);
     u32 vpe = d + 1, spe = h_down_deg[d][d - 1];
-    Buf<u32> will(n), gv(n), gd(n);
+    Buf<u32> will(n), gv(n), gd(n), same(n);
     {
-      // collapse_to_ent (collapses_to_ents.cpp:11-82)
+      // collapse_to_ent (collapses_to_ents.cpp:11-82); also marks the
+      // entities with no collapsing vertex, which survive unchanged
       u32 const* go = goff.get();
       u32 const* g = gvv.get();
       u32 const* fe = fused[d] ? fused[d].get() : nullptr;
@@ -1219,6 +1220,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       u32* wl = will.get();
       u32* gvp = gv.get();
       u32* gdp = gd.get();
+      u32* smp = same.get();
       parallel_for(n, OHX_LAMBDA(u32 i) {
         u32 const* v = vent + (size_t) i * vpe;
         u32 col = INVALID, dir = INVALID;
@@ -1244,22 +1246,17 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
         wl[i] = w;
         gvp[i] = gvv_;
         gdp[i] = gd_;
+        smp[i] = dir == INVALID ? 1u : 0u;
       });
     }
     Buf<u32> egoff(n + 1), esoff(n + 1);
     {
       u32 const* wl = will.get();
-      u32 const* go = goff.get();
+      u32 const* smp = same.get();
       exscan_fn(n, OHX_LAMBDA(u32 i) { return wl[i]; }, egoff.get());
-      // entities with no collapsing vertex survive unchanged
-      exscan_fn(n, OHX_LAMBDA(u32 i) {
-        for (u32 j = 0; j < vpe; ++j) {
-          u32 v = vent[(size_t) i * vpe + j];
-          if (go[v] != go[v + 1]) return 0u;
-        }
-        return 1u;
-      }, esoff.get());
+      exscan_fn(n, OHX_LAMBDA(u32 i) { return smp[i]; }, esoff.get());
     }
+    same.release();
     plan[d] = DimPlan{std::move(gv), std::move(gd), std::move(egoff), std::move(esoff)};
   }
   u32 tots[8];
