@@ -66,7 +66,7 @@ ADAPT_ARGS = (1.0 / 3.0, 0.3, 4, 50)
 # on this workload (mean over the dispatches of a 2-step run), scaled by the
 # same counters read for a copy kernel of known size (FETCH_SIZE x 2,
 # WRITE_SIZE x 1)
-TRAFFIC = {"scan": 2977972}
+TRAFFIC = {"scan": 2499499}
 
 
 def prepare(lib, on_device):
