@@ -3548,6 +3548,26 @@ double orc_metric_length(double const* a6, double const* b6, double const* x0, d
   return sqrt(metric_len2(a6, b6, e));
 }
 
+/* test access to the candidate-quality loops: refine_qualities on edges
+ * (refine_qualities.cpp:84-133) and coarsen_qualities
+ * (coarsen_qualities.cpp:81-114); cands / codes are updated in place as
+ * the passes update them, out gets the goodness slots (ne, or 2 ne) */
+void orc_refine_qualities(orc_mesh* m, unsigned* cands, double qual_floor,
+    int require_better, double* out)
+{
+  double* q = refine_qualities(m, 1, cands, qual_floor, require_better);
+  memcpy(out, q, sizeof(double) * mesh_count(m, 1));
+  free(q);
+}
+
+void orc_coarsen_qualities(orc_mesh* m, unsigned* codes, double qual_floor,
+    int require_better, double* out)
+{
+  double* q = coarsen_qualities(m, codes, qual_floor, require_better);
+  memcpy(out, q, sizeof(double) * 2 * mesh_count(m, 1));
+  free(q);
+}
+
 /* ------------------------------------------------ size-field producers
  * The warp loop's size field (test_warp.cpp:70-83): element gradients of a
  * vertex field, recovered to vertices by volume weighting, twice, then a

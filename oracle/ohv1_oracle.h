@@ -97,6 +97,12 @@ double orc_metric_length(double const* a6, double const* b6, double const* x0,
     double const* x1);
 void orc_metric_from_hessian(orc_mesh* m, char const* hessian, double eps,
     double hmin, double hmax, char const* out_name);
+/* the candidate-quality loops of refine (edges) and coarsen, for tests:
+ * cands / codes updated in place, goodness out (ne, or 2 ne values) */
+void orc_refine_qualities(orc_mesh* m, unsigned* cands, double qual_floor,
+    int require_better, double* out);
+void orc_coarsen_qualities(orc_mesh* m, unsigned* codes, double qual_floor,
+    int require_better, double* out);
 
 /* round limit of the independent set (default 100, as indset.cpp:33) */
 void orc_set_indset_max_rounds(unsigned n);

@@ -23,6 +23,8 @@ def orc():
     d.orc_tet_quality.argtypes = [P]
     d.orc_triangle_xy_quality.restype = C.c_double
     d.orc_triangle_xy_quality.argtypes = [P]
+    d.orc_triangle_quality.restype = C.c_double
+    d.orc_triangle_quality.argtypes = [P]
     return d
 
 
@@ -124,3 +126,309 @@ def test_metric_from_hessian_clamps_sizes():
         n = np.ones(3) / np.sqrt(3)
         mn = full(mt[k]) @ n
         assert np.allclose(mn, (n @ mn) * n, atol=1e-9 * np.abs(mt[k]).max())
+
+
+# ---------------------------------------------------------------------------
+# Independent numpy restatements (numpy's Cholesky, determinant and norms,
+# not the oracle's formulas), used to pin the parts of the extension that the
+# invariance tests above leave open: distinct vertex metrics (the element
+# metric is their mean), the 2D forms, the unclamped eigenvalue map and the
+# metric given to a split vertex.
+
+def np_tet_quality(p):
+    """v1's tet measure V / A_rms^(3/2), normalised so the regular tet is 1
+    (quality.hpp:68-89 as mathematics): volume from a determinant, face
+    areas from cross-product norms."""
+    p = np.asarray(p, dtype=float)
+    vol = np.linalg.det(np.stack([p[1] - p[0], p[2] - p[0], p[3] - p[0]])) / 6.0
+    faces = [(0, 1, 2), (1, 0, 3), (2, 1, 3), (0, 2, 3)]
+    a2 = [(np.linalg.norm(np.cross(p[b] - p[a], p[c] - p[a])) / 2.0) ** 2 for a, b, c in faces]
+    arms = np.sqrt(np.mean(a2))
+    reg = (1.0 / (6.0 * np.sqrt(2.0))) / (np.sqrt(3.0) / 4.0) ** 1.5
+    return vol / arms ** 1.5 / reg
+
+
+def np_tri_quality(p, signed):
+    """v1's triangle measure A / L_rms^2 / (sqrt(3)/4), signed by the z
+    component of the normal or unsigned (quality.hpp:34-66)."""
+    p = np.asarray(p, dtype=float)
+    n = np.cross(p[1] - p[0], p[2] - p[0])
+    area = (n[2] if signed else np.linalg.norm(n)) / 2.0
+    l2 = np.mean([np.sum((p[1] - p[0]) ** 2), np.sum((p[2] - p[1]) ** 2), np.sum((p[0] - p[2]) ** 2)])
+    return area / l2 / (np.sqrt(3.0) / 4.0)
+
+
+def np_metric_quality(x, m6s, dim, signed=True):
+    """The extension's definition: mean of the vertex metrics, its upper
+    Cholesky factor U (U^T U = M), v1's measure of the mapped points."""
+    mbar = np.mean([full(m) for m in m6s], axis=0)
+    u = np.linalg.cholesky(mbar).T
+    y = (np.asarray(x, dtype=float) - x[0]) @ u.T
+    return np_tet_quality(y) if dim == 3 else np_tri_quality(y, signed)
+
+
+def random_spd(rng, spread=2.0):
+    r, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return r @ np.diag(10.0 ** rng.uniform(-spread, spread, 3)) @ r.T
+
+
+def test_numpy_restatement_matches_v1_closed_forms():
+    """The numpy measures used as the pins below give v1's known answers."""
+    kuhn = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 1, 1]], dtype=float)
+    assert abs(np_tet_quality(kuhn) - 0.8408964152537141) < 1e-14
+    reg = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=float)
+    assert abs(abs(np_tet_quality(reg)) - 1.0) < 1e-14
+    assert abs(np_tri_quality([[0, 0, 0], [1, 0, 0], [1, 1, 0]], True) - 0.8660254037844387) < 1e-14
+    eq = [[0, 0, 0], [1, 0, 0], [0.5, np.sqrt(3) / 2, 0]]
+    assert abs(np_tri_quality(eq, False) - 1.0) < 1e-14
+    assert abs(np_tri_quality(eq[::-1], True) + 1.0) < 1e-14
+
+
+def test_metric_quality_distinct_vertex_metrics_3d(orc):
+    """Four different anisotropic vertex metrics: the oracle's quality equals
+    v1's measure in the mean metric.  A wrong average (a vertex left out or
+    weighted, one vertex's metric alone) or the lower factor in place of
+    the upper one fails this."""
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(400):
+        x = rng.normal(size=(4, 3))
+        if np.linalg.det(np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]])) < 0:
+            x[[1, 2]] = x[[2, 1]]
+        ms = np.array([sym6(random_spd(rng)) for _ in range(4)])
+        got = orc.orc_metric_quality(3, ptr(x), ptr(ms), 1)
+        want = np_metric_quality(x, ms, 3)
+        worst = max(worst, abs(got - want) / max(abs(want), 1e-3))
+    assert worst < 1e-11, worst
+
+
+def test_metric_quality_catches_single_vertex_metric(orc):
+    """The pin above is not vacuous: with distinct metrics the mean differs
+    from any single vertex's metric, and so does the quality."""
+    rng = np.random.default_rng(12)
+    x = rng.normal(size=(4, 3))
+    ms = np.array([sym6(random_spd(rng)) for _ in range(4)])
+    got = orc.orc_metric_quality(3, ptr(x), ptr(ms), 1)
+    for k in range(4):
+        alone = np_metric_quality(x, np.tile(ms[k], (4, 1)), 3)
+        assert abs(alone - got) > 1e-6
+
+
+def test_metric_quality_2d_reduces_to_v1(orc):
+    """dim = 2 with M = h^-2 I: the signed form equals v1's xy quality and
+    the unsigned form v1's triangle quality (up to rounding of the map)."""
+    rng = np.random.default_rng(13)
+    for _ in range(200):
+        x = np.zeros((3, 3))
+        x[:, :2] = rng.normal(size=(3, 2))
+        h = 10.0 ** rng.uniform(-3, 1)
+        m = np.tile(sym6(np.eye(3) / h ** 2), (3, 1))
+        qs = orc.orc_metric_quality(2, ptr(x), ptr(m), 1)
+        qu = orc.orc_metric_quality(2, ptr(x), ptr(m), 0)
+        assert abs(qs - orc.orc_triangle_xy_quality(ptr(x))) <= 1e-12
+        assert abs(qu - orc.orc_triangle_quality(ptr(x))) <= 1e-12
+        assert qu >= 0 and abs(abs(qs) - qu) <= 1e-12
+
+
+def test_metric_quality_2d_affine_invariance(orc):
+    """dim = 2: quality in M = A^T A (upper-left block; the z row is
+    immaterial to planar points) equals v1's xy quality of the points mapped
+    by A, for det A > 0, and the orientation flips with det A < 0."""
+    rng = np.random.default_rng(14)
+    for _ in range(200):
+        x = np.zeros((3, 3))
+        x[:, :2] = rng.normal(size=(3, 2))
+        a = rng.normal(size=(2, 2)) + 2 * np.eye(2)
+        if np.linalg.det(a) < 0:
+            a[0] = -a[0]
+        mm = np.eye(3) * rng.uniform(0.5, 5.0)
+        mm[:2, :2] = a.T @ a
+        m = np.tile(sym6(mm), (3, 1))
+        y = x.copy()
+        y[:, :2] = x[:, :2] @ a.T
+        qs = orc.orc_metric_quality(2, ptr(x), ptr(m), 1)
+        assert abs(qs - orc.orc_triangle_xy_quality(ptr(y))) <= 1e-10 * max(1.0, abs(qs))
+        yf = y.copy()
+        yf[:, 0] = -yf[:, 0]
+        assert qs * orc.orc_triangle_xy_quality(ptr(yf)) <= 0
+
+
+def test_metric_quality_distinct_vertex_metrics_2d(orc):
+    """dim = 2 with three different metrics: the mean of the three, signed
+    and unsigned forms, against the numpy restatement."""
+    rng = np.random.default_rng(15)
+    for _ in range(300):
+        x = np.zeros((3, 3))
+        x[:, :2] = rng.normal(size=(3, 2))
+        ms = []
+        for _k in range(3):
+            mm = np.eye(3)
+            mm[:2, :2] = random_spd(rng)[:2, :2]
+            ms.append(sym6(mm))
+        ms = np.array(ms)
+        for signed in (1, 0):
+            got = orc.orc_metric_quality(2, ptr(x), ptr(ms), signed)
+            want = np_metric_quality(x, ms, 2, bool(signed))
+            assert abs(got - want) <= 1e-11 * max(1.0, abs(want))
+
+
+def _hessian_mesh(orc_lib, hessians):
+    """A one-cube oracle mesh (8 vertices) carrying the given vertex
+    Hessians, mapped to a metric."""
+    x, c = ML.box(3, 1)
+    m = ML.Mesh(orc_lib, 3, x, c)
+    m.add_field(0, "hessian", np.asarray(hessians, dtype=float).ravel())
+    return m
+
+
+def test_metric_from_hessian_unclamped_map():
+    """Eigenvalues inside the clamp: M = R diag(|l| / eps) R^T exactly (to
+    rounding), for diagonal and rotated Hessians and for negative
+    eigenvalues (which give |l|).  Writing l * eps, l / eps without the
+    absolute value, or the inverse map fails this."""
+    orc_lib = ML.load("oracle")
+    rng = np.random.default_rng(21)
+    eps, hmin, hmax = 0.05, 0.01, 1.0
+    lo, hi = 1.0 / hmax ** 2, 1.0 / hmin ** 2
+    for trial in range(6):
+        hs, want = [], []
+        for v in range(8):
+            # |l| / eps inside [lo, hi] with a margin; signs mixed
+            mag = np.exp(rng.uniform(np.log(lo * 2), np.log(hi / 2), 3)) * eps
+            lam = mag * rng.choice([-1.0, 1.0], 3)
+            if trial == 0:
+                r = np.eye(3)
+            else:
+                r, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+            hs.append(sym6(r @ np.diag(lam) @ r.T))
+            want.append(r @ np.diag(np.abs(lam) / eps) @ r.T)
+        m = _hessian_mesh(orc_lib, hs)
+        m.metric_from_hessian("hessian", eps, hmin, hmax)
+        got = m.field(0, "metric").reshape(-1, 6)
+        for v in range(8):
+            w = want[v]
+            assert np.allclose(full(got[v]), w, rtol=0, atol=1e-12 * np.abs(w).max()), (trial, v)
+
+
+def test_metric_from_hessian_clamps_each_eigenvalue():
+    """Eigenvalues on both sides of the clamp in one tensor: each is clamped
+    on its own (1/hmax^2 below, 1/hmin^2 above), the eigenvectors kept."""
+    orc_lib = ML.load("oracle")
+    rng = np.random.default_rng(22)
+    eps, hmin, hmax = 0.1, 0.01, 0.5
+    lo, hi = 1.0 / hmax ** 2, 1.0 / hmin ** 2
+    hs, want = [], []
+    for v in range(8):
+        r, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        lam = np.array([lo * eps * 1e-3, -np.sqrt(lo * hi) * eps, -hi * eps * 1e3])
+        hs.append(sym6(r @ np.diag(lam) @ r.T))
+        want.append(r @ np.diag([lo, np.sqrt(lo * hi), hi]) @ r.T)
+    m = _hessian_mesh(orc_lib, hs)
+    m.metric_from_hessian("hessian", eps, hmin, hmax)
+    got = m.field(0, "metric").reshape(-1, 6)
+    for v in range(8):
+        assert np.allclose(full(got[v]), want[v], rtol=0, atol=1e-10 * hi)
+
+
+def _jittered_metric_mesh(orc_lib, n, seed):
+    """A 3D Kuhn box with jittered interior vertices and a different random
+    anisotropic metric at every vertex (full, classified)."""
+    rng = np.random.default_rng(seed)
+    x, c = ML.box(3, n)
+    inner = np.all((x > 1e-9) & (x < 1 - 1e-9), axis=1)
+    x = x.copy()
+    x[inner] += rng.uniform(-0.15, 0.15, size=(int(inner.sum()), 3)) / n
+    m = ML.Mesh(orc_lib, 3, x, c)
+    m.derive_model_full()
+    ms = np.array([sym6(random_spd(rng, 0.7) * n * n) for _ in range(len(x))])
+    m.add_field(0, "metric", ms.ravel())
+    return m, x, c, ms
+
+
+def test_refine_qualities_split_vertex_metric():
+    """The worst child quality of every edge split, with distinct vertex
+    metrics: the split vertex sits at the edge midpoint with the mean of the
+    edge's two metrics, each child keeps its parent's orientation, and the
+    goodness is the minimum over the children (refine_qualities.cpp:16-82
+    with the metric in place of the isotropic measure)."""
+    orc_lib = ML.load("oracle")
+    d = orc_lib.dll
+    d.orc_refine_qualities.argtypes = [P, P, C.c_double, C.c_int, P]
+    m, x, conn, ms = _jittered_metric_mesh(orc_lib, 2, 31)
+    ev = m.down(1, 0).reshape(-1, 2)
+    tv = m.down(3, 0).reshape(-1, 4)
+    ne = len(ev)
+    cands = np.ones(ne, dtype=np.uint32)
+    out = np.zeros(ne)
+    d.orc_refine_qualities(m.h, cands.ctypes.data_as(P), C.c_double(-1e300), 0, out.ctypes.data_as(P))
+    assert cands.all()
+    worst = 0.0
+    for i, (a, b) in enumerate(ev):
+        mid = (x[a] + x[b]) / 2
+        mmid = (ms[a] + ms[b]) / 2
+        qs = []
+        for t in np.nonzero(np.any(tv == a, axis=1) & np.any(tv == b, axis=1))[0]:
+            for end in (a, b):
+                px = x[tv[t]].copy()
+                pm = ms[tv[t]].copy()
+                k = int(np.nonzero(tv[t] == end)[0][0])
+                px[k] = mid
+                pm[k] = mmid
+                qs.append(np_metric_quality(px, pm, 3))
+        want = min(qs)
+        worst = max(worst, abs(out[i] - want) / want)
+    assert worst < 1e-11, worst
+
+
+def test_refined_vertex_metric_is_edge_mean():
+    """After a refine pass every new vertex's metric is the mean of its
+    edge's two metrics, bit for bit ((0 + a) + b) / 2 = (a + b) / 2 in IEEE
+    arithmetic), and the old vertices keep theirs."""
+    orc_lib = ML.load("oracle")
+    m, x, conn, ms = _jittered_metric_mesh(orc_lib, 2, 32)
+    ev = m.down(1, 0).reshape(-1, 2)
+    nv = m.count(0)
+    m.uniformly_refine()  # splits one independent set of edges
+    got = m.field(0, "metric").reshape(-1, 6)
+    xn = m.coords().reshape(-1, 3)
+    assert np.array_equal(got[:nv], ms)
+    assert len(got) > nv
+    # each new vertex is the midpoint of the edge it split (the jitter makes
+    # midpoints unique); new vertices come in edge order
+    mids = {tuple((x[a] + x[b]) / 2): i for i, (a, b) in enumerate(ev)}
+    split = [mids[tuple(p)] for p in xn[nv:]]
+    assert split == sorted(split)
+    want = (ms[ev[split, 0]] + ms[ev[split, 1]]) / 2
+    assert np.array_equal(got[nv:], want)
+
+
+def test_coarsen_qualities_distinct_vertex_metrics():
+    """Collapse qualities with distinct vertex metrics: for each edge and
+    direction, the minimum over the elements around the collapsing vertex
+    that do not hold the target, each re-measured with the collapsing
+    vertex (position and metric) replaced by the target's."""
+    orc_lib = ML.load("oracle")
+    d = orc_lib.dll
+    d.orc_coarsen_qualities.argtypes = [P, P, C.c_double, C.c_int, P]
+    m, x, conn, ms = _jittered_metric_mesh(orc_lib, 2, 33)
+    ev = m.down(1, 0).reshape(-1, 2)
+    tv = m.down(3, 0).reshape(-1, 4)
+    ne = len(ev)
+    codes = np.full(ne, 3, dtype=np.uint32)
+    out = np.zeros(2 * ne)
+    d.orc_coarsen_qualities(m.h, codes.ctypes.data_as(P), C.c_double(-1e300), 0, out.ctypes.data_as(P))
+    assert (codes == 3).all()
+    worst = 0.0
+    for i, pair in enumerate(ev):
+        for j in range(2):
+            col, gen = pair[j], pair[1 - j]
+            want = 1.0
+            for t in np.nonzero(np.any(tv == col, axis=1) & ~np.any(tv == gen, axis=1))[0]:
+                px = x[tv[t]].copy()
+                pm = ms[tv[t]].copy()
+                k = int(np.nonzero(tv[t] == col)[0][0])
+                px[k] = x[gen]
+                pm[k] = ms[gen]
+                want = min(want, np_metric_quality(px, pm, 3))
+            worst = max(worst, abs(out[2 * i + j] - want) / max(abs(want), 1e-3))
+    assert worst < 1e-10, worst
