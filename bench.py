@@ -1,41 +1,56 @@
 """Benchmark of the MI355X adapt hot path.
 
-Workload (one "step"): the 2D drop-in configuration -- a Kuhn square of
-256 x 256 quads split into 131072 triangles, warped by
-u = 0.1 sin(2 pi x) cos(2 pi y) in x and y through warp_to_limit(0.2), then
-one mesh_adapt with isotropic size h = 1/200 (size floor 1/3, good quality
-0.3, 4 sliver layers, at most 50 operations).  The warped input mesh is
-prepared once and stays in HBM; every step adapts a device-side copy of it.
+Default workload (c3): FP32-mixed vs FP64 elements/sec per adapt pass at
+equal min quality, on one GPU.
+  Mesh: 3D Kuhn box 64^3 cubes x 6 = 1,572,864 tets, generated on the GPU.
+  Field: per cycle, the metric from the analytic Hessian of
+  f = tanh(50 (x + y + z - 1.5)) (evaluated on the host in FP64, as the
+  reference evaluates analytic fields) through the FP64 eigendecomposition,
+  eps 0.01, sizes clamped to [0.002, 0.1].
+  Run: 8 cycles of mesh_adapt (size floor 1/3, good quality 0.3, 4 sliver
+  layers).  Every cycle runs to completion: the operation limit is set to
+  1000, above the most any cycle needs (the cycles take 70 to ~420
+  operations), so no cycle stops where the reference's mesh_adapt would
+  abort (adapt.cpp:36-45).  The independent sets of this workload need more
+  rounds than the reference's 100 (indset.cpp:33): its ties in goodness
+  form long chains; the round limit is raised to 2000 (an extension; the
+  rounds the workload needed are reported).
 
-Metric: elements/sec per adapt pass = (sum over the passes that changed the
-mesh of their input element count) / (wall time).  The wall time is the
-whole timed region (copy of the input mesh, every pass including the ones
-that changed nothing, the per-pass summaries), which makes the figure a
-lower bound of the per-pass definition.
+  A "step" is one mesh_adapt cycle.  One untimed survey run per precision
+  adapts the whole 8-cycle sequence from a fresh box with every kernel timed
+  (per-kernel table, dominant kernel) and keeps each cycle's input mesh on
+  the GPU: FP32-mixed and FP64 must give the same result, entity counts and
+  min quality after every cycle and the identical final mesh (SHA-256 of
+  every array).  Then, per precision, W untimed and K
+  timed steps adapt copies of the cycle inputs in order (cycle k mod 8; the
+  copies are made before the clock starts, so inputs are resident in HBM),
+  and each step's return value, pass count and entity counts are checked
+  against the survey's.  value = sum over the timed steps of the input
+  element counts of the passes that changed the mesh / wall time of the
+  timed region (FP32-mixed); fp64_value likewise for the FP64 path.
 
-Prints one JSON line (see the keys at the end of main()).
+  roofline: the dominant kernel of the FP32-mixed survey; in the mixed timed
+  region only that kernel carries start/stop events (in its own dispatch),
+  which give its average duration.
+  cpu_baseline: the reference's OpenMP loop build (compiled from its sources
+  by oracle/Makefile) on the host's cores, on the isotropic stand-in of this
+  workload (v1 cannot express a metric: adapt_size from v1's
+  size_from_hessian of the same Hessian, min 0.002, max 0.1), a bounded
+  sample of its first passes; the product's FP64 path runs the same
+  stand-in on the GPU for comparison and the two final meshes are compared.
 
 Other workloads (--workload), each printing one JSON line of its own:
-  c3  3D Kuhn box 64^3 x 6 = 1,572,864 tets; per cycle the metric comes from
-      the analytic Hessian of f = tanh(50 (x + y + z - 1.5)) through the FP64
-      eigendecomposition (eps 0.01, h in [0.002, 0.1]), then one mesh_adapt
-      (at most 50 operations, the reference default); 8 cycles.  A cycle that
-      reaches the operation limit ends there and the next cycle starts from
-      the mesh as it is (the reference would abort).  --precision picks the
-      FP64 path or the FP32-mixed one.
-  c4  256 independent meshes: cubes per axis n log-uniform in [8, 32], a
-      uniform random rotation and axis lengths log-uniform in [0.005, 0.1]
-      per mesh (numpy PCG64, seed 2024), each mesh the Kuhn box [0, L]^3 with
-      L = n (h1 h2 h3)^(1/3) and the constant metric R diag(h^-2) R^T; one
-      mesh_adapt per mesh (<= 50 ops).  The meshes are adapted concurrently
-      by worker threads on HIP streams of their own (ohx_adapt_many, 4 and 8
-      streams); the same meshes adapted one call at a time on the same GPU
-      are the comparison, and return values and sampled meshes are checked
-      identical between the two.
+  c2  2D Kuhn square 256 x 256 x 2 = 131,072 triangles, warped by
+      u = 0.1 sin(2 pi x) cos(2 pi y) through warp_to_limit(0.2), then one
+      mesh_adapt with h = 1/200 (the FP64 drop-in); a step is one adapt.
+  c4  256 independent meshes (numpy PCG64 seed 2024), n log-uniform in
+      [8, 32], random rotation, metric axis lengths log-uniform in
+      [0.005, 0.1]; one mesh_adapt each, one mesh at a time and
+      concurrently (ohx_adapt_many).
   c5  3D Kuhn box 160^3 x 6 = 24,576,000 tets, rotating anisotropic metric
-      (short axis 1e-3 in the xy plane at angle 2 pi z, 20:1), the first 8
-      passes of mesh_adapt: per-kernel GB/s at the real shape and the peak
-      device memory.
+      (short axis 1e-3 in the xy plane at angle 2 pi z, 20:1): the first
+      passes of the first cycle, then one swap pass, for per-kernel GB/s at
+      the real shape and the peak device memory.
   --calibrate-copy runs plain device-to-device copies of known size, for
   scaling the FETCH_SIZE / WRITE_SIZE counters of a profiler run.
 """
@@ -51,36 +66,41 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-import meshlib as ML  # noqa: E402
+from omega_h_v1_amd import seam as S  # noqa: E402
 from omega_h_v1_amd import workloads as W  # noqa: E402
 
 _LIBC = C.CDLL(None)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+# c3
+C3_N, C3_CYCLES = 64, 8
+C3_A, C3_S0, C3_EPS, C3_HMIN, C3_HMAX = 50.0, 1.5, 0.01, 0.002, 0.1
+C3_MAX_OPS = 1000
+C3_INDSET_ROUNDS = 2000
+ADAPT3 = (1.0 / 3.0, 0.3, 4)
+# isotropic stand-in of c3 for the CPU baseline: operations of the bounded
+# sample (the reference's adapt returns at the limit in the harness)
+C3_CPU_OPS = 7
+
+# c2
 N_CELLS = 256
 H = 1.0 / 200.0
 WARP_FLOOR = 0.2
 ADAPT_ARGS = (1.0 / 3.0, 0.3, 4, 50)
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# on this workload (mean over the dispatches of a 2-step run), scaled by the
-# same counters read for a copy kernel of known size (FETCH_SIZE x 2,
-# WRITE_SIZE x 1)
-TRAFFIC = {"scan": 2499499}
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+# scaled by the same counters read for a copy kernel of known size
+# (FETCH_SIZE x 2, WRITE_SIZE x 1); per workload and kernel
+TRAFFIC = {("c2", "scan"): 2499499}
 
 
-def prepare(lib, on_device):
-    if on_device:
-        m = ML.Mesh.box_on_device(lib, 2, N_CELLS)
-    else:
-        x, c = ML.box(2, N_CELLS)
-        m = ML.Mesh(lib, 2, x, c)
-    m.derive_model_full()
-    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), H))
-    xs = m.coords().reshape(-1, 3)
-    m.add_field(0, "warp", W.sine_warp_2d(xs).ravel())
-    m.warp_to_limit(WARP_FLOOR)
-    return m
+def _checker(which):
+    """The reference's own builds (compiled from its sources into oracle/_ref
+    by oracle/Makefile): the CPU baseline's libraries, never the product."""
+    path = os.path.join(ROOT, "oracle", "_ref", {"ref": "libohv1_ref_serial.so",
+                                                 "ref_omp": "libohv1_ref_omp.so"}[which])
+    return S.Lib(path, "ref_") if os.path.exists(path) else None
 
 
 def digest(m):
@@ -91,6 +111,17 @@ def digest(m):
     return h.hexdigest()
 
 
+def light_digest(m):
+    """counts, every dimension's vertex connectivity and the coordinates"""
+    h = hashlib.sha256()
+    h.update(np.asarray(m.counts(), dtype=np.int64).tobytes())
+    for d in range(1, m.dim + 1):
+        h.update(m.down(d, 0).tobytes())
+    h.update(m.coords().tobytes())
+    return h.hexdigest()
+
+
+# ------------------------------------------------------------ profiling API
 def profile_get(dll, name):
     f = dll.ohx_profile_get
     f.restype = C.c_int
@@ -98,11 +129,6 @@ def profile_get(dll, name):
     ms, n, units = C.c_double(), C.c_ulonglong(), C.c_double()
     f(name.encode(), C.byref(ms), C.byref(n), C.byref(units))
     return ms.value, n.value, units.value
-
-
-def _set_profile_api(dll):
-    dll.ohx_profile_only.argtypes = [C.c_char_p]
-    dll.ohx_profile_only.restype = None
 
 
 def profile_names(dll):
@@ -115,76 +141,554 @@ def profile_names(dll):
     return [s for s in buf.value.decode().split("\n") if s]
 
 
-def peak_bytes(dll):
-    f = dll.ohx_device_bytes_high_water
-    f.restype = C.c_ulonglong
-    return f()
+def tuning_report(dll):
+    f = dll.ohx_tuning_report
+    f.restype = C.c_uint
+    f.argtypes = [C.c_char_p, C.c_uint]
+    n = f(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    f(buf, n + 1)
+    return [x for x in buf.value.decode().split("\n") if x]
 
 
-def _time_cpu(lib, budget_s, product_digest, product_passes):
-    """Repeat the whole adapt on a CPU library (fresh warped input each time,
-    set-up untimed) for about budget_s seconds of mesh_adapt."""
-    total_el, total_pass_t, total_t, runs = 0.0, 0.0, 0.0, 0
-    parity = None
-    t_start = time.time()
-    while runs < 1 or (time.time() - t_start < budget_s and runs < 50):
-        mm = prepare(lib, False)
-        devnull = os.open(os.devnull, os.O_WRONLY)
-        saved = os.dup(1)
-        os.dup2(devnull, 1)  # the reference prints per-pass lines
-        try:
-            r, st = mm.adapt_timed(*ADAPT_ARGS)
-        finally:
-            _LIBC.fflush(None)  # what the C library buffered goes to /dev/null
-            os.dup2(saved, 1)
-            os.close(devnull)
-            os.close(saved)
-        total_el += st[1]
-        total_pass_t += st[2]
-        total_t += st[3]
-        if runs == 0:
-            parity = (digest(mm) == product_digest) and (int(st[0]) == product_passes)
-        runs += 1
-    return total_el, total_pass_t, total_t, runs, parity
+def setup_api(dll):
+    dll.ohx_profile_only.argtypes = [C.c_char_p]
+    dll.ohx_profile_only.restype = None
+    dll.ohx_set_precision.argtypes = [C.c_int]
+    dll.ohx_set_indset_max_rounds.argtypes = [C.c_uint]
+    dll.ohx_indset_rounds_max.argtypes = [C.c_int]
+    dll.ohx_indset_rounds_max.restype = C.c_uint
+    dll.ohx_mixed_stats.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
+    dll.ohx_device_bytes_high_water.restype = C.c_ulonglong
 
 
-def cpu_baseline(product_digest, product_passes):
-    """The reference itself, compiled from its sources, on this host's cores:
-    its OpenMP loop build with every thread OMP_NUM_THREADS allows (the
-    reported baseline) and its serial build (one core); the oracle stands in
-    when the reference was not built."""
+def kernel_table(dll):
+    tab = {name: profile_get(dll, name) for name in profile_names(dll)}
+    host_sync = tab.pop("host_sync", None)
+    host_malloc = tab.pop("host_malloc", None)
+    return tab, host_sync, host_malloc
+
+
+def per_kernel(tab):
+    return {name: {"ms": round(ms, 3), "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
+                   "GB/s": round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+            for name, (ms, n, nb) in sorted(tab.items(), key=lambda kv: -kv[1][0])}
+
+
+def roofline_of(dll, dom, workload):
+    if not dom:
+        return None
+    ms, n, nbytes = profile_get(dll, dom)
+    if not n or ms <= 0:
+        return None
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get((workload, dom)),
+            "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
+            "algorithmic_bytes_per_launch": round(nbytes / max(n, 1))}
+
+
+def mixed_stats(dll):
+    err, screened, reeval = C.c_double(), C.c_ulonglong(), C.c_ulonglong()
+    dll.ohx_mixed_stats(C.byref(err), C.byref(screened), C.byref(reeval))
+    return {"max_abs_q32_minus_q64": err.value, "decisions_in_fp32": screened.value,
+            "fp64_reevaluations": reeval.value}
+
+
+# -------------------------------------------------------------------- c3
+def c3_set_metric(m):
+    x = m.coords().reshape(-1, 3)
+    if m.tag(0, "metric", f64=True) is not None:
+        m.free_field(0, "metric")
+    m.add_field(0, "hessian", W.tanh_front_hessian(x, a=C3_A, s0=C3_S0, dim=3).ravel())
+    m.metric_from_hessian("hessian", C3_EPS, C3_HMIN, C3_HMAX)
+    m.free_field(0, "hessian")
+
+
+def c3_survey(lib, mixed):
+    """The whole 8-cycle sequence from a fresh box, every kernel timed; keeps
+    each cycle's input and hashes each cycle's output."""
+    dll = lib.dll
+    dll.ohx_set_precision(1 if mixed else 0)
+    dll.ohx_mixed_stats_reset()
+    dll.ohx_device_reset_high_water()
+    m = S.Mesh.box_on_device(lib, 3, C3_N)
+    m.derive_model_full()
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(None)
+    dll.ohx_profile_enable(1)
+    cycles, inputs = [], []
+    field_s = 0.0
+    for _ in range(C3_CYCLES):
+        tf = time.perf_counter()
+        c3_set_metric(m)
+        field_s += time.perf_counter() - tf
+        inputs.append(m.clone())
+        t0 = time.perf_counter()
+        r, st = m.adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], C3_MAX_OPS)
+        wall = time.perf_counter() - t0
+        cycles.append({"result": r, "passes": int(st[0]), "elements": float(st[1]), "pass_seconds": st[2],
+                       "wall_seconds": wall, "counts": m.counts(), "min_quality": m.min_quality()})
+    dll.ohx_synchronize()
+    dll.ohx_profile_enable(0)
+    tab, host_sync, host_malloc = kernel_table(dll)
+    dom = max(tab, key=lambda k: tab[k][0]) if tab else None
+    out = {
+        "cycles": cycles, "inputs": inputs, "kernels": per_kernel(tab), "dominant": dom,
+        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
+        "host_malloc": {"count": host_malloc[1], "ms": round(host_malloc[0], 3)} if host_malloc else None,
+        "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
+        "mixed": mixed_stats(dll) if mixed else None,
+        "host_field_seconds": field_s,
+        "final_min_quality": m.min_quality(),
+    }
+    # the survey's own rate over the complete 8-cycle run (events on every kernel)
+    el = sum(c["elements"] for c in cycles)
+    pt = sum(c["pass_seconds"] for c in cycles)
+    out["full_run"] = {"elements_over_passes": el, "passes": sum(c["passes"] for c in cycles),
+                       "pass_seconds": round(pt, 4), "per_pass_metric": el / pt if pt else None,
+                       "wall_seconds": round(sum(c["wall_seconds"] for c in cycles), 4)}
+    return out, m
+
+
+def c3_steps(lib, mixed, inputs, survey, warmup, steps, only):
+    """warmup untimed and steps timed mesh_adapt cycles on copies of the
+    survey's cycle inputs, in cycle order; returns the timed totals."""
+    dll = lib.dll
+    dll.ohx_set_precision(1 if mixed else 0)
+
+    def check(k, r, st, m):
+        c = survey["cycles"][k % C3_CYCLES]
+        if r != c["result"] or int(st[0]) != c["passes"] or m.counts() != c["counts"]:
+            raise RuntimeError("step on cycle %d differs from the survey run: %r vs %r" % (
+                k % C3_CYCLES, (r, int(st[0]), m.counts()), (c["result"], c["passes"], c["counts"])))
+
+    for k in range(warmup):
+        m = inputs[k % C3_CYCLES].clone()
+        r, st = m.adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], C3_MAX_OPS)
+        check(k, r, st, m)
+        del m
+    work = [inputs[(warmup + k) % C3_CYCLES].clone() for k in range(steps)]
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(only.encode() if only else None)
+    dll.ohx_profile_enable(1 if only else 0)
+    dll.ohx_synchronize()
+    elems = passes = pass_s = 0.0
+    res = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r, st = work[k].adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], C3_MAX_OPS)
+        res.append((r, st))
+        elems += st[1]
+        passes += st[0]
+        pass_s += st[2]
+    dll.ohx_synchronize()
+    wall = time.perf_counter() - t0
+    dll.ohx_profile_enable(0)
+    for k in range(steps):
+        check(warmup + k, res[k][0], res[k][1], work[k])
+    del work
+    return {"wall_seconds": wall, "elements_over_passes": elems, "passes": passes, "pass_seconds": pass_s,
+            "value": elems / wall, "per_pass_metric": elems / pass_s if pass_s else None}
+
+
+def c3_cpu_baseline(lib):
+    """The isotropic stand-in of c3 (v1 has only a scalar size): the 64^3 box,
+    adapt_size = v1 size_from_hessian of the same analytic Hessian (9
+    components, weight 1, min 0.002, max 0.1), mesh_adapt until C3_CPU_OPS
+    operations.  Timed on the reference's OpenMP build (all host threads) and
+    run by the product's FP64 path on the GPU; final meshes compared."""
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    legs = []
-    for which, kind, cores in (("ref_omp", "reference", threads), ("ref", "reference", 1)):
-        lib = ML.load(which)
-        if lib is not None:
-            legs.append((which, kind, cores, lib))
-    if not legs:
-        lib = ML.load("oracle")
-        if lib is None:
-            return None
-        legs.append(("oracle", "port", 1, lib))
+    ref = _checker("ref_omp")
+    if ref is None:
+        return None
+
+    def standin(l, on_device):
+        if on_device:
+            m = S.Mesh.box_on_device(l, 3, C3_N)
+        else:
+            x, c = S.box(3, C3_N)
+            m = S.Mesh(l, 3, x, c)
+        m.derive_model_full()
+        h6 = W.tanh_front_hessian(m.coords().reshape(-1, 3), a=C3_A, s0=C3_S0, dim=3)
+        h9 = h6[:, [0, 1, 2, 1, 3, 4, 2, 4, 5]]
+        m.add_field(0, "hessian9", h9.ravel())
+        m.size_from_hessian("hessian9", None, C3_HMIN, C3_HMAX)
+        m.free_field(0, "hessian9")
+        return m
+
+    t0 = time.perf_counter()
+    mc = standin(ref, False)
+    setup = time.perf_counter() - t0
+    print("c3 cpu baseline: stand-in set up in %.1f s" % setup, file=sys.stderr, flush=True)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+    os.dup2(devnull, 1)  # the reference prints per-pass lines
+    try:
+        r, st = mc.adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], C3_CPU_OPS)
+    finally:
+        _LIBC.fflush(None)
+        os.dup2(saved, 1)
+        os.close(devnull)
+        os.close(saved)
+    print("c3 cpu baseline: %d passes in %.1f s" % (int(st[0]), st[2]), file=sys.stderr, flush=True)
+    lib.dll.ohx_set_precision(0)
+    mg = standin(lib, True)
+    rg, sg = mg.adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], C3_CPU_OPS)
+    same = (r == rg) and light_digest(mc) == light_digest(mg)
+    return {
+        "value": st[1] / st[2] if st[2] else None,
+        "unit": "elements/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": "reference OpenMP loop build, %d threads: the first %d passes of mesh_adapt on the isotropic "
+                  "stand-in of this workload (64^3 box, adapt_size from v1 size_from_hessian of the same Hessian, "
+                  "min 0.002, max 0.1), %.0f input elements in %.1f s of passes (set-up %.1f s untimed); v1 "
+                  "cannot express the anisotropic metric" % (threads, int(st[0]), st[1], st[2], setup),
+        "gpu_fp64_same_standin": {"value": sg[1] / sg[2] if sg[2] else None, "passes": int(sg[0]),
+                                  "same_final_mesh_as_cpu_reference": bool(same)},
+    }
+
+
+def run_c3(args, lib):
+    dll = lib.dll
+    dll.ohx_set_indset_max_rounds(C3_INDSET_ROUNDS)
+    dll.ohx_indset_rounds_max(1)
+    surveys, finals = {}, {}
+    for prec, mixed in (("fp64", False), ("mixed", True)):
+        sv, m = c3_survey(lib, mixed)
+        surveys[prec] = sv
+        finals[prec] = m
+        print("c3 survey %s: %d passes, %.3g el/s per pass" % (
+            prec, sv["full_run"]["passes"], sv["full_run"]["per_pass_metric"]), file=sys.stderr, flush=True)
+    rounds_seen = int(dll.ohx_indset_rounds_max(1))
+    # decision identity: every cycle's counts and min quality, and the
+    # final mesh bit for bit
+    identical_cycles = all(
+        surveys["fp64"]["cycles"][k]["counts"] == surveys["mixed"]["cycles"][k]["counts"]
+        and surveys["fp64"]["cycles"][k]["min_quality"] == surveys["mixed"]["cycles"][k]["min_quality"]
+        for k in range(C3_CYCLES))
+    final_digest = {p: digest(finals[p]) for p in finals}
+    identical_final = final_digest["fp64"] == final_digest["mixed"]
+    # cycle inputs are the same for both precisions (identical decisions):
+    # the timed steps of both run on the FP64 survey's inputs
+    inputs = surveys["fp64"]["inputs"]
+    surveys["mixed"]["inputs"] = None
+    del finals
+    dom = surveys["mixed"]["dominant"]
+    fp = c3_steps(lib, False, inputs, surveys["fp64"], args.warmup, args.steps, None)
+    print("c3 timed fp64: %.3g el/s" % fp["value"], file=sys.stderr, flush=True)
+    dll.ohx_mixed_stats_reset()
+    mx = c3_steps(lib, True, inputs, surveys["mixed"], args.warmup, args.steps, dom)
+    print("c3 timed mixed: %.3g el/s" % mx["value"], file=sys.stderr, flush=True)
+    roof = roofline_of(dll, dom, "c3")
+    mstats = mixed_stats(dll)
+    dll.ohx_set_precision(0)
+    tuning = tuning_report(dll)
+    del inputs
+    surveys["fp64"]["inputs"] = None
+    cpu = None if args.no_cpu else c3_cpu_baseline(lib)
+    dll.ohx_set_indset_max_rounds(0)
+    svm, svf = surveys["mixed"], surveys["fp64"]
+    out = {
+        "metric": "elements/sec per adapt pass, FP32-mixed vs FP64 on 1 GPU, at equal min quality",
+        "value": round(mx["value"], 1),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mx["wall_seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "c3: 3D Kuhn box 64^3x6 = 1572864 tets, metric from the Hessian of tanh(50(x+y+z-1.5)) "
+                        "(FP64 eigendecomposition, eps 0.01, h in [0.002, 0.1]), 8 cycles of mesh_adapt; a step "
+                        "is one mesh_adapt cycle (cycle k mod 8 of the sequence, from its input mesh)",
+            "precision": "FP32 quality screen with FP64 re-evaluation within 1e-4 of each decision",
+            "max_ops": C3_MAX_OPS,
+            "ops_per_cycle": [c["passes"] for c in svf["cycles"]],
+            "cycles_at_op_limit": sum(1 for c in svf["cycles"] if c["result"] < 0),
+            "indset_max_rounds": C3_INDSET_ROUNDS,
+            "indset_rounds_needed_max": rounds_seen,
+            "indset_note": "the reference stops an independent set after 100 rounds (indset.cpp:33) and would "
+                           "abort this workload in its first cycle; the limit is raised here",
+            "timed_passes": {"fp64": fp["passes"], "mixed": mx["passes"]},
+            "per_pass_metric": {"fp64": fp["per_pass_metric"], "mixed": mx["per_pass_metric"]},
+        },
+        "fp64_value": round(fp["value"], 1),
+        "mixed_over_fp64": mx["value"] / fp["value"],
+        "decisions": {
+            "identical_final_mesh": bool(identical_final),
+            "identical_every_cycle_counts_and_min_quality": bool(identical_cycles),
+            "final_min_quality": {"fp64": svf["final_min_quality"], "mixed": svm["final_min_quality"]},
+            "equal_final_min_quality": svf["final_min_quality"] == svm["final_min_quality"],
+            "mixed_stats_timed": mstats,
+            "mixed_stats_survey": svm["mixed"],
+        },
+        "full_run": {"fp64": svf["full_run"], "mixed": svm["full_run"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": svm["kernels"],
+        "kernels_fp64": svf["kernels"],
+        "kernels_note": "survey runs (the complete 8-cycle sequence) with every kernel timed by dispatch-carried "
+                        "events; value and roofline come from the timed steps, where only the dominant kernel "
+                        "carries events",
+        "host_sync": svm["host_sync"],
+        "host_malloc": svm["host_malloc"],
+        "peak_device_bytes": max(svm["peak_device_bytes"], svf["peak_device_bytes"]),
+        "host_field_seconds": round(svm["host_field_seconds"], 4),
+        "autotune": tuning,
+    }
+    print(json.dumps(out))
+
+
+# -------------------------------------------------------------------- c2
+def c2_prepare(lib, on_device):
+    if on_device:
+        m = S.Mesh.box_on_device(lib, 2, N_CELLS)
+    else:
+        x, c = S.box(2, N_CELLS)
+        m = S.Mesh(lib, 2, x, c)
+    m.derive_model_full()
+    m.add_field(0, "adapt_size", W.uniform_size(m.count(0), H))
+    xs = m.coords().reshape(-1, 3)
+    m.add_field(0, "warp", W.sine_warp_2d(xs).ravel())
+    m.warp_to_limit(WARP_FLOOR)
+    return m
+
+
+def c2_cpu_baseline(product_digest, product_passes):
+    """The reference itself on this host's cores: its OpenMP loop build with
+    every thread OMP_NUM_THREADS allows (the reported baseline) and its
+    serial build (one core), ~10 s of repeated full adapts each."""
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     out = None
-    for which, kind, cores, lib in legs:
-        el, pt, tt, runs, parity = _time_cpu(lib, 10.0, product_digest, product_passes)
-        rec = {
-            "value": el / tt,
-            "unit": "elements/s",
-            "cores": cores,
-            "kind": kind,
-            "sample": "%d full mesh_adapt runs of the same workload (%.1f s), %s build, same metric "
-                      "definition (wall time of mesh_adapt)" % (
-                          runs, tt, {"ref_omp": "reference OpenMP loop", "ref": "reference serial loop",
-                                     "oracle": "oracle (serial C port)"}[which]),
-            "per_pass_metric": el / pt if pt else None,
-            "same_result_as_gpu": bool(parity),
-        }
+    for which, cores in (("ref_omp", threads), ("ref", 1)):
+        lib = _checker(which)
+        if lib is None:
+            continue
+        total_el = total_pass_t = total_t = 0.0
+        runs, parity = 0, None
+        t_start = time.time()
+        while runs < 1 or (time.time() - t_start < 10.0 and runs < 50):
+            mm = c2_prepare(lib, False)
+            devnull = os.open(os.devnull, os.O_WRONLY)
+            saved = os.dup(1)
+            os.dup2(devnull, 1)
+            try:
+                r, st = mm.adapt_timed(*ADAPT_ARGS)
+            finally:
+                _LIBC.fflush(None)
+                os.dup2(saved, 1)
+                os.close(devnull)
+                os.close(saved)
+            total_el += st[1]
+            total_pass_t += st[2]
+            total_t += st[3]
+            if runs == 0:
+                parity = (digest(mm) == product_digest) and (int(st[0]) == product_passes)
+            runs += 1
+        rec = {"value": total_el / total_t, "unit": "elements/s", "cores": cores, "kind": "reference",
+               "sample": "%d full mesh_adapt runs of the same workload (%.1f s), reference %s loop build" % (
+                   runs, total_t, "OpenMP" if which == "ref_omp" else "serial"),
+               "per_pass_metric": total_el / total_pass_t if total_pass_t else None,
+               "same_result_as_gpu": bool(parity)}
         if out is None:
             out = rec
         else:
             out.setdefault("other_builds", []).append(rec)
     return out
 
+
+def run_c2(args, lib):
+    dll = lib.dll
+    base = c2_prepare(lib, True)
+    t_in = base.count(2)
+    for _ in range(args.warmup):
+        m = base.clone()
+        m.adapt_timed(*ADAPT_ARGS)
+    dll.ohx_synchronize()
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(None)
+    dll.ohx_profile_enable(1)
+    ref_m = base.clone()
+    r_ref, st_ref = ref_m.adapt_timed(*ADAPT_ARGS)
+    dll.ohx_synchronize()
+    dll.ohx_profile_enable(0)
+    tab, host_sync, host_malloc = kernel_table(dll)
+    dom = max(tab, key=lambda k: tab[k][0]) if tab else None
+    ref_digest = digest(ref_m)
+    inputs = [base.clone() for _ in range(args.steps)]
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(dom.encode() if dom else None)
+    dll.ohx_profile_enable(1)
+    dll.ohx_synchronize()
+    elems = passes = pass_time = 0.0
+    res = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r, st = inputs[k].adapt_timed(*ADAPT_ARGS)
+        res.append(r)
+        elems += st[1]
+        passes += st[0]
+        pass_time += st[2]
+    dll.ohx_synchronize()
+    wall = time.perf_counter() - t0
+    dll.ohx_profile_enable(0)
+    dll.ohx_profile_only(None)
+    if any(r != r_ref for r in res) or digest(inputs[-1]) != ref_digest:
+        raise RuntimeError("c2: a timed step differs from the survey step")
+    roof = roofline_of(dll, dom, "c2")
+    cpu = None if args.no_cpu else c2_cpu_baseline(ref_digest, int(st_ref[0]))
+    out = {
+        "metric": "elements/sec per adapt pass (FP64 drop-in, 1 GPU)",
+        "value": round(elems / wall, 1), "unit": "elements/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "c2: 2D Kuhn square 256x256x2 = %d triangles, sine warp 0.1 via warp_to_limit(0.2), "
+                               "mesh_adapt h=1/200 (FP64 drop-in); a step is one mesh_adapt" % t_in,
+                   "passes_per_step": passes / args.steps, "elements_per_step": elems / args.steps,
+                   "per_pass_metric": round(elems / pass_time, 1) if pass_time else None},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": per_kernel(tab),
+        "kernels_note": "one untimed step with every kernel timed by dispatch-carried events",
+        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
+        "peak_device_bytes": int(dll.ohx_device_bytes_high_water()), "autotune": tuning_report(dll),
+    }
+    print(json.dumps(out))
+
+
+# -------------------------------------------------------------------- c5
+C5_N = 160
+C5_OPS = 40
+
+
+def run_c5(args, lib):
+    """The first C5_OPS passes of the first cycle (the cycle itself needs
+    thousands: each coarsen pass removes ~0.1 % of 36 M tets), then one swap
+    pass on the result: per-kernel GB/s at the real shape, peak memory."""
+    dll = lib.dll
+    mixed = args.precision == "mixed"
+    dll.ohx_set_precision(1 if mixed else 0)
+    dll.ohx_set_indset_max_rounds(C3_INDSET_ROUNDS)
+    dll.ohx_device_reset_high_water()
+    t0 = time.perf_counter()
+    m = S.Mesh.box_on_device(lib, 3, C5_N)
+    m.derive_model_full()
+    dll.ohx_synchronize()
+    setup = time.perf_counter() - t0
+    t_in0 = m.count(3)
+    x = m.coords().reshape(-1, 3)
+    m.add_field(0, "metric", W.rotating_metric(x, 1e-3, 20.0, dim=3).ravel())
+    dll.ohx_profile_reset()
+    dll.ohx_profile_only(None)
+    dll.ohx_profile_enable(1)
+    r, st = m.adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], C5_OPS)
+    counts_after = m.counts()
+    ts = time.perf_counter()
+    swapped = m.swap_slivers(0.3, 4)
+    dll.ohx_synchronize()
+    swap_s = time.perf_counter() - ts
+    dll.ohx_profile_enable(0)
+    tab, host_sync, host_malloc = kernel_table(dll)
+    dom = max(tab, key=lambda k: tab[k][0]) if tab else None
+    out = {
+        "metric": "elements/sec per adapt pass (%s, 1 GPU)" % ("FP32-mixed" if mixed else "FP64"),
+        "value": round(st[1] / st[2], 1) if st[2] else None, "unit": "elements/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": "c5: 3D Kuhn box 160^3x6 = %d tets, rotating metric 1e-3 x 20:1; the first %d "
+                               "operations of cycle 1 of mesh_adapt, then one swap pass" % (t_in0, C5_OPS),
+                   "passes": int(st[0]), "elements_over_passes": st[1], "pass_seconds": round(st[2], 4),
+                   "adapt_result": r, "counts_after_passes": counts_after, "swap_changed_mesh": int(swapped),
+                   "swap_seconds": round(swap_s, 4), "final_counts": m.counts(), "setup_seconds": round(setup, 4)},
+        "roofline": roofline_of(dll, dom, "c5"), "kernels": per_kernel(tab),
+        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
+        "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
+        "mixed": mixed_stats(dll) if mixed else None, "autotune": tuning_report(dll),
+    }
+    dll.ohx_set_precision(0)
+    dll.ohx_set_indset_max_rounds(0)
+    print(json.dumps(out))
+
+
+# -------------------------------------------------------------------- c4
+def run_c4(args, lib):
+    """256 meshes one mesh_adapt call at a time, then concurrently
+    (ohx_adapt_many: worker threads, one HIP stream each)."""
+    dll = lib.dll
+    mixed = args.precision == "mixed"
+    specs = W.batch_specs(256, seed=2024)
+
+    def part(s):
+        x, c = S.box(3, s["n"])
+        m = S.Mesh(lib, 3, x * s["side"], c)
+        m.derive_model_full()
+        m.add_field(0, "metric", W.constant_metric(m.count(0), s["rot"], s["lengths"]).ravel())
+        return m
+
+    t0 = time.perf_counter()
+    parts = [part(s) for s in specs]
+    t_in = sum(p.count(3) for p in parts)
+    dll.ohx_synchronize()
+    t_setup = time.perf_counter() - t0
+    # one at a time, FP64 (the drop-in) and in the chosen precision
+    seq = {}
+    singles = None
+    for prec in ("fp64", "mixed") if mixed else ("fp64",):
+        dll.ohx_set_precision(1 if prec == "mixed" else 0)
+        clones = [p.clone() for p in parts]
+        dll.ohx_synchronize()
+        el = 0.0
+        res = []
+        t0 = time.perf_counter()
+        for m in clones:
+            r, st = m.adapt_timed(ADAPT3[0], ADAPT3[1], ADAPT3[2], 50)
+            res.append(r)
+            el += st[1]
+        dll.ohx_synchronize()
+        wall = time.perf_counter() - t0
+        seq[prec] = {"wall_seconds": round(wall, 4), "wall_metric": el / wall, "results": res}
+        if singles is None:
+            singles = clones
+    sample = list(range(0, 256, 16))
+    ref_digests = {c: digest(singles[c]) for c in sample}
+    dll.ohx_set_precision(1 if mixed else 0)
+    runs = {}
+    for nthreads in (4, 8):
+        clones = [p.clone() for p in parts]
+        dll.ohx_synchronize()
+        t0 = time.perf_counter()
+        limited, res, st = S.Mesh.adapt_many(lib, clones, ADAPT3[0], ADAPT3[1], ADAPT3[2], 50, nthreads)
+        wall = time.perf_counter() - t0
+        runs[nthreads] = {
+            "wall_seconds": round(wall, 4), "wall_metric": st[1] / wall, "elements_over_passes": float(st[1]),
+            "components_at_op_limit": int(limited),
+            "identical_return_values": bool([int(v) for v in res] == [int(v) for v in seq["fp64"]["results"]]),
+            "identical_sampled_meshes_to_fp64_one_at_a_time": bool(
+                all(digest(clones[c]) == ref_digests[c] for c in sample)),
+        }
+        del clones
+    best = max(runs, key=lambda k: runs[k]["wall_metric"])
+    dll.ohx_set_precision(0)
+    value = runs[best]["wall_metric"]
+    for p in seq:
+        seq[p].pop("results")
+    out = {
+        "metric": "elements/sec per adapt pass (256 meshes at once, %s, 1 GPU)" % ("FP32-mixed" if mixed else "FP64"),
+        "value": round(value, 1), "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32+f64" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": "c4: 256 meshes (seed 2024), n log-uniform in [8, 32], %d tets in all, random "
+                               "rotation, metric axis lengths log-uniform in [0.005, 0.1]; one mesh_adapt each "
+                               "(<= 50 ops)" % t_in,
+                   "setup_seconds": round(t_setup, 3), "streams": best,
+                   "value_is": "sum of the meshes' pass input elements / wall time of the concurrent call"},
+        "concurrent": runs, "one_at_a_time": seq,
+        "concurrent_over_fp64_one_at_a_time": value / seq["fp64"]["wall_metric"],
+        "identity_sample": sample, "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
+    }
+    print(json.dumps(out))
 
 
 COPY_BYTES = 1 << 28  # 256 MiB
@@ -207,378 +711,27 @@ def calibrate_copy():
     print(json.dumps({"calibrate_copy_bytes": COPY_BYTES, "copies": 5}))
 
 
-def tuning_report(dll):
-    f = dll.ohx_tuning_report
-    f.restype = C.c_uint
-    f.argtypes = [C.c_char_p, C.c_uint]
-    n = f(None, 0)
-    buf = C.create_string_buffer(n + 1)
-    f(buf, n + 1)
-    return [x for x in buf.value.decode().split("\n") if x]
-
-
-def _profile_table(dll):
-    tab = {name: profile_get(dll, name) for name in profile_names(dll)}
-    host_sync = tab.pop("host_sync", None)
-    tab.pop("host_malloc", None)
-    return tab, host_sync
-
-
-def _host_malloc(dll):
-    """count and host ms of the arena's hipMalloc calls while profiling"""
-    names = profile_names(dll)
-    if "host_malloc" not in names:
-        return {"count": 0, "ms": 0.0}
-    ms, n, _ = profile_get(dll, "host_malloc")
-    return {"count": n, "ms": round(ms, 3)}
-
-
-def _roofline(tab):
-    if not tab:
-        return None, {}
-    dom = max(tab, key=lambda k: tab[k][0])
-    ms, n, nbytes = tab[dom]
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get(dom),
-            "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
-            "algorithmic_bytes_per_launch": round(nbytes / max(n, 1))}
-    per = {name: {"ms": round(ms, 3), "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
-                  "GB/s": round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-           for name, (ms, n, nb) in sorted(tab.items(), key=lambda kv: -kv[1][0])}
-    return roof, per
-
-
-def _large_run(lib, workload, mixed, only=None):
-    """One run of c3 or c5 from a fresh mesh.  only=None: every kernel timed
-    by dispatch-carried events (the survey); only=name: just that kernel
-    carries events (the measured run)."""
-    dll = lib.dll
-    dll.ohx_set_precision(1 if mixed else 0)
-    dll.ohx_mixed_stats_reset()
-    dll.ohx_set_indset_max_rounds(2000)
-    if workload == "c3":
-        n, cycles, max_ops = 64, 8, 50
-    else:
-        n, cycles, max_ops = 160, 1, 8
-    dll.ohx_device_reset_high_water()
-    t0 = time.perf_counter()
-    m = ML.Mesh.box_on_device(lib, 3, n)
-    m.derive_model_full()
-    dll.ohx_synchronize()
-    t_setup = time.perf_counter() - t0
-    t_in0 = m.count(3)
-    dll.ohx_profile_reset()
-    dll.ohx_profile_only(only.encode() if only else None)
-    dll.ohx_profile_enable(1)
-    elems = passes = pass_time = adapt_time = field_time = 0.0
-    limited = 0
-    for _ in range(cycles):
-        tf = time.perf_counter()
-        x = m.coords().reshape(-1, 3)
-        if m.tag(0, "metric", f64=True) is not None:
-            m.free_field(0, "metric")
-        if workload == "c3":
-            m.add_field(0, "hessian", W.tanh_front_hessian(x, a=50.0, s0=1.5, dim=3).ravel())
-            m.metric_from_hessian("hessian", 0.01, 0.002, 0.1)
-            m.free_field(0, "hessian")
-        else:
-            m.add_field(0, "metric", W.rotating_metric(x, 1e-3, 20.0, dim=3).ravel())
-        field_time += time.perf_counter() - tf
-        r, st = m.adapt_timed(1.0 / 3.0, 0.3, 4, max_ops)
-        limited += int(r < 0)
-        passes += st[0]
-        elems += st[1]
-        pass_time += st[2]
-        adapt_time += st[3]
-    dll.ohx_synchronize()
-    dll.ohx_profile_enable(0)
-    dll.ohx_profile_only(None)
-    host_malloc = _host_malloc(dll)
-    tab, host_sync = _profile_table(dll)
-    roof, per = _roofline(tab)
-    err, screened, reeval = C.c_double(), C.c_ulonglong(), C.c_ulonglong()
-    dll.ohx_mixed_stats(C.byref(err), C.byref(screened), C.byref(reeval))
-    dll.ohx_set_precision(0)
-    return {
-        "value": elems / pass_time if pass_time else None,
-        "t_in0": t_in0, "cycles": cycles, "cycles_at_op_limit": limited, "passes": passes,
-        "elements_over_passes": elems, "pass_seconds": round(pass_time, 4), "adapt_seconds": round(adapt_time, 4),
-        "host_field_seconds": round(field_time, 4), "setup_seconds": round(t_setup, 4),
-        "final_counts": m.counts(), "final_min_quality": m.min_quality(),
-        "roofline": roof, "kernels": per,
-        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
-        "host_malloc": host_malloc,
-        "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
-        "digest": digest(m) if workload == "c3" else None,
-        "mixed": {"max_abs_q32_minus_q64": err.value, "decisions_in_fp32": screened.value,
-                  "fp64_reevaluations": reeval.value} if mixed else None,
-    }
-
-
-def _survey_then_measure(lib, workload, mixed):
-    """A survey run with every kernel timed (it also settles the tuner),
-    then the measured run from a fresh mesh, where only the survey's
-    dominant kernel carries events: its value, the survey's kernel table."""
-    sv = _large_run(lib, workload, mixed)
-    dom = sv["roofline"]["kernel"] if sv["roofline"] else None
-    r = _large_run(lib, workload, mixed, only=dom)
-    r["kernels"] = sv["kernels"]
-    r["survey_value"] = sv["value"]
-    return r
-
-
-def run_large(args, lib):
-    """c3: the FP64 path and the FP32-mixed path from the same input, with
-    their final meshes compared; c5: one run at the real shape."""
-    dll = lib.dll
-    dll.ohx_set_precision.argtypes = [C.c_int]
-    dll.ohx_mixed_stats.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
-    dll.ohx_device_bytes_high_water.restype = C.c_ulonglong
-    desc = {"c3": "3D Kuhn box 64^3x6 = %d tets, tanh(50(x+y+z-1.5)) Hessian metric (eps 0.01, h in [0.002, 0.1]), "
-                  "8 cycles of mesh_adapt (<=50 ops each)",
-            "c5": "3D Kuhn box 160^3x6 = %d tets, rotating metric 1e-3 x 20:1, first 8 passes of mesh_adapt"}
-    if args.workload == "c3":
-        runs = {"fp64": _survey_then_measure(lib, "c3", False), "mixed": _survey_then_measure(lib, "c3", True)}
-        main_run = runs["mixed"] if args.precision == "mixed" else runs["fp64"]
-        cmp = {"fp64_value": runs["fp64"]["value"], "mixed_value": runs["mixed"]["value"],
-               "mixed_over_fp64": runs["mixed"]["value"] / runs["fp64"]["value"],
-               "identical_final_mesh": runs["fp64"]["digest"] == runs["mixed"]["digest"],
-               "equal_final_min_quality": runs["fp64"]["final_min_quality"] == runs["mixed"]["final_min_quality"],
-               "mixed_stats": runs["mixed"]["mixed"]}
-    else:
-        main_run = _survey_then_measure(lib, "c5", args.precision == "mixed")
-        cmp = None
-    r = main_run
-    out = {
-        "metric": "elements/sec per adapt pass (%s, 1 GPU)" % ("FP32-mixed" if args.precision == "mixed" else "FP64"),
-        "value": round(r["value"], 1) if r["value"] else None,
-        "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
-        "config": dict({"workload": desc[args.workload] % r["t_in0"], "precision": args.precision},
-                       **{k: r[k] for k in ("cycles", "cycles_at_op_limit", "passes", "elements_over_passes",
-                                            "pass_seconds", "adapt_seconds", "host_field_seconds", "setup_seconds",
-                                            "final_counts", "final_min_quality")}),
-        "fp64_vs_mixed": cmp,
-        "roofline": r["roofline"],
-        "kernels": r["kernels"],
-        "kernels_note": "a survey run with every kernel timed by dispatch-carried events; value, roofline and "
-                        "host_sync come from the following measured run, where only the dominant kernel carries "
-                        "events (survey_value: the survey run's own rate)",
-        "survey_value": round(r["survey_value"], 1) if r["survey_value"] else None,
-        "host_sync": r["host_sync"],
-        "host_malloc": r["host_malloc"],
-        "peak_device_bytes": r["peak_device_bytes"],
-        "autotune": tuning_report(dll),
-    }
-    print(json.dumps(out))
-
-
-def run_batch(args, lib):
-    """c4: 256 meshes adapted concurrently (worker threads, one HIP stream
-    each) against the same meshes one mesh_adapt call at a time."""
-    dll = lib.dll
-    dll.ohx_set_precision.argtypes = [C.c_int]
-    dll.ohx_device_bytes_high_water.restype = C.c_ulonglong
-    dll.ohx_set_precision(1 if args.precision == "mixed" else 0)
-    specs = W.batch_specs(256, seed=2024)
-
-    def part(s):
-        x, c = ML.box(3, s["n"])
-        m = ML.Mesh(lib, 3, x * s["side"], c)
-        m.derive_model_full()
-        m.add_field(0, "metric", W.constant_metric(m.count(0), s["rot"], s["lengths"]).ravel())
-        return m
-
-    t0 = time.perf_counter()
-    parts = [part(s) for s in specs]
-    t_in = sum(p.count(3) for p in parts)
-    dll.ohx_synchronize()
-    t_setup = time.perf_counter() - t0
-    print("c4: %d meshes, %d tets, set up in %.2f s" % (len(parts), t_in, t_setup), file=sys.stderr, flush=True)
-    # one at a time (clones, so every run starts from the same meshes)
-    seq_el = seq_pass = 0.0
-    seq_res = []
-    singles = []
-    t0 = time.perf_counter()
-    for p in parts:
-        m = p.clone()
-        r, st = m.adapt_timed(1.0 / 3.0, 0.3, 4, 50)
-        seq_res.append(r)
-        seq_el += st[1]
-        seq_pass += st[2]
-        singles.append(m)
-    dll.ohx_synchronize()
-    seq_wall = time.perf_counter() - t0
-    print("c4: one at a time in %.2f s" % seq_wall, file=sys.stderr, flush=True)
-    runs = {}
-    sample = list(range(0, 256, 16))
-    dll.ohx_device_reset_high_water()
-    for nthreads in (4, 8):
-        clones = [p.clone() for p in parts]
-        dll.ohx_synchronize()
-        t0 = time.perf_counter()
-        limited, res, st = ML.Mesh.adapt_many(lib, clones, 1.0 / 3.0, 0.3, 4, 50, nthreads)
-        wall = time.perf_counter() - t0
-        print("c4: %d streams in %.2f s" % (nthreads, wall), file=sys.stderr, flush=True)
-        runs[nthreads] = {
-            "wall_seconds": round(wall, 4), "wall_metric": st[1] / wall, "elements_over_passes": float(st[1]),
-            "passes": float(st[0]), "components_at_op_limit": int(limited),
-            "identical_return_values": bool([int(v) for v in res] == [int(v) for v in seq_res]),
-            "identical_sampled_meshes": bool(all(digest(clones[c]) == digest(singles[c]) for c in sample)),
-        }
-        del clones
-    best = max(runs, key=lambda k: runs[k]["wall_metric"])
-    dll.ohx_set_precision(0)
-    value = runs[best]["wall_metric"]
-    out = {
-        "metric": "elements/sec per adapt pass (256 meshes at once, %s, 1 GPU)" % (
-            "FP32-mixed" if args.precision == "mixed" else "FP64"),
-        "value": round(value, 1),
-        "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
-        "config": {"workload": "256 meshes (seed 2024), n log-uniform in [8, 32], %d tets in all, random rotation, "
-                               "metric axis lengths log-uniform in [0.005, 0.1]; one mesh_adapt each" % t_in,
-                   "precision": args.precision, "setup_seconds": round(t_setup, 3), "streams": best,
-                   "value_is": "sum of the meshes' pass input elements / wall time of the concurrent call"},
-        "concurrent": runs,
-        "one_at_a_time": {"wall_seconds": round(seq_wall, 4), "wall_metric": seq_el / seq_wall,
-                          "per_pass_metric": seq_el / seq_pass if seq_pass else None,
-                          "elements_over_passes": seq_el},
-        "concurrent_over_one_at_a_time": value / (seq_el / seq_wall),
-        "identity_sample": sample,
-        "peak_device_bytes": int(dll.ohx_device_bytes_high_water()),
-    }
-    print(json.dumps(out))
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c2")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--workload", choices=["c2", "c3", "c4", "c5"], default="c3")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="c4/c5 only (c3 always runs both)")
     ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the whole mesh lives in one GPU's HBM); "
-              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU only (the whole mesh lives in one GPU's HBM; no run here "
+              "uses a second GPU); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-
     if args.calibrate_copy:
         calibrate_copy()
         return
-    lib = ML.load("product")
-    if lib is None:
-        sys.exit("bench.py: product library not built (run __graft_entry__.build())")
-    dll = lib.dll
-    _set_profile_api(dll)
-    if args.workload == "c4":
-        run_batch(args, lib)
-        return
-    if args.workload != "c2":
-        run_large(args, lib)
-        return
-    base = prepare(lib, True)
-    t_in = base.count(2)
-
-    def step():
-        m = base.clone()
-        r, st = m.adapt_timed(*ADAPT_ARGS)
-        return m, r, st
-
-    for _ in range(args.warmup):
-        step()
-    dll.ohx_synchronize()
-
-    # one untimed survey step with every kernel timed: the per-kernel table
-    # and the dominant kernel (most total time)
-    dll.ohx_profile_reset()
-    dll.ohx_profile_only(None)
-    dll.ohx_profile_enable(1)
-    step()
-    dll.ohx_synchronize()
-    dll.ohx_profile_enable(0)
-    survey = {name: profile_get(dll, name) for name in profile_names(dll)}
-    host_sync = survey.pop("host_sync", None)
-    host_malloc = _host_malloc(dll)
-    survey.pop("host_malloc", None)
-    dom = max(survey, key=lambda k: survey[k][0]) if survey else None
-    per_kernel = {
-        name: {"ms": round(ms, 3), "launches": n, "avg_us": round(1e3 * ms / max(n, 1), 2),
-               "GB/s": round(nb / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-        for name, (ms, n, nb) in sorted(survey.items(), key=lambda kv: -kv[1][0])}
-
-    # the timed region; the dominant kernel's launches carry start/stop
-    # events in their own dispatches, which give its average duration
-    dll.ohx_profile_reset()
-    dll.ohx_profile_only(dom.encode() if dom else None)
-    dll.ohx_profile_enable(1)
-    elems = 0.0
-    passes = 0
-    pass_time = 0.0
-    # the inputs (one copy of the warped mesh per step) are made before the
-    # clock starts, so the timed region is the adapt runs alone, as in the
-    # CPU baseline, which times mesh_adapt only
-    inputs = [base.clone() for _ in range(args.steps)]
-    dll.ohx_synchronize()
-    t0 = time.perf_counter()
-    last = None
-    for k in range(args.steps):
-        last, inputs[k] = inputs[k], None
-        r, st = last.adapt_timed(*ADAPT_ARGS)
-        elems += st[1]
-        passes += int(st[0])
-        pass_time += st[2]
-    dll.ohx_synchronize()
-    t1 = time.perf_counter()
-    dll.ohx_profile_enable(0)
-    dll.ohx_profile_only(None)
-    wall = t1 - t0
-    value = elems / wall
-
-    roofline = None
-    if dom:
-        ms, n, nbytes = profile_get(dll, dom)
-        achieved = nbytes / (ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": TRAFFIC.get(dom),
-                    "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
-                    "algorithmic_bytes_per_launch": round(nbytes / max(n, 1))}
-
-    prod_digest = digest(last)
-    cpu = None if args.no_cpu else cpu_baseline(prod_digest, passes // args.steps)
-
-    out = {
-        "metric": "elements/sec per adapt pass (FP64 drop-in, 1 GPU)",
-        "value": round(value, 1),
-        "unit": "elements/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "2D Kuhn square 256x256x2 = %d triangles, sine warp 0.1 via warp_to_limit(0.2), "
-                               "mesh_adapt h=1/200 (FP64 drop-in)" % t_in,
-                   "passes_per_step": passes / args.steps,
-                   "elements_per_step": elems / args.steps,
-                   "per_pass_metric": round(elems / pass_time, 1) if pass_time else None},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": per_kernel,
-        "kernels_note": "one untimed step with every kernel timed by dispatch-carried events",
-        "host_sync": {"count": host_sync[1], "ms": round(host_sync[0], 3)} if host_sync else None,
-        "host_malloc": host_malloc,
-        "peak_device_bytes": int(peak_bytes(dll)),
-        "autotune": tuning_report(dll),
-    }
-    print(json.dumps(out))
+    lib = S.product()
+    setup_api(lib.dll)
+    {"c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args, lib)
 
 
 if __name__ == "__main__":

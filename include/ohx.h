@@ -235,6 +235,10 @@ void ohx_set_verbose(int on) OSH_PUBLIC;
  * whose equal-goodness candidates form long chains (each round settles at
  * least the lowest-numbered candidate of a tie chain) */
 void ohx_set_indset_max_rounds(unsigned n) OSH_PUBLIC;
+/* the most rounds any independent set needed since the last reset, among
+ * the sets that took more than one batch of 8 rounds (0 if none did);
+ * reset != 0 clears it after reading */
+unsigned ohx_indset_rounds_max(int reset) OSH_PUBLIC;
 /* 0: the FP64 path (default, the reference's arithmetic); 1: FP32-mixed --
  * qualities and metric lengths are screened in FP32 and every value within
  * a margin of a decision (a threshold or a competing value) is re-evaluated

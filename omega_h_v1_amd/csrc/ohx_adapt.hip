@@ -1,6 +1,7 @@
 // Refine, coarsen and swap passes, the independent set, the adapt driver
 // and warp_to_limit, on a device-resident mesh.  Every per-entity loop is
 // a gfx950 kernel; the host only sequences passes and reads counts.
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -16,6 +17,9 @@ namespace ohx {
 int g_verbose = 0;
 // indset.cpp:33 stops after 100 rounds; raising it is an extension
 u32 g_indset_max_rounds = 100;
+// the most rounds a set that went to the checked loop needed (a set that
+// settles in the first unchecked batch needs at most 8)
+std::atomic<u32> g_indset_rounds_seen{0};
 // FP32 screening with FP64 re-evaluation near every decision (extension)
 bool g_mixed = false;
 
@@ -177,7 +181,15 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
     }
     u32 h[BATCH];
     to_host(h, flags, nr);
-    if (!h[nr - 1]) return a;
+    if (!h[nr - 1]) {
+      // h[r]: entities still undecided after round r of this batch
+      u32 k = 0;
+      while (h[k]) ++k;
+      u32 used = done + k + 1, seen = g_indset_rounds_seen.load();
+      while (used > seen && !g_indset_rounds_seen.compare_exchange_weak(seen, used)) {
+      }
+      return a;
+    }
   }
   fail("independent set did not converge within the round limit (indset.cpp:107-116)", __FILE__, __LINE__);
   return a;
@@ -188,10 +200,13 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
 }
 
 // a first batch of rounds with no host check; if the set has not settled
-// the caller starts over with find_indset (same rounds, same result)
+// the caller starts over with find_indset (same rounds, same result).  The
+// batch never runs more rounds than the limit allows, so a set that needs
+// more than the limit always reaches find_indset's failure.
 static Buf<u32> find_indset_bounded(Mesh& m, u32 ent_dim, u32 const* filter, double const* good,
                                     u32 const** last_flag) {
-  return indset_rounds(m, ent_dim, filter, good, 8, last_flag);
+  u32 const rounds = g_indset_max_rounds < 8 ? g_indset_max_rounds : 8;
+  return indset_rounds(m, ent_dim, filter, good, rounds, last_flag);
 }
 
 // ------------------------------------------------------------- inherit

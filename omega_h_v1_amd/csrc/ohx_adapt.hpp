@@ -1,6 +1,8 @@
 // Adapt passes on a device-resident mesh.
 #pragma once
 
+#include <atomic>
+
 #include "ohx_mesh.hpp"
 
 namespace ohx {
@@ -83,6 +85,7 @@ void batch_adapt(Mesh& m, BatchCtx& b, double size_floor, double good_qual, u32 
 
 extern int g_verbose;
 extern u32 g_indset_max_rounds;
+extern std::atomic<u32> g_indset_rounds_seen;
 extern bool g_mixed;
 void mixed_stats(double* max_err, unsigned long long* screened, unsigned long long* reeval);
 void mixed_stats_reset();

@@ -727,6 +727,9 @@ unsigned ohx_tuning_report(char* buf, unsigned cap) {
   });
 }
 void ohx_set_indset_max_rounds(unsigned n) { g_indset_max_rounds = n ? n : 100; }
+unsigned ohx_indset_rounds_max(int reset) {
+  return reset ? g_indset_rounds_seen.exchange(0) : g_indset_rounds_seen.load();
+}
 void ohx_set_verbose(int on) { g_verbose = on; }
 void ohx_profile_enable(int on) { prof_enable(on != 0); }
 void ohx_profile_only(char const* name) { prof_only(name); }

@@ -1,6 +1,8 @@
 // Device-wide scans and reductions for gfx950 (64-lane wavefronts).
 #include <cfloat>
 
+#include <atomic>
+
 #include "ohx_prims.hpp"
 #include "ohx_scan.hpp"
 
@@ -81,14 +83,26 @@ namespace {
 // (each scan tags its words with a new epoch)
 struct ScanState {
   u64* tiles = nullptr;
+  u32* ticket = nullptr;
   u32 cap = 0;
   u32 epoch = 0;
 };
+std::atomic<u32> g_scan_threads{0};
 ScanState& scan_state() {
   static thread_local ScanState st;
+  if (!st.ticket) {
+    // the thread's ticket word, zero between scans (the last block of a
+    // ticketed scan clears it); a thread that scans stays counted, since
+    // driving threads live until the process ends
+    st.ticket = static_cast<u32*>(dev_alloc(sizeof(u32)));
+    OHX_CHECK(hipMemsetAsync(st.ticket, 0, sizeof(u32), stream()));
+    g_scan_threads.fetch_add(1);
+  }
   return st;
 }
 }  // namespace
+
+bool scan_concurrent() { return g_scan_threads.load(std::memory_order_relaxed) > 1; }
 
 ScanTiles scan_tiles_for(u32 nt) {
   ScanState& st = scan_state();
@@ -104,7 +118,7 @@ ScanTiles scan_tiles_for(u32 nt) {
   }
   // epochs stay below 2^30 - 1, so fresh all-ones words never match one
   st.epoch = st.epoch + 1 >= 0x3fffffffu ? 1u : st.epoch + 1;
-  return ScanTiles{st.tiles, st.epoch};
+  return ScanTiles{st.tiles, st.ticket, st.epoch};
 }
 
 void exscan(u32 const* a, u32 n, u32* out) {

@@ -17,12 +17,17 @@ namespace ohx {
 
 constexpr u32 SCAN_T = 256;
 
-// tile words shared by every scan of a host thread (its stream)
+// tile words shared by every scan of a host thread (its stream), and the
+// thread's tile ticket word (zero between scans)
 struct ScanTiles {
   u64* tiles;
+  u32* ticket;
   u32 epoch;
 };
 ScanTiles scan_tiles_for(u32 ntiles);
+// true once more than one host thread drives the library: scans then take
+// their tile from a ticket (see scan_fn_kernel)
+bool scan_concurrent();
 
 namespace scan_detail {
 
@@ -59,23 +64,40 @@ __device__ inline u64 pack(u32 epoch, u32 inclusive, u32 v) {
 
 __device__ inline u32 pad(u32 i) { return i + (i >> 5); }
 
-template <u32 SCAN_I, u32 LB, class F>
+template <u32 SCAN_I, u32 LB, bool TICKET, class F>
 __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles, u32* __restrict__ out,
-                                                         u64* __restrict__ tiles,
+                                                         u64* __restrict__ tiles, u32* __restrict__ ticket,
                                                          u32 epoch, u32* __restrict__ host_total,
                                                          u32* __restrict__ err) {
   constexpr u32 SCAN_TILE = SCAN_T * SCAN_I;
   __shared__ u32 tile[SCAN_TILE + SCAN_TILE / 32];
   __shared__ u32 lds_w[SCAN_T / 64];
   __shared__ u32 s_prefix;
-  // tile = block index.  Workgroups are dispatched in index order within
-  // each XCD, so the lowest-numbered resident block always finds every
-  // earlier tile finished or on an XCD with free slots: the look-back
-  // cannot wait on a block that is never scheduled.  (A fetch-add ticket
-  // would give the same guarantee without that argument, but same-address
-  // atomics retire one per ~12 ns on this part, which serialises large
-  // scans.)  The spin stays bounded regardless.
+  __shared__ u32 s_tile;
+  // Which tile a block scans.  With one host thread (one stream of scans)
+  // the tile is the block index: workgroups are dispatched in index order
+  // within each XCD, so the lowest-numbered resident block always finds
+  // every earlier tile finished or on an XCD with free slots, and the
+  // look-back cannot wait on a block that is never scheduled.  That
+  // argument does not cover scans of other streams holding the free slots,
+  // so with several host threads (TICKET) a block takes its tile from a
+  // fetch-add ticket: tiles are then claimed only by resident blocks, in
+  // order, and the lowest unfinished tile always belongs to a running block
+  // whose predecessors are all done, whatever else occupies the GPU.  (The
+  // ticket costs one same-address atomic per block, which retire one per
+  // ~12 ns on this part, so the single-stream path does without it.)  The
+  // block that draws the last ticket clears the word for the next scan of
+  // the stream.  The spin stays bounded regardless.
   u32 t = blockIdx.x;
+  if (TICKET) {
+    if (threadIdx.x == 0) {
+      u32 k = atomicAdd(ticket, 1u);
+      if (k == ntiles - 1) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_tile = k;
+    }
+    __syncthreads();
+    t = s_tile;
+  }
   u32 base = t * SCAN_TILE;
 #pragma unroll
   for (u32 k = 0; k < SCAN_I; ++k) {
@@ -184,22 +206,30 @@ __global__ __launch_bounds__(SCAN_T) void scan_fn_kernel(u32 n, F f, u32 ntiles,
 
 }  // namespace scan_detail
 
-template <u32 I, u32 LB, class F>
-void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hipEvent_t eb) {
+template <u32 I, u32 LB, bool TICKET, class F>
+void exscan_launch_t(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hipEvent_t eb) {
   constexpr u32 tile = SCAN_T * I;
   u32 nt = n ? (n + tile - 1) / tile : 1;
   ScanTiles st = scan_tiles_for(nt);
   if (ea) {
-    hipExtLaunchKernelGGL(scan_detail::scan_fn_kernel<I, LB, F>, dim3(nt), dim3(SCAN_T), 0, stream(), ea, eb, 0, n, f, nt,
-                          out, st.tiles, st.epoch, host_total_dev, dev_error_word());
+    hipExtLaunchKernelGGL(scan_detail::scan_fn_kernel<I, LB, TICKET, F>, dim3(nt), dim3(SCAN_T), 0, stream(), ea, eb,
+                          0, n, f, nt, out, st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
     check_launch("scan");
     return;
   }
   // compulsory bytes: one 4-byte input per item (a lower bound for functors
   // that read more) and the n + 1 offsets out
   double prof_bytes = (double) n * 4 + (double) (n + 1) * 4;
-  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<I, LB, F>, dim3(nt), dim3(SCAN_T), n, f, nt, out,
-               st.tiles, st.epoch, host_total_dev, dev_error_word());
+  launch_timed("scan", prof_bytes, scan_detail::scan_fn_kernel<I, LB, TICKET, F>, dim3(nt), dim3(SCAN_T), n, f, nt,
+               out, st.tiles, st.ticket, st.epoch, host_total_dev, dev_error_word());
+}
+
+template <u32 I, u32 LB, class F>
+void exscan_launch(u32 n, F f, u32* out, u32* host_total_dev, hipEvent_t ea, hipEvent_t eb) {
+  if (scan_concurrent())
+    exscan_launch_t<I, LB, true>(n, f, out, host_total_dev, ea, eb);
+  else
+    exscan_launch_t<I, LB, false>(n, f, out, host_total_dev, ea, eb);
 }
 
 template <class F>

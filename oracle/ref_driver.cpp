@@ -301,7 +301,8 @@ double ref_triangle_xy_quality(double const* x9)
  * stats[1] = sum of input element counts over those passes,
  * stats[2] = seconds spent in those passes,
  * stats[3] = seconds of the whole call.
- * Returns -1 instead of aborting when max_ops is exceeded. */
+ * Returns -1 instead of aborting when max_ops is exceeded (with the stats of
+ * the passes run until then). */
 static void summary_quiet(struct mesh* m)
 {
   (void) mesh_min_quality(m);
@@ -328,7 +329,9 @@ int ref_adapt_timed(void* vm, double size_floor, double good_qual,
     if (ok) { npass += 1; nelems += tin; tpass += t1 - t0; } \
   } while (0)
 #define COUNT_OP() do { \
-    if (ops > max_ops) return -1; \
+    if (ops > max_ops) { \
+      stats[0] = npass; stats[1] = nelems; stats[2] = tpass; stats[3] = now_s() - t_begin; \
+      return -1; } \
     ++ops; summary_quiet(m); } while (0)
   {
     double qual_floor = mesh_min_quality(m);
