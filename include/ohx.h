@@ -53,6 +53,18 @@ unsigned ohx_measure_edges_f64(unsigned nedges, unsigned const* verts_of_edges,
 /* uints_exscan (ints.cpp:25-36): out[0]=0, out[i+1]=a[0]+...+a[i] (n+1 values) */
 unsigned ohx_exscan_u32(unsigned const* a, unsigned n, unsigned* out, void* stream) OSH_PUBLIC;
 
+/* No counterpart in the reference: ohx_exscan_u32 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It allocates nothing: its
+ * look-back tile words live in a fixed device array of the library (256 KiB
+ * per GPU, held until the process ends), so n is limited to 2^27 =
+ * 134217728 items; a larger n returns OHX_ERROR with ohx_last_error() and
+ * launches nothing.  Calls on one GPU share that array (it keeps the last
+ * call's contents), so calls that overlap on one GPU (two host threads, or
+ * two streams with no ordering between them) race.  stream: the caller's
+ * stream (NULL: the null stream), synchronised before the call returns. */
+unsigned ohx_exscan_u32_noalloc(unsigned const* a, unsigned n, unsigned* out, void* stream) OSH_PUBLIC;
+
 /* up_from_down (up_from_down.cpp:24-47) over invert_map
  * (invert_map.cpp:118-136): offsets (nlows+1), highs and directions
  * (nhighs*deg each) sorted by (high, direction) per low. */

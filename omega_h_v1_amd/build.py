@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["ohx_runtime.hip", "ohx_prims.hip", "ohx_mesh.hip", "ohx_adapt.hip", "ohx_transfer.hip", "ohx_fields.hip", "ohx_batch.hip", "ohx_vtk.hip",
-           "ohx_diff.hip", "ohx_api.hip"]
+           "ohx_diff.hip", "ohx_api.hip", "ohx_noalloc.hip"]
 # host-only command-line tools linked against the library
 TOOLS = {"vtkdiff": "ohx_vtkdiff.cpp"}
 BIN = os.path.join(HERE, "bin")

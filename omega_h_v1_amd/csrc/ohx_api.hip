@@ -44,6 +44,15 @@ struct ohx_mesh {
 static thread_local std::string t_err;
 static thread_local bool t_has_err = false;
 
+namespace ohx {
+// for entry points defined in other sources
+void api_set_error(char const* what) {
+  t_err = what;
+  t_has_err = true;
+}
+void noalloc_setup_current_device();  // ohx_noalloc.hip
+}  // namespace ohx
+
 template <class F, class R>
 static R guard(R bad, F&& f) {
   try {
@@ -766,6 +775,8 @@ void ohx_swap_int_edges(unsigned rs, unsigned code, unsigned* out) {
 void osh_init(int*, char***) {
   osh_guard([]() {
     rt();
+    // the fixed scan scratch of ohx_exscan_u32_noalloc on this device
+    noalloc_setup_current_device();
     return 0;
   });
 }
