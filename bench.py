@@ -429,6 +429,8 @@ def run_c3(args, lib):
             "mixed_stats_survey": svm["mixed"],
         },
         "full_run": {"fp64": svf["full_run"], "mixed": svm["full_run"]},
+        "per_cycle": {p: [{"passes": c["passes"], "elements": c["elements"], "pass_seconds": round(c["pass_seconds"], 4),
+                           "tets_out": c["counts"][3]} for c in surveys[p]["cycles"]] for p in ("fp64", "mixed")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": svm["kernels"],

@@ -827,15 +827,17 @@ __device__ inline double group_min(double v) {
 
 template <u32 DIM, u32 G, bool MIXED>
 __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
-    u32 ne, u32 const* __restrict__ ev, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
-    u32 const* __restrict__ ud, u32 const* __restrict__ vel, double const* __restrict__ x,
-    double const* __restrict__ mt, double const* __restrict__ eq, double qual_floor_all,
-    double const* __restrict__ qfe, u32* __restrict__ codes, double* __restrict__ out, u32* __restrict__ words, u32 tag,
-    ScreenStats st) {
+    u32 ncand, u32 const* __restrict__ list, u32 const* __restrict__ ev, u32 const* __restrict__ uo,
+    u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ vel,
+    double const* __restrict__ x, double const* __restrict__ mt, double const* __restrict__ eq,
+    double qual_floor_all, double const* __restrict__ qfe, u32* __restrict__ codes, double* __restrict__ out,
+    u32* __restrict__ words, u32 tag, ScreenStats st) {
   size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
-  size_t i = gid / G;
+  size_t slot = gid / G;
   u32 lane = (u32) (gid % G);
-  if (i >= ne) return;  // whole groups leave together
+  if (slot >= ncand) return;  // whole groups leave together
+  // the candidate edges, compacted
+  u32 const i = list[slot];
   u32 c = codes[i];
   if (c == DONT_COLLAPSE) return;
   // one floor for the mesh, or its component's in a batch
@@ -1025,8 +1027,23 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   u32 ne = count(m, 1);
   check_basic_class(m, codes.get());
   for (u32 d = 2; d <= dim; ++d) check_exposed_sides(m, d, codes.get());
-  // no read here: the quality kernel skips edges without a collapse and
-  // flags whether any survive, which is read once below
+  // the edges with a collapse direction left, compacted: the quality
+  // kernel's groups then all have work (the count, read once, is also the
+  // early return when the class checks leave nothing)
+  u32 ncand;
+  Buf<u32> clist;
+  {
+    Buf<u32> coff(ne + 1);
+    u32 const* cp = codes.get();
+    ncand = exscan_fn_total(ne, OHX_LAMBDA(u32 i) { return cp[i] != DONT_COLLAPSE ? 1u : 0u; }, coff.get());
+    if (!ncand) return 0;
+    clist = Buf<u32>(ncand);
+    u32 const* co = coff.get();
+    u32* cl = clist.get();
+    parallel_for(ne, OHX_LAMBDA(u32 i) {
+      if (cp[i] != DONT_COLLAPSE) cl[co[i]] = i;
+    });
+  }
   Buf<double> eq;
   u32 const* ev = ask_down(m, 1, 0);
   u32 const* vel = ask_down(m, dim, 0);
@@ -1049,18 +1066,19 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32 const* udir = ue.dirs.get();
     double const* xs = coords(m);
     double const* mts = metric_of(m);
+    u32 const* cl = clist.get();
     auto go = [&](auto kernel, char const* kn, u32 g, hipEvent_t ea, hipEvent_t eb) {
-      dim3 grid(nblocks((size_t) ne * g));
+      dim3 grid(nblocks((size_t) ncand * g));
       if (ea) {
-        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ne, ev, uoff, uadj, udir, vel, xs,
-                              mts, eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
+        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ncand, cl, ev, uoff, uadj, udir, vel,
+                              xs, mts, eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
         check_launch(kn);
       } else {
-        launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ne, ev, uoff, uadj, udir, vel, xs, mts, eq.get(),
-                     qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
+        launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ncand, cl, ev, uoff, uadj, udir, vel, xs, mts,
+                     eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
       }
     };
-    tuned(tuner, ne, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
+    tuned(tuner, ncand, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
       u32 g = var & 15u;
       bool screen = (var & 16u) != 0;
       char const* kf = "coarsen_quality_f64";
@@ -1524,104 +1542,254 @@ __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 cons
   return rs;
 }
 
-// choose_edge_swap (edge_swap.cpp:230-280); with vertex metrics (em, rm
-// non-null) each tet is measured in the average metric of its corners
-__device__ inline u32 choose_edge_swap(u32 rs, V3 const* ex, V3 const* rx, double* quality,
-                                       Sym const* em = nullptr, Sym const* rm = nullptr) {
-  u32 tpm = k_sw_sizes[rs], nm = k_sw_counts[rs];
-  unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs];
-  unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
-  u64 cached = 0;
-  double cache[35];
-  u32 code = INVALID;
-  double best = -1;
-  for (u32 i = 0; i < nm; ++i) {
-    double mesh_minq = 1;
-    for (u32 j = 0; j < tpm; ++j) {
-      u32 tri = mesh[j];
-      if (!((cached >> tri) & 1ull)) {
-        u32 a0 = tris[tri][0], a1 = tris[tri][1], a2 = tris[tri][2];
-        V3 t0 = rx[a0], t1 = rx[a1], t2 = rx[a2];
-        double tmin = 1, q;
-        if (em) {
-          V3 pa[4] = {t0, t1, t2, ex[1]};
-          Sym ma[4] = {rm[a0], rm[a1], rm[a2], em[1]};
-          q = metric_quality<3>(pa, ma, true);
-          if (q < tmin) tmin = q;
-          V3 pb[4] = {t1, t0, t2, ex[0]};
-          Sym mb[4] = {rm[a1], rm[a0], rm[a2], em[0]};
-          q = metric_quality<3>(pb, mb, true);
-          if (q < tmin) tmin = q;
-        } else {
-          q = tet_quality(t0, t1, t2, ex[1]);
-          if (q < tmin) tmin = q;
-          q = tet_quality(t1, t0, t2, ex[0]);
-          if (q < tmin) tmin = q;
-        }
-        cached |= (1ull << tri);
-        cache[tri] = tmin;
-      }
-      double tq = cache[tri];
-      if (tq < mesh_minq) mesh_minq = tq;
-      if (mesh_minq < 0) break;
-      // a triangulation at or below the best so far cannot replace it (a
-      // later one must be strictly better), so its remaining triangles are
-      // not needed: the choice and its quality are unchanged
-      if (!(mesh_minq > best)) break;
-    }
-    if (mesh_minq > best) {
-      code = i;
-      best = mesh_minq;
-    }
-    mesh += tpm;
+// swap_quality (swap_qualities.cpp:13-65) with choose_edge_swap
+// (edge_swap.cpp:230-280), SW_G lanes per edge.  One lane per edge would
+// hold the ring's coordinates and metrics and the quality of every unique
+// triangle (up to 35) in registers, which spills; here a group of lanes
+// shares them in LDS:
+//   A. lane j < ring size reads the edge's j-th tet: its opposite edge
+//      (a, b) oriented as find_edge_ring orients it, and its quality
+//      (old_minq is the minimum over the group);
+//   B. lane 0 chains the pairs into the ring (edge_ring.cpp:12-74);
+//   C. lane j stages ring vertex j (or edge vertex j - ring size): its
+//      coordinates and metric;
+//   D. lane l measures unique triangles l, l + SW_G, ...: the two tets it
+//      forms with the edge's ends, as choose_edge_swap does;
+//   E. lane l takes triangulations l, l + SW_G, ...: the running minimum of
+//      its triangles, stopped at the first negative one as the reference's
+//      loop stops, and the lane's best with a strict > in triangulation
+//      order; the group combines them keeping the higher value and, on
+//      ties, the lower triangulation index -- the reference's sequential
+//      strict > choice.
+// FP32-mixed (MIXED): D and E run first with the FP32 quality; a candidate
+// whose FP32 best lies more than the margin below old_minq (and none of
+// whose triangles is within the margin of zero, where the reference's
+// early stop could fall elsewhere) is rejected there, every other one is
+// measured again in FP64, so every accepted value and code is the FP64
+// path's.  Accepted candidates store the value (goodness), the ring size
+// and the triangulation code, which swap_cavity uses.
+constexpr u32 SW_G = 8;
+constexpr u32 SW_EDGES = BLOCK / SW_G;
+constexpr u32 SW_MAXTRI = 35;
+static __constant__ u32 k_sw_ntri[8] = {0, 0, 0, 1, 4, 10, 20, 35};
+
+template <bool MT>
+struct SwapLds {
+  u32 ra[SW_EDGES][MAX_EDGE_SWAP], rb[SW_EDGES][MAX_EDGE_SWAP];
+  u32 ring[SW_EDGES][MAX_EDGE_SWAP];
+  u32 rs[SW_EDGES];      // ring size of an active edge, else 0
+  u32 again[SW_EDGES];   // FP32-mixed: measure again in FP64
+  double old_minq[SW_EDGES];
+  double x[SW_EDGES][MAX_EDGE_SWAP + 2][3];          // ring, then edge vertices
+  double m[MT ? SW_EDGES : 1][MAX_EDGE_SWAP + 2][6];  // their metrics
+  double tq[SW_EDGES][SW_MAXTRI];
+};
+
+template <bool MT, bool F32>
+__device__ inline double swap_tet_q(SwapLds<MT> const& L, u32 g, u32 a0, u32 a1, u32 a2, u32 e) {
+  V3 p[4];
+  u32 const idx[4] = {a0, a1, a2, e};
+#pragma unroll
+  for (u32 k = 0; k < 4; ++k) p[k] = V3{L.x[g][idx[k]][0], L.x[g][idx[k]][1], L.x[g][idx[k]][2]};
+  if (MT) {
+    Sym pm[4];
+#pragma unroll
+    for (u32 k = 0; k < 4; ++k)
+#pragma unroll
+      for (u32 c = 0; c < 6; ++c) pm[k].m[c] = L.m[g][idx[k]][c];
+    return F32 ? (double) quality32<3>(p, pm, true) : metric_quality<3>(p, pm, true);
   }
-  *quality = best;
-  return code;
+  return F32 ? (double) quality32<3>(p, nullptr, true) : tet_quality(p[0], p[1], p[2], p[3]);
 }
 
-// swap_quality (swap_qualities.cpp:13-65)
-__global__ __launch_bounds__(BLOCK) void swap_quality_f64(
-    u32 ne, u32 const* __restrict__ uo, u32 const* __restrict__ ua, u32 const* __restrict__ ud,
-    u32 const* __restrict__ ev, u32 const* __restrict__ vt, double const* __restrict__ x,
-    double const* __restrict__ mt, double const* __restrict__ eq, u32* __restrict__ cand,
-    double* __restrict__ out, u32* __restrict__ rsz, u32* __restrict__ words, u32 tag, u32* __restrict__ err) {
-  u32 i = xcd_block() * BLOCK + threadIdx.x;
-  if (i >= ne || !cand[i]) return;
-  double old_minq = 1;
-  for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
-    double q = eq[ua[j]];
-    if (q < old_minq) old_minq = q;
+// phases D and E for the group of this lane; returns the group's (best,
+// code) in every lane of the group, *near0 set if any triangle's quality is
+// within SCREEN_MARGIN of zero
+template <bool MT, bool F32>
+__device__ inline void swap_choose(SwapLds<MT>& L, u32 g, u32 lane, bool active, double* best, u32* code,
+                                   bool* near0) {
+  u32 rs = active ? L.rs[g] : 0u;
+  bool nz = false;
+  if (rs) {
+    u32 ntri = k_sw_ntri[rs];
+    unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
+    u32 const E0 = rs, E1 = rs + 1;  // edge vertices follow the ring in LDS
+    for (u32 t = lane; t < ntri; t += SW_G) {
+      u32 a0 = tris[t][0], a1 = tris[t][1], a2 = tris[t][2];
+      double tmin = 1, q;
+      // the triangle with edge end 1, then with the first two corners
+      // exchanged and edge end 0 (choose_edge_swap's two tets)
+      q = swap_tet_q<MT, F32>(L, g, a0, a1, a2, E1);
+      if (q < tmin) tmin = q;
+      q = swap_tet_q<MT, F32>(L, g, a1, a0, a2, E0);
+      if (q < tmin) tmin = q;
+      if (F32 && !(tmin > (double) SCREEN_MARGIN || tmin < -(double) SCREEN_MARGIN)) nz = true;
+      L.tq[g][t] = tmin;
+    }
   }
-  u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
-  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
-  if (rs > MAX_EDGE_SWAP) {
-    cand[i] = 0;
-    return;
+  __syncthreads();
+  double b = -1;
+  u32 c = INVALID;
+  if (rs) {
+    u32 tpm = k_sw_sizes[rs], nm = k_sw_counts[rs];
+    unsigned char const* meshes = k_sw_meshes + k_sw_mesh_off[rs];
+    for (u32 k = lane; k < nm; k += SW_G) {
+      double mq = 1;
+      for (u32 j = 0; j < tpm; ++j) {
+        double tq = L.tq[g][meshes[k * tpm + j]];
+        if (tq < mq) mq = tq;
+        if (mq < 0) break;
+      }
+      if (mq > b) {
+        b = mq;
+        c = k;
+      }
+    }
   }
-  if (rs < 3) {
-    *err = ERR_EDGE_RING;
-    cand[i] = 0;
-    return;
+  // combine over the group: higher value, then lower triangulation index
+#pragma unroll
+  for (u32 d = SW_G / 2; d >= 1; d >>= 1) {
+    double ob = __shfl_xor(b, d, SW_G);
+    u32 oc = (u32) __shfl_xor((int) c, d, SW_G);
+    if (ob > b || (ob == b && oc < c)) {
+      b = ob;
+      c = oc;
+    }
+    nz = nz || __shfl_xor((int) nz, d, SW_G);
   }
-  V3 ex[2] = {ld3(x + 3 * (size_t) edge_v[0]), ld3(x + 3 * (size_t) edge_v[1])};
-  V3 rx[MAX_EDGE_SWAP];
-  for (u32 j = 0; j < rs; ++j) rx[j] = ld3(x + 3 * (size_t) ring_v[j]);
-  double sq;
-  if (mt) {
-    Sym em[2] = {ld6(mt + 6 * (size_t) edge_v[0]), ld6(mt + 6 * (size_t) edge_v[1])};
-    Sym rm[MAX_EDGE_SWAP];
-    for (u32 j = 0; j < rs; ++j) rm[j] = ld6(mt + 6 * (size_t) ring_v[j]);
-    choose_edge_swap(rs, ex, rx, &sq, em, rm);
+  *best = b;
+  *code = c;
+  *near0 = nz;
+}
+
+template <bool MT, bool MIXED>
+__global__ __launch_bounds__(BLOCK) void swap_quality_k(
+    u32 ncand, u32 const* __restrict__ list, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
+    u32 const* __restrict__ ud, u32 const* __restrict__ ev, u32 const* __restrict__ vt,
+    double const* __restrict__ x, double const* __restrict__ mt, double const* __restrict__ eq,
+    u32* __restrict__ cand, double* __restrict__ out, u32* __restrict__ rsz, u32* __restrict__ codes,
+    u32* __restrict__ words, u32 tag, u32* __restrict__ err, ScreenStats st) {
+  __shared__ SwapLds<MT> L;
+  u32 const g = threadIdx.x / SW_G, lane = threadIdx.x % SW_G;
+  // the candidates, compacted: group g of the block takes one
+  u32 const slot = xcd_block() * SW_EDGES + g;
+  bool active = slot < ncand;
+  u32 const i = active ? list[slot] : 0u;
+  // A: the tets around the edge
+  u32 f = 0, rs = 0, e0 = 0, e1 = 0;
+  double qmin = 1;
+  if (active) {
+    f = uo[i];
+    rs = uo[i + 1] - f;
+    e0 = ev[2 * (size_t) i];
+    e1 = ev[2 * (size_t) i + 1];
+    if (rs > MAX_EDGE_SWAP || rs < 3) {
+      if (lane == 0) {
+        if (rs < 3) *err = ERR_EDGE_RING;
+        cand[i] = 0;
+      }
+      active = false;
+    } else if (lane < rs) {
+      u32 tet = ua[f + lane], te = ud[f + lane];
+      u32 const* tv = vt + 4 * (size_t) tet;
+      u32 const* tve = canon(3, 1, 0, te);
+      u32 al = (tv[tve[0]] != e0) ? 1u : 0u;
+      u32 const* tvo = canon(3, 1, 0, opp(3, 1)[te]);
+      L.ra[g][lane] = tv[tvo[al]];
+      L.rb[g][lane] = tv[tvo[al ^ 1u]];
+      double q = eq[tet];
+      if (q < qmin) qmin = q;
+    }
+  }
+#pragma unroll
+  for (u32 d = SW_G / 2; d >= 1; d >>= 1) {
+    double o = __shfl_xor(qmin, d, SW_G);
+    if (o < qmin) qmin = o;
+  }
+  __syncthreads();
+  // B: the ring, chained as find_edge_ring does
+  if (active && lane == 0) {
+    u32* ra = L.ra[g];
+    u32* rb = L.rb[g];
+    bool closed = true;
+    for (u32 a = 0; a + 1 < rs; ++a) {
+      u32 j;
+      for (j = a + 1; j < rs; ++j)
+        if (ra[j] == rb[a]) break;
+      if (j == rs) {
+        closed = false;
+        break;
+      }
+      u32 ta = ra[j], tb = rb[j];
+      ra[j] = ra[a + 1];
+      rb[j] = rb[a + 1];
+      ra[a + 1] = ta;
+      rb[a + 1] = tb;
+    }
+    if (!closed) {
+      *err = ERR_EDGE_RING;
+      cand[i] = 0;
+    }
+    L.rs[g] = closed ? rs : 0u;
+    L.old_minq[g] = qmin;
+    for (u32 a = 0; a < rs; ++a) L.ring[g][a] = ra[a];
+  } else if (lane == 0) {
+    L.rs[g] = 0;
+  }
+  __syncthreads();
+  // C: the ring's and the edge's vertices
+  rs = active ? L.rs[g] : 0u;
+  active = rs != 0;
+  // (a 7-ring has 9 vertices: more than the group's lanes)
+  for (u32 k = lane; active && k < rs + 2; k += SW_G) {
+    u32 v = k < rs ? L.ring[g][k] : (k == rs ? e0 : e1);
+#pragma unroll
+    for (u32 c = 0; c < 3; ++c) L.x[g][k][c] = x[3 * (size_t) v + c];
+    if (MT)
+#pragma unroll
+      for (u32 c = 0; c < 6; ++c) L.m[g][k][c] = mt[6 * (size_t) v + c];
+  }
+  if (lane == 0) L.again[g] = 0;
+  // the block's work needs the staged data and every lane of the group
+  // (and of the block: the phases below are separated by barriers)
+  __syncthreads();
+  double best;
+  u32 code;
+  bool near0;
+  double const old_minq = active ? L.old_minq[g] : 1.0;
+  if (MIXED) {
+    swap_choose<MT, true>(L, g, lane, active, &best, &code, &near0);
+    bool reject = active && !near0 && best + (double) SCREEN_MARGIN <= old_minq;
+    if (lane == 0) {
+      if (active) L.again[g] = reject ? 0u : 1u;
+    }
+    wave_count(st.screened, lane == 0 && reject);
+    wave_count(st.reeval, lane == 0 && active && !reject);
+    if (reject && lane == 0) cand[i] = 0;
+    __syncthreads();
+    double best32 = best;
+    bool const valid32 = !near0;  // no triangle near zero: best32 is the FP64 choice's, rounded
+    bool again = active && L.again[g] != 0;
+    swap_choose<MT, false>(L, g, lane, again, &best, &code, &near0);
+    if (again && valid32 && lane == 0 && best == best) {
+      double dd = best - best32;
+      note_max_err(st.max_err, (float) (dd < 0 ? -dd : dd));
+    }
+    active = again;
   } else {
-    choose_edge_swap(rs, ex, rx, &sq);
+    swap_choose<MT, false>(L, g, lane, active, &best, &code, &near0);
   }
-  if (sq > old_minq) {
-    out[i] = sq;
-    rsz[i] = rs;
-    wave_flag_value(words + W_ANY, true, tag);
-  } else {
-    cand[i] = 0;
+  bool accept = active && best > old_minq;
+  if (active && lane == 0) {
+    if (accept) {
+      out[i] = best;
+      rsz[i] = rs;
+      codes[i] = code;
+    } else {
+      cand[i] = 0;
+    }
   }
+  wave_flag_value(words + W_ANY, accept, tag);
 }
 
 // count_swap_ents (edge_swap.cpp:375-386)
@@ -1630,30 +1798,18 @@ static inline __host__ __device__ u32 count_swap_ents(u32 rs, u32 d, u32 nint, u
 }
 
 // swap_cavity (swap_topology.cpp:46-75) with get_swap_ents
-// (edge_swap.cpp:282-373): recompute the ring and choice, emit products
-__global__ __launch_bounds__(BLOCK) void swap_cavity_f64(
+// (edge_swap.cpp:282-373): the ring again (integer work) and the
+// triangulation code that swap_quality chose, then the products
+__global__ __launch_bounds__(BLOCK) void swap_cavity_k(
     u32 ne, u32 d, u32 const* __restrict__ indset, u32 const* __restrict__ goff, u32 const* __restrict__ uo,
     u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
-    u32 const* __restrict__ vt, double const* __restrict__ x, double const* __restrict__ mt,
-    u32* __restrict__ out) {
+    u32 const* __restrict__ vt, u32 const* __restrict__ codes, u32* __restrict__ out) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ne || !indset[i]) return;
   u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
   u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
   if (rs < 3 || rs > MAX_EDGE_SWAP) return;  // cannot happen for an indset edge
-  V3 ex[2] = {ld3(x + 3 * (size_t) edge_v[0]), ld3(x + 3 * (size_t) edge_v[1])};
-  V3 rx[MAX_EDGE_SWAP];
-  for (u32 j = 0; j < rs; ++j) rx[j] = ld3(x + 3 * (size_t) ring_v[j]);
-  double sq;
-  u32 code;
-  if (mt) {
-    Sym em[2] = {ld6(mt + 6 * (size_t) edge_v[0]), ld6(mt + 6 * (size_t) edge_v[1])};
-    Sym rm[MAX_EDGE_SWAP];
-    for (u32 j = 0; j < rs; ++j) rm[j] = ld6(mt + 6 * (size_t) ring_v[j]);
-    code = choose_edge_swap(rs, ex, rx, &sq, em, rm);
-  } else {
-    code = choose_edge_swap(rs, ex, rx, &sq);
-  }
+  u32 const code = codes[i];
   u32 tpm = k_sw_sizes[rs];
   unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs] + code * tpm;
   unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
@@ -1746,9 +1902,26 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
 static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   check_index_space(m);
   u32 ne = count(m, 1);
-  if (!max_u32(cand.get(), ne)) return 0;
   unmark_boundary(m, 1, cand.get());
-  if (!max_u32(cand.get(), ne)) return 0;
+  // the candidates, compacted (their count is the reference's early
+  // return, swap.cpp:103-106, in one read)
+  Buf<u32> coff(ne + 1);
+  u32 ncand;
+  {
+    u32 const* cp = cand.get();
+    ncand = exscan_fn_total(ne, OHX_LAMBDA(u32 i) { return cp[i] ? 1u : 0u; }, coff.get());
+  }
+  if (!ncand) return 0;
+  Buf<u32> clist(ncand);
+  {
+    u32 const* cp = cand.get();
+    u32 const* co = coff.get();
+    u32* cl = clist.get();
+    parallel_for(ne, OHX_LAMBDA(u32 i) {
+      if (cp[i]) cl[co[i]] = i;
+    });
+  }
+  coff.release();
   make_swap_tables();
   Up const& u = ask_up(m, 1, 3);
   u32 const* ev = ask_down(m, 1, 0);
@@ -1756,12 +1929,45 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   Buf<double> eq = mesh_qualities(m);
   Buf<double> q(ne);
   Buf<u32> rsz = filled(ne, 0);
+  Buf<u32> codes(ne);
   u32 tag = next_any_tag();
   {
-    double prof_bytes = (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 + (double) count(m, 0) * 24;
-    launch_timed("swap_quality_f64", prof_bytes, swap_quality_f64, dim3(nblocks(ne)), dim3(BLOCK), ne, u.offsets.get(),
-                       u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m), eq.get(), cand.get(), q.get(),
-                       rsz.get(), scratch_words(), tag, dev_error_word());
+    // compulsory bytes: the edges' candidate flags, vertices and outputs,
+    // the tets around them (uses + directions), their qualities and
+    // vertices, and the vertex coordinates (and metrics)
+    double const* mts = metric_of(m);
+    double prof_bytes = (double) ne * 24 + (double) u.adj.size() * 8 + (double) count(m, 3) * 24 +
+                        (double) count(m, 0) * (mts ? 72 : 24);
+    ScreenStats sst = screen_stats();
+    dim3 grid((unsigned) ((ncand + SW_EDGES - 1) / SW_EDGES));
+    auto go = [&](auto kernel, char const* kn, hipEvent_t ea, hipEvent_t eb) {
+      if (ea) {
+        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ncand, clist.get(), u.offsets.get(),
+                              u.adj.get(), u.dirs.get(), ev, vt, coords(m), mts, eq.get(), cand.get(), q.get(),
+                              rsz.get(), codes.get(), scratch_words(), tag, dev_error_word(), sst);
+        check_launch(kn);
+      } else {
+        launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ncand, clist.get(), u.offsets.get(), u.adj.get(),
+                     u.dirs.get(), ev, vt, coords(m), mts, eq.get(), cand.get(), q.get(), rsz.get(), codes.get(),
+                     scratch_words(), tag, dev_error_word(), sst);
+      }
+    };
+    // in the mixed mode the FP32 screen is a variant chosen online: it pays
+    // only where it rejects enough candidates (most swap candidates improve
+    // their cavity and need their exact FP64 value anyway)
+    static thread_local Tuner tuner("swap_quality_3d_mixed", {0, 1});
+    auto run = [&](u32 screen, hipEvent_t ea, hipEvent_t eb) {
+      if (screen)
+        mts ? go(swap_quality_k<true, true>, "swap_quality_mixed", ea, eb)
+            : go(swap_quality_k<false, true>, "swap_quality_mixed", ea, eb);
+      else
+        mts ? go(swap_quality_k<true, false>, "swap_quality_f64", ea, eb)
+            : go(swap_quality_k<false, false>, "swap_quality_f64", ea, eb);
+    };
+    if (g_mixed)
+      tuned(tuner, ncand, run);
+    else
+      run(0, nullptr, nullptr);
   }
   eq.release();
   if (read_word(W_ANY) != tag) return 0;
@@ -1818,9 +2024,8 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     {
       double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
-      launch_timed("swap_cavity_f64", prof_bytes, swap_cavity_f64, dim3(nblocks(ne)), dim3(BLOCK), ne, d, is, goff.get(),
-                         u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, coords(m), metric_of(m),
-                         vout.get() + (size_t) nsame * vpe);
+      launch_timed("swap_cavity", prof_bytes, swap_cavity_k, dim3(nblocks(ne)), dim3(BLOCK), ne, d, is, goff.get(),
+                   u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), vout.get() + (size_t) nsame * vpe);
     }
     Buf<u32> cdo, cio;
     SwapKeepArgs a{};
