@@ -825,13 +825,86 @@ __device__ inline double group_min(double v) {
   return v;
 }
 
-template <u32 DIM, u32 G, bool MIXED>
-__global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
-    u32 ncand, u32 const* __restrict__ list, u32 const* __restrict__ ev, u32 const* __restrict__ uo,
-    u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ vel,
-    double const* __restrict__ x, double const* __restrict__ mt, double const* __restrict__ eq,
-    double qual_floor_all, double const* __restrict__ qfe, u32* __restrict__ codes, double* __restrict__ out,
-    u32* __restrict__ words, u32 tag, ScreenStats st) {
+// Modes of the collapse-quality kernel.  CQ_F64: the reference's
+// arithmetic.  CQ_SCREEN: an FP32 pass rejects the directions certainly
+// below the floor, every other direction is measured again in FP64.
+// CQ_CERTIFIED: the FP32 pass settles every direction whose FP32 minimum is
+// farther than the margin from the floor -- rejected, or accepted with the
+// FP32 minimum as its goodness, marked inexact (bit 2 + direction of the
+// code) -- and measures only the rest in FP64.  An inexact goodness is used
+// only where every comparison it takes part in is certified (see
+// coarsen_certify); CQ_FORCE then measures in FP64 the inexact directions
+// of the vertices that need exact values.
+enum CqMode : u32 { CQ_F64 = 0, CQ_SCREEN = 1, CQ_CERTIFIED = 2, CQ_FORCE = 3 };
+__host__ __device__ inline u32 inexact_bit(u32 d) { return 4u << d; }
+
+template <u32 DIM, bool F32>
+__device__ inline double collapse_elem_q(u32 e, u32 dir, u32 gen, V3 const& pg, Sym const& mg,
+                                         u32 const* __restrict__ vel, double const* __restrict__ x,
+                                         double const* __restrict__ mt, bool* adj) {
+  constexpr u32 VPE = DIM + 1;
+  u32 const* v = vel + (size_t) e * VPE;
+  bool a = false;
+#pragma unroll
+  for (u32 l = 0; l < VPE; ++l)
+    if (v[l] == gen) a = true;
+  *adj = a;
+  if (a) return 1.0;
+  u32 const* bv = canon(DIM, DIM - 1, 0, opp(DIM, 0)[dir]);
+  V3 p[VPE];
+  p[VPE - 1] = pg;
+#pragma unroll
+  for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
+  if (mt) {
+    Sym pm[VPE];
+    pm[VPE - 1] = mg;
+#pragma unroll
+    for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
+    return F32 ? (double) quality32<DIM>(p, pm, true) : metric_quality<DIM>(p, pm, true);
+  }
+  return F32 ? (double) quality32<DIM>(p, nullptr, true) : element_quality(DIM, p);
+}
+
+struct CqArgs {
+  u32 ncand;
+  u32 const* list;  // candidate edges, compacted
+  u32 const* ev;
+  u32 const* uo;    // up(0, dim)
+  u32 const* ua;
+  u32 const* ud;
+  u32 const* vel;   // down(dim, 0)
+  double const* x;
+  double const* mt;
+  double const* eq;  // element qualities (require_better) or null
+  double qual_floor;
+  double const* qfe;  // per-edge floors (batch) or null
+  u32* codes;
+  double* out;
+  u32 const* vmark;  // CQ_FORCE: vertices that need exact goodness
+  u32* words;
+  u32 tag;
+  ScreenStats st;
+};
+
+template <u32 DIM, u32 G, u32 MODE>
+__global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(CqArgs A) {
+  u32 const ncand = A.ncand, tag = A.tag;
+  u32 const* __restrict__ list = A.list;
+  u32 const* __restrict__ ev = A.ev;
+  u32 const* __restrict__ uo = A.uo;
+  u32 const* __restrict__ ua = A.ua;
+  u32 const* __restrict__ ud = A.ud;
+  u32 const* __restrict__ vel = A.vel;
+  double const* __restrict__ x = A.x;
+  double const* __restrict__ mt = A.mt;
+  double const* __restrict__ eq = A.eq;
+  double const qual_floor_all = A.qual_floor;
+  double const* __restrict__ qfe = A.qfe;
+  u32* __restrict__ codes = A.codes;
+  double* __restrict__ out = A.out;
+  u32 const* __restrict__ vmark = A.vmark;
+  u32* __restrict__ words = A.words;
+  ScreenStats const st = A.st;
   size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
   size_t slot = gid / G;
   u32 lane = (u32) (gid % G);
@@ -842,105 +915,150 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(
   if (c == DONT_COLLAPSE) return;
   // one floor for the mesh, or its component's in a batch
   double const qual_floor = qfe ? qfe[i] : qual_floor_all;
-  constexpr u32 VPE = DIM + 1;
-  u32 const* bo = opp(DIM, 0);
+  bool const f32 = (MODE == CQ_SCREEN || MODE == CQ_CERTIFIED) && !eq;
   for (u32 j = 0; j < 2; ++j) {
     if (!collapses(c, j)) continue;
     u32 col = ev[2 * i + j], gen = ev[2 * i + (1 - j)];
+    if (MODE == CQ_FORCE && !((c & inexact_bit(j)) && vmark[col])) continue;
     V3 const pg = ld3(x + 3 * (size_t) gen);
     Sym mg;
     if (mt) mg = ld6(mt + 6 * (size_t) gen);
-    float minq32 = 1.f;
-    if (MIXED && !eq) {
-      // FP32 screen: a cavity certainly below the floor is rejected here
+    double minq32 = 1;
+    if (f32) {
+      // FP32: a direction certainly below the floor is rejected at its first
+      // element below it; CQ_CERTIFIED also accepts one certainly above
       bool nan = false;
+      double const lo = qual_floor - (double) SCREEN_MARGIN;
       for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
-        u32 const* v = vel + (size_t) ua[k] * VPE;
-        bool adj = false;
-#pragma unroll
-        for (u32 l = 0; l < VPE; ++l)
-          if (v[l] == gen) adj = true;
+        bool adj;
+        double q = collapse_elem_q<DIM, true>(ua[k], ud[k], gen, pg, mg, vel, x, mt, &adj);
         if (adj) continue;
-        u32 const* bv = canon(DIM, DIM - 1, 0, bo[ud[k]]);
-        V3 p[VPE];
-        p[VPE - 1] = pg;
-#pragma unroll
-        for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
-        float q;
-        if (mt) {
-          Sym pm[VPE];
-          pm[VPE - 1] = mg;
-#pragma unroll
-          for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
-          q = quality32<DIM>(p, pm, true);
-        } else {
-          q = quality32<DIM>(p, nullptr, true);
-        }
         if (!(q == q)) nan = true;
-        minq32 = q < minq32 ? q : minq32;
-        // one element certainly below the floor settles the rejection
-        if (minq32 < (float) qual_floor - SCREEN_MARGIN) break;
+        if (q < minq32) minq32 = q;
+        if (minq32 < lo) break;
       }
+      minq32 = group_min<G>(minq32);
 #pragma unroll
-      for (u32 d = G / 2; d >= 1; d >>= 1) {
-        float o = __shfl_xor(minq32, d, G);
-        minq32 = o < minq32 ? o : minq32;
-        nan = nan || __shfl_xor((int) nan, d, G);
-      }
-      bool reject = !nan && (minq32 < (float) qual_floor - SCREEN_MARGIN);
-      if (lane == 0) wave_count(st.screened, reject);
+      for (u32 d = G / 2; d >= 1; d >>= 1) nan = nan || __shfl_xor((int) nan, d, G);
+      bool reject = !nan && minq32 < lo;
+      bool accept = MODE == CQ_CERTIFIED && !nan && minq32 > qual_floor + (double) SCREEN_MARGIN;
+      if (lane == 0) wave_count(st.screened, reject || accept);
       if (reject) {
         c = dont_collapse(c, j);
         continue;
       }
+      if (accept) {
+        c |= inexact_bit(j);
+        if (lane == 0) out[2 * i + j] = minq32;
+        continue;
+      }
       if (lane == 0) wave_count(st.reeval, true);
     }
+    if (MODE == CQ_FORCE && lane == 0) wave_count(st.reeval, true);
     double minq = 1, old_minq = 1;
     for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
       u32 e = ua[k];
-      u32 const* v = vel + (size_t) e * VPE;
       if (eq && eq[e] < old_minq) old_minq = eq[e];
-      bool adj = false;
-#pragma unroll
-      for (u32 l = 0; l < VPE; ++l)
-        if (v[l] == gen) adj = true;
+      bool adj;
+      double q = collapse_elem_q<DIM, false>(e, ud[k], gen, pg, mg, vel, x, mt, &adj);
       if (adj) continue;
-      u32 const* bv = canon(DIM, DIM - 1, 0, bo[ud[k]]);
-      V3 p[VPE];
-      p[VPE - 1] = pg;
-#pragma unroll
-      for (u32 l = 0; l + 1 < VPE; ++l) p[l] = ld3(x + 3 * (size_t) v[bv[l]]);
-      double q;
-      if (mt) {
-        Sym pm[VPE];
-        pm[VPE - 1] = mg;
-#pragma unroll
-        for (u32 l = 0; l + 1 < VPE; ++l) pm[l] = ld6(mt + 6 * (size_t) v[bv[l]]);
-        q = metric_quality<DIM>(p, pm, true);
-      } else {
-        q = element_quality(DIM, p);
-      }
       if (q < minq) minq = q;
       // below the floor the direction is rejected whatever the remaining
       // elements give (the minimum only falls), so the lane stops; the
-      // mixed kernel keeps the full minimum for its error statistic
-      if (!MIXED && minq < qual_floor) break;
+      // screened kernels keep the full minimum for their error statistic
+      if (MODE == CQ_F64 && minq < qual_floor) break;
     }
     minq = group_min<G>(minq);
     if (eq) old_minq = group_min<G>(old_minq);
-    if (MIXED && !eq && lane == 0 && minq == minq) {
-      float d = (float) minq - minq32;
-      note_max_err(st.max_err, d < 0 ? -d : d);
+    if ((f32 || MODE == CQ_FORCE) && lane == 0 && minq == minq) {
+      double d = MODE == CQ_FORCE ? minq - out[2 * i + j] : minq - minq32;
+      note_max_err(st.max_err, (float) (d < 0 ? -d : d));
     }
+    c &= ~inexact_bit(j);
     if ((minq < qual_floor) || (eq && (minq <= old_minq)))
-      c = dont_collapse(c, j);
+      c = dont_collapse(c, j) & ~inexact_bit(j);
     else if (lane == 0)
       out[2 * i + j] = minq;
   }
   if (lane == 0) {
     codes[i] = c;
-    wave_flag_value(words + W_ANY, c != DONT_COLLAPSE, tag);
+    wave_flag_value(words + W_ANY, (c & 3u) != DONT_COLLAPSE, tag);
   }
+}
+
+// collapse_to_vert (collapses_to_verts.cpp:13-42): whether each vertex
+// collapses and the best goodness of its collapse directions; inex (when
+// given) says whether that goodness is an FP32 value
+__global__ __launch_bounds__(BLOCK) void collapse_to_vert_k(u32 nv, u32 const* __restrict__ uo,
+                                                            u32 const* __restrict__ ua, u32 const* __restrict__ ud,
+                                                            u32 const* __restrict__ cd, double const* __restrict__ q,
+                                                            u32* __restrict__ cand, double* __restrict__ vq,
+                                                            u32* __restrict__ inex) {
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
+  if (i >= nv) return;
+  double maxq = 0;
+  u32 is = 0, mi = 0;
+  for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
+    u32 e = ua[j], d = ud[j];
+    u32 c = cd[e];
+    if (!collapses(c, d)) continue;
+    double qq = q[2 * (size_t) e + d];
+    if (!is) {
+      is = 1;
+      maxq = qq;
+      mi = (c & inexact_bit(d)) ? 1u : 0u;
+    } else if (qq > maxq) {
+      maxq = qq;
+      mi = (c & inexact_bit(d)) ? 1u : 0u;
+    }
+  }
+  cand[i] = is;
+  vq[i] = is ? maxq : 0.0;
+  if (inex) inex[i] = mi;
+}
+
+// The comparisons an FP32 goodness takes part in, certified or not: a
+// collapsing vertex needs exact goodness (mark[v] = 1) if one of its
+// directions within the margin of its best is inexact while another is
+// also within it (the destination is the first direction equal to the
+// best, collapses_to_verts.cpp:86-101), or if a neighbouring candidate's
+// best lies within the margin of its own and either is inexact (the
+// independent set compares them with == and <, indset.cpp:72-83).  FP32
+// values are within half the margin of their FP64 values, so every
+// comparison left unmarked has the FP64 outcome.
+__global__ __launch_bounds__(BLOCK) void coarsen_certify(u32 nv, u32 const* __restrict__ uo,
+                                                         u32 const* __restrict__ ua, u32 const* __restrict__ ud,
+                                                         u32 const* __restrict__ cd, double const* __restrict__ q,
+                                                         u32 const* __restrict__ so, u32 const* __restrict__ sa,
+                                                         u32 const* __restrict__ cand, double const* __restrict__ vq,
+                                                         u32 const* __restrict__ inex, u32* __restrict__ mark) {
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
+  if (i >= nv) return;
+  u32 mk = 0;
+  if (cand[i]) {
+    double const m = (double) SCREEN_MARGIN;
+    double const best = vq[i];
+    u32 near = 0, near_inex = 0;
+    for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
+      u32 e = ua[j], d = ud[j];
+      u32 c = cd[e];
+      if (!collapses(c, d)) continue;
+      double qq = q[2 * (size_t) e + d];
+      if (!(qq < best - m)) {
+        ++near;
+        if (c & inexact_bit(d)) near_inex = 1;
+      }
+    }
+    if (near > 1 && near_inex) mk = 1;
+    u32 const mine = inex[i];
+    for (u32 k = so[i]; !mk && k < so[i + 1]; ++k) {
+      u32 o = sa[k];
+      if (!cand[o]) continue;
+      double dd = vq[o] - best;
+      if (!(dd > m || dd < -m) && (mine || inex[o])) mk = 1;
+    }
+  }
+  mark[i] = mk;
 }
 
 // coarsen_size_code (coarsen.cpp:13-21), fused with measure_edge
@@ -1020,6 +1138,40 @@ __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
   if (a.out_ci) a.out_ci[o] = a.ci[i];
 }
 
+static char const* cq_name(u32 mode) {
+  return mode == CQ_F64 ? "coarsen_quality_f64"
+                        : mode == CQ_SCREEN ? "coarsen_quality_mixed"
+                                            : mode == CQ_CERTIFIED ? "coarsen_quality_certified" : "coarsen_quality_force";
+}
+
+template <u32 DIM, u32 MODE>
+static void cq_launch(u32 g, CqArgs const& a, double bytes, hipEvent_t ea, hipEvent_t eb) {
+  dim3 grid(nblocks((size_t) a.ncand * g));
+  char const* kn = cq_name(MODE);
+  auto one = [&](auto kernel) {
+    if (ea) {
+      hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, a);
+      check_launch(kn);
+    } else {
+      launch_timed(kn, bytes, kernel, grid, dim3(BLOCK), a);
+    }
+  };
+  if (g == 1) one(coarsen_quality_f64<DIM, 1, MODE>);
+  else if (g == 2) one(coarsen_quality_f64<DIM, 2, MODE>);
+  else if (g == 4) one(coarsen_quality_f64<DIM, 4, MODE>);
+  else one(coarsen_quality_f64<DIM, 8, MODE>);
+}
+
+template <u32 DIM>
+static void cq_launch_mode(u32 mode, u32 g, CqArgs const& a, double bytes, hipEvent_t ea, hipEvent_t eb) {
+  switch (mode) {
+    case CQ_F64: cq_launch<DIM, CQ_F64>(g, a, bytes, ea, eb); break;
+    case CQ_SCREEN: cq_launch<DIM, CQ_SCREEN>(g, a, bytes, ea, eb); break;
+    case CQ_CERTIFIED: cq_launch<DIM, CQ_CERTIFIED>(g, a, bytes, ea, eb); break;
+    default: cq_launch<DIM, CQ_FORCE>(g, a, bytes, ea, eb); break;
+  }
+}
+
 static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better,
                           double const* qfe = nullptr) {
   check_index_space(m);
@@ -1051,62 +1203,56 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   if (require_better) eq = element_qualities(dim, count(m, dim), vel, coords(m), count(m, 0), metric_of(m));
   Buf<double> cq((size_t) 2 * ne);
   u32 tag = next_any_tag();
+  u32 cq_mode = CQ_F64, cq_g = 1;
+  CqArgs cq_args{};
+  double cq_bytes = 0;
   {
     double prof_bytes = (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28;
-    // variants, chosen online per size class: lanes per edge (1, 2, 4 or 8)
-    // and, in the mixed mode, whether the FP32 screen runs (it pays only
-    // where it rejects enough candidates); all give identical results
-    static thread_local Tuner tuners[2][2] = {
-        {Tuner("coarsen_quality_2d_f64", {1, 2, 4, 8}), Tuner("coarsen_quality_2d_mixed", {1, 2, 4, 8, 17, 18, 20, 24})},
-        {Tuner("coarsen_quality_3d_f64", {1, 2, 4, 8}), Tuner("coarsen_quality_3d_mixed", {1, 2, 4, 8, 17, 18, 20, 24})}};
-    Tuner& tuner = tuners[dim == 3][g_mixed ? 1 : 0];
-    ScreenStats sst = screen_stats();
-    u32 const* uoff = ue.offsets.get();
-    u32 const* uadj = ue.adj.get();
-    u32 const* udir = ue.dirs.get();
-    double const* xs = coords(m);
-    double const* mts = metric_of(m);
-    u32 const* cl = clist.get();
-    auto go = [&](auto kernel, char const* kn, u32 g, hipEvent_t ea, hipEvent_t eb) {
-      dim3 grid(nblocks((size_t) ncand * g));
-      if (ea) {
-        hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ncand, cl, ev, uoff, uadj, udir, vel,
-                              xs, mts, eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
-        check_launch(kn);
-      } else {
-        launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ncand, cl, ev, uoff, uadj, udir, vel, xs, mts,
-                     eq.get(), qual_floor, qfe, codes.get(), cq.get(), scratch_words(), tag, sst);
-      }
-    };
+    // variants, chosen online per size class: lanes per edge (1, 2, 4 or
+    // 8) and, in the mixed mode, the FP32 screen (+16) or the certified FP32
+    // pass (+32); all give identical results.  The mixed mode takes the
+    // certified pass from CQ_CERTIFIED_MIN candidates up (below that its
+    // three extra small launches cost more than the FP64 work it saves),
+    // lanes per edge tuned; the screen stays available to pin.
+    static thread_local Tuner tuners[2][3] = {
+        {Tuner("coarsen_quality_2d_f64", {1, 2, 4, 8}),
+         Tuner("coarsen_quality_2d_mixed", {33, 34, 36, 40, 17, 18, 20, 24}),
+         Tuner("coarsen_quality_2d_mixed_small", {1, 2, 4, 8})},
+        {Tuner("coarsen_quality_3d_f64", {1, 2, 4, 8}),
+         Tuner("coarsen_quality_3d_mixed", {33, 34, 36, 40, 17, 18, 20, 24}),
+         Tuner("coarsen_quality_3d_mixed_small", {1, 2, 4, 8})}};
+    constexpr u32 CQ_CERTIFIED_MIN = 4096;
+    bool const big = ncand >= CQ_CERTIFIED_MIN || tuners[dim == 3][1].forced();
+    Tuner& tuner = tuners[dim == 3][g_mixed ? (big ? 1 : 2) : 0];
+    CqArgs a{};
+    a.ncand = ncand;
+    a.list = clist.get();
+    a.ev = ev;
+    a.uo = ue.offsets.get();
+    a.ua = ue.adj.get();
+    a.ud = ue.dirs.get();
+    a.vel = vel;
+    a.x = coords(m);
+    a.mt = metric_of(m);
+    a.eq = eq.get();
+    a.qual_floor = qual_floor;
+    a.qfe = qfe;
+    a.codes = codes.get();
+    a.out = cq.get();
+    a.vmark = nullptr;
+    a.words = scratch_words();
+    a.tag = tag;
+    a.st = screen_stats();
     tuned(tuner, ncand, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
       u32 g = var & 15u;
-      bool screen = (var & 16u) != 0;
-      char const* kf = "coarsen_quality_f64";
-      char const* km = "coarsen_quality_mixed";
-      if (dim == 3 && screen) {
-        if (g == 1) go(coarsen_quality_f64<3, 1, true>, km, g, ea, eb);
-        else if (g == 2) go(coarsen_quality_f64<3, 2, true>, km, g, ea, eb);
-        else if (g == 4) go(coarsen_quality_f64<3, 4, true>, km, g, ea, eb);
-        else go(coarsen_quality_f64<3, 8, true>, km, g, ea, eb);
-      } else if (dim == 3) {
-        if (g == 1) go(coarsen_quality_f64<3, 1, false>, kf, g, ea, eb);
-        else if (g == 2) go(coarsen_quality_f64<3, 2, false>, kf, g, ea, eb);
-        else if (g == 4) go(coarsen_quality_f64<3, 4, false>, kf, g, ea, eb);
-        else go(coarsen_quality_f64<3, 8, false>, kf, g, ea, eb);
-      } else if (screen) {
-        if (g == 1) go(coarsen_quality_f64<2, 1, true>, km, g, ea, eb);
-        else if (g == 2) go(coarsen_quality_f64<2, 2, true>, km, g, ea, eb);
-        else if (g == 4) go(coarsen_quality_f64<2, 4, true>, km, g, ea, eb);
-        else go(coarsen_quality_f64<2, 8, true>, km, g, ea, eb);
-      } else {
-        if (g == 1) go(coarsen_quality_f64<2, 1, false>, kf, g, ea, eb);
-        else if (g == 2) go(coarsen_quality_f64<2, 2, false>, kf, g, ea, eb);
-        else if (g == 4) go(coarsen_quality_f64<2, 4, false>, kf, g, ea, eb);
-        else go(coarsen_quality_f64<2, 8, false>, kf, g, ea, eb);
-      }
+      u32 mode = (var & 32u) ? (u32) CQ_CERTIFIED : (var & 16u) ? (u32) CQ_SCREEN : (u32) CQ_F64;
+      cq_mode = mode;
+      cq_g = g;
+      dim == 3 ? cq_launch_mode<3>(mode, g, a, prof_bytes, ea, eb) : cq_launch_mode<2>(mode, g, a, prof_bytes, ea, eb);
     });
+    cq_args = a;
+    cq_bytes = prof_bytes;
   }
-  eq.release();
   // no host check here: if the quality check left no collapse, the
   // candidates below are empty, the independent set settles at once and the
   // one read of its total returns 0 (check_coarsen_quality's early return)
@@ -1116,32 +1262,34 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   Up const& uv = ask_up(m, 0, 1);
   Buf<u32> cand(nv);
   Buf<double> vq(nv);
-  {
-    u32 const* uo = uv.offsets.get();
-    u32 const* ua = uv.adj.get();
-    u32 const* ud = uv.dirs.get();
-    u32 const* cd = codes.get();
-    double const* q = cq.get();
-    u32* ca = cand.get();
-    double* vqp = vq.get();
-    parallel_for(nv, OHX_LAMBDA(u32 i) {
-      double maxq = 0;
-      u32 is = 0;
-      for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
-        u32 e = ua[j], d = ud[j];
-        if (!collapses(cd[e], d)) continue;
-        double qq = q[2 * (size_t) e + d];
-        if (!is) {
-          is = 1;
-          maxq = qq;
-        } else if (qq > maxq) {
-          maxq = qq;
-        }
-      }
-      ca[i] = is;
-      vqp[i] = is ? maxq : 0.0;
-    });
+  bool const certified = cq_mode == CQ_CERTIFIED && !require_better;
+  Buf<u32> vinex, vmark;
+  if (certified) vinex = Buf<u32>(nv);
+  auto to_vert = [&] {
+    if (!nv) return;
+    hipLaunchKernelGGL(collapse_to_vert_k, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
+                       uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), cand.get(), vq.get(),
+                       certified ? vinex.get() : nullptr);
+    check_launch("collapse_to_vert");
+  };
+  to_vert();
+  if (certified && nv) {
+    // the comparisons the FP32 goodness takes part in: the vertices whose
+    // choices are not certified get exact values (their inexact
+    // directions measured in FP64), then their best again
+    Graph const& sg = ask_star(m, 0, dim);
+    vmark = Buf<u32>(nv);
+    hipLaunchKernelGGL(coarsen_certify, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
+                       uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), sg.offsets.get(), sg.adj.get(),
+                       cand.get(), vq.get(), vinex.get(), vmark.get());
+    check_launch("coarsen_certify");
+    CqArgs f = cq_args;
+    f.vmark = vmark.get();
+    dim == 3 ? cq_launch_mode<3>(CQ_FORCE, cq_g, f, cq_bytes, nullptr, nullptr)
+             : cq_launch_mode<2>(CQ_FORCE, cq_g, f, cq_bytes, nullptr, nullptr);
+    to_vert();
   }
+  eq.release();
   // a first batch of indset rounds, the collapse destinations and the
   // vertex offsets, then one read of the convergence flag and the total
   u32 const* flag = nullptr;

@@ -332,6 +332,8 @@ struct Tuner {
   u32 pick(u32 n, hipEvent_t* a, hipEvent_t* b);
   void harvest();
   int chosen(u32 n) const;
+  // pinned to a variant by tuning_force
+  bool forced() const;
 };
 // run launch(variant, start_event, stop_event) with the tuner's choice
 template <class L>

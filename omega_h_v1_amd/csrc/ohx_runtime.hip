@@ -478,6 +478,8 @@ void Tuner::harvest() {
   }
 }
 
+bool Tuner::forced() const { return forced_variants().count(name) != 0; }
+
 u32 Tuner::pick(u32 n, hipEvent_t* a, hipEvent_t* b) {
   if (std::find(tuners().begin(), tuners().end(), this) == tuners().end()) tuners().push_back(this);
   auto f = forced_variants().find(name);

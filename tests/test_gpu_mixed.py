@@ -17,8 +17,15 @@ import parity_cases as P
 pytestmark = pytest.mark.gpu
 
 MARGIN = 1e-4
-SCREENED = {"coarsen_quality_2d_mixed": 18, "coarsen_quality_3d_mixed": 24,
-            "refine_quality_2d_mixed": 1, "refine_quality_3d_mixed": 1}
+# the FP32 variants pinned on: the screen (FP32 rejects, FP64 for the rest)
+# or the certified pass (FP32 settles every decision it can certify, FP64
+# only near a threshold or a competing goodness); lanes per edge alternate
+MODES = {
+    "screen": {"coarsen_quality_2d_mixed": 18, "coarsen_quality_3d_mixed": 24,
+               "refine_quality_2d_mixed": 1, "refine_quality_3d_mixed": 1, "swap_quality_3d_mixed": 1},
+    "certified": {"coarsen_quality_2d_mixed": 36, "coarsen_quality_3d_mixed": 34,
+                  "refine_quality_2d_mixed": 1, "refine_quality_3d_mixed": 1, "swap_quality_3d_mixed": 1},
+}
 
 
 @pytest.fixture(scope="module")
@@ -31,14 +38,17 @@ def product():
     d.ohx_mixed_stats.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
     d.ohx_tuning_force.argtypes = [C.c_char_p, C.c_int]
     d.ohx_set_precision(1)
-    # the FP32 screen is a tuned variant: pin it on, so that every decision
-    # here goes through it (lanes per edge alternate between tests)
-    for name, v in SCREENED.items():
-        d.ohx_tuning_force(name.encode(), v)
     yield lib
-    for name in SCREENED:
+    for name in MODES["screen"]:
         d.ohx_tuning_force(name.encode(), -1)
     d.ohx_set_precision(0)
+
+
+def _pin(lib, mode):
+    # the FP32 variants are tuned choices: pin one on, so that every
+    # decision here goes through it
+    for name, v in MODES[mode].items():
+        lib.dll.ohx_tuning_force(name.encode(), v)
 
 
 @pytest.fixture(scope="module")
@@ -60,8 +70,10 @@ CASES.update(P.ANISO_CASES)
 CASES["c2_n256"] = lambda lib, dev=False: P.c2_steps(lib, on_device=dev)
 
 
+@pytest.mark.parametrize("mode", sorted(MODES))
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_mixed_decisions_equal_fp64(case, product, oracle):
+def test_mixed_decisions_equal_fp64(case, mode, product, oracle):
+    _pin(product, mode)
     product.dll.ohx_mixed_stats_reset()
     n = 0
     for (la, ma), (lb, mb) in zip(CASES[case](product, True), CASES[case](oracle, False)):
@@ -79,6 +91,7 @@ def test_mixed_decisions_equal_fp64(case, product, oracle):
 
 def test_mixed_stats_account_for_every_candidate(product):
     """On C1 the screen sees candidates, and the counters reset."""
+    _pin(product, "screen")
     product.dll.ohx_mixed_stats_reset()
     assert _stats(product) == (0.0, 0, 0)
     for _ in P.c1_steps(product, True):
