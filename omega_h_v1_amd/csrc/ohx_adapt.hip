@@ -867,7 +867,7 @@ __device__ inline double collapse_elem_q(u32 e, u32 dir, u32 gen, V3 const& pg, 
 
 struct CqArgs {
   u32 ncand;
-  u32 const* list;  // candidate edges, compacted
+  u32 const* list;  // candidate directions 2 e + j, grouped by collapsing vertex
   u32 const* ev;
   u32 const* uo;    // up(0, dim)
   u32 const* ua;
@@ -881,15 +881,19 @@ struct CqArgs {
   u32* codes;
   double* out;
   u32 const* vmark;  // CQ_FORCE: vertices that need exact goodness
-  u32* words;
-  u32 tag;
   ScreenStats st;
 };
 
+// One collapse direction per group of G lanes.  The directions come
+// grouped by collapsing vertex (the list is built from each vertex's edges
+// in turn), so the groups of one block share the vertex's element star and
+// its vertices in the caches: measured on C3, the edge-ordered form fetched
+// about seven times the arrays' size from beyond L2 per launch.  Both
+// directions of an edge may be in flight at once, so the code bits are
+// cleared (or the inexact bit set) with atomics on the edge's word.
 template <u32 DIM, u32 G, u32 MODE>
 __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(CqArgs A) {
-  u32 const ncand = A.ncand, tag = A.tag;
-  u32 const* __restrict__ list = A.list;
+  u32 const ncand = A.ncand;
   u32 const* __restrict__ ev = A.ev;
   u32 const* __restrict__ uo = A.uo;
   u32 const* __restrict__ ua = A.ua;
@@ -898,91 +902,82 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(CqArgs A) {
   double const* __restrict__ x = A.x;
   double const* __restrict__ mt = A.mt;
   double const* __restrict__ eq = A.eq;
-  double const qual_floor_all = A.qual_floor;
-  double const* __restrict__ qfe = A.qfe;
   u32* __restrict__ codes = A.codes;
   double* __restrict__ out = A.out;
-  u32 const* __restrict__ vmark = A.vmark;
-  u32* __restrict__ words = A.words;
   ScreenStats const st = A.st;
   size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
   size_t slot = gid / G;
   u32 lane = (u32) (gid % G);
   if (slot >= ncand) return;  // whole groups leave together
-  // the candidate edges, compacted
-  u32 const i = list[slot];
-  u32 c = codes[i];
-  if (c == DONT_COLLAPSE) return;
+  u32 const ed = A.list[slot];
+  u32 const i = ed >> 1, j = ed & 1u;
+  u32 const col = ev[2 * (size_t) i + j], gen = ev[2 * (size_t) i + (1 - j)];
+  if (MODE == CQ_FORCE && !((codes[i] & inexact_bit(j)) && A.vmark[col])) return;
   // one floor for the mesh, or its component's in a batch
-  double const qual_floor = qfe ? qfe[i] : qual_floor_all;
+  double const qual_floor = A.qfe ? A.qfe[i] : A.qual_floor;
   bool const f32 = (MODE == CQ_SCREEN || MODE == CQ_CERTIFIED) && !eq;
-  for (u32 j = 0; j < 2; ++j) {
-    if (!collapses(c, j)) continue;
-    u32 col = ev[2 * i + j], gen = ev[2 * i + (1 - j)];
-    if (MODE == CQ_FORCE && !((c & inexact_bit(j)) && vmark[col])) continue;
-    V3 const pg = ld3(x + 3 * (size_t) gen);
-    Sym mg;
-    if (mt) mg = ld6(mt + 6 * (size_t) gen);
-    double minq32 = 1;
-    if (f32) {
-      // FP32: a direction certainly below the floor is rejected at its first
-      // element below it; CQ_CERTIFIED also accepts one certainly above
-      bool nan = false;
-      double const lo = qual_floor - (double) SCREEN_MARGIN;
-      for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
-        bool adj;
-        double q = collapse_elem_q<DIM, true>(ua[k], ud[k], gen, pg, mg, vel, x, mt, &adj);
-        if (adj) continue;
-        if (!(q == q)) nan = true;
-        if (q < minq32) minq32 = q;
-        if (minq32 < lo) break;
-      }
-      minq32 = group_min<G>(minq32);
-#pragma unroll
-      for (u32 d = G / 2; d >= 1; d >>= 1) nan = nan || __shfl_xor((int) nan, d, G);
-      bool reject = !nan && minq32 < lo;
-      bool accept = MODE == CQ_CERTIFIED && !nan && minq32 > qual_floor + (double) SCREEN_MARGIN;
-      if (lane == 0) wave_count(st.screened, reject || accept);
-      if (reject) {
-        c = dont_collapse(c, j);
-        continue;
-      }
-      if (accept) {
-        c |= inexact_bit(j);
-        if (lane == 0) out[2 * i + j] = minq32;
-        continue;
-      }
-      if (lane == 0) wave_count(st.reeval, true);
-    }
-    if (MODE == CQ_FORCE && lane == 0) wave_count(st.reeval, true);
-    double minq = 1, old_minq = 1;
+  V3 const pg = ld3(x + 3 * (size_t) gen);
+  Sym mg;
+  if (mt) mg = ld6(mt + 6 * (size_t) gen);
+  double minq32 = 1;
+  if (f32) {
+    // FP32: a direction certainly below the floor is rejected at its first
+    // element below it; CQ_CERTIFIED also accepts one certainly above
+    bool nan = false;
+    double const lo = qual_floor - (double) SCREEN_MARGIN;
     for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
-      u32 e = ua[k];
-      if (eq && eq[e] < old_minq) old_minq = eq[e];
       bool adj;
-      double q = collapse_elem_q<DIM, false>(e, ud[k], gen, pg, mg, vel, x, mt, &adj);
+      double q = collapse_elem_q<DIM, true>(ua[k], ud[k], gen, pg, mg, vel, x, mt, &adj);
       if (adj) continue;
-      if (q < minq) minq = q;
-      // below the floor the direction is rejected whatever the remaining
-      // elements give (the minimum only falls), so the lane stops; the
-      // screened kernels keep the full minimum for their error statistic
-      if (MODE == CQ_F64 && minq < qual_floor) break;
+      if (!(q == q)) nan = true;
+      if (q < minq32) minq32 = q;
+      if (minq32 < lo) break;
     }
-    minq = group_min<G>(minq);
-    if (eq) old_minq = group_min<G>(old_minq);
-    if ((f32 || MODE == CQ_FORCE) && lane == 0 && minq == minq) {
-      double d = MODE == CQ_FORCE ? minq - out[2 * i + j] : minq - minq32;
-      note_max_err(st.max_err, (float) (d < 0 ? -d : d));
+    minq32 = group_min<G>(minq32);
+#pragma unroll
+    for (u32 d = G / 2; d >= 1; d >>= 1) nan = nan || __shfl_xor((int) nan, d, G);
+    bool reject = !nan && minq32 < lo;
+    bool accept = MODE == CQ_CERTIFIED && !nan && minq32 > qual_floor + (double) SCREEN_MARGIN;
+    if (lane == 0) wave_count(st.screened, reject || accept);
+    if (reject) {
+      if (lane == 0) atomicAnd(&codes[i], ~((1u << j) | inexact_bit(j)));
+      return;
     }
-    c &= ~inexact_bit(j);
-    if ((minq < qual_floor) || (eq && (minq <= old_minq)))
-      c = dont_collapse(c, j) & ~inexact_bit(j);
-    else if (lane == 0)
-      out[2 * i + j] = minq;
+    if (accept) {
+      if (lane == 0) {
+        out[2 * (size_t) i + j] = minq32;
+        atomicOr(&codes[i], inexact_bit(j));
+      }
+      return;
+    }
+    if (lane == 0) wave_count(st.reeval, true);
   }
-  if (lane == 0) {
-    codes[i] = c;
-    wave_flag_value(words + W_ANY, (c & 3u) != DONT_COLLAPSE, tag);
+  if (MODE == CQ_FORCE && lane == 0) wave_count(st.reeval, true);
+  double minq = 1, old_minq = 1;
+  for (u32 k = uo[col] + lane; k < uo[col + 1]; k += G) {
+    u32 e = ua[k];
+    if (eq && eq[e] < old_minq) old_minq = eq[e];
+    bool adj;
+    double q = collapse_elem_q<DIM, false>(e, ud[k], gen, pg, mg, vel, x, mt, &adj);
+    if (adj) continue;
+    if (q < minq) minq = q;
+    // below the floor the direction is rejected whatever the remaining
+    // elements give (the minimum only falls), so the lane stops; the
+    // screened kernels keep the full minimum for their error statistic
+    if (MODE == CQ_F64 && minq < qual_floor) break;
+  }
+  minq = group_min<G>(minq);
+  if (eq) old_minq = group_min<G>(old_minq);
+  if (lane != 0) return;
+  if ((f32 || MODE == CQ_FORCE) && minq == minq) {
+    double d = MODE == CQ_FORCE ? minq - out[2 * (size_t) i + j] : minq - minq32;
+    note_max_err(st.max_err, (float) (d < 0 ? -d : d));
+  }
+  if ((minq < qual_floor) || (eq && (minq <= old_minq))) {
+    atomicAnd(&codes[i], ~((1u << j) | inexact_bit(j)));
+  } else {
+    out[2 * (size_t) i + j] = minq;
+    if (MODE == CQ_FORCE) atomicAnd(&codes[i], ~inexact_bit(j));
   }
 }
 
@@ -1179,21 +1174,33 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   u32 ne = count(m, 1);
   check_basic_class(m, codes.get());
   for (u32 d = 2; d <= dim; ++d) check_exposed_sides(m, d, codes.get());
-  // the edges with a collapse direction left, compacted: the quality
-  // kernel's groups then all have work (the count, read once, is also the
-  // early return when the class checks leave nothing)
+  // the collapse directions left, listed vertex by vertex (each vertex's
+  // edges in up(0,1) order): the quality kernel's groups then all have
+  // work, and neighbouring groups share a vertex's star (the count, read
+  // once, is also the early return when the class checks leave nothing)
   u32 ncand;
   Buf<u32> clist;
+  u32 const nv0 = count(m, 0);
   {
-    Buf<u32> coff(ne + 1);
+    Up const& uv = ask_up(m, 0, 1);
+    u32 const* uo = uv.offsets.get();
+    u32 const* ua = uv.adj.get();
+    u32 const* ud = uv.dirs.get();
     u32 const* cp = codes.get();
-    ncand = exscan_fn_total(ne, OHX_LAMBDA(u32 i) { return cp[i] != DONT_COLLAPSE ? 1u : 0u; }, coff.get());
+    Buf<u32> voff(nv0 + 1);
+    ncand = exscan_fn_total(nv0, OHX_LAMBDA(u32 v) {
+      u32 c = 0;
+      for (u32 k = uo[v]; k < uo[v + 1]; ++k) c += collapses(cp[ua[k]], ud[k]);
+      return c;
+    }, voff.get());
     if (!ncand) return 0;
     clist = Buf<u32>(ncand);
-    u32 const* co = coff.get();
+    u32 const* vo = voff.get();
     u32* cl = clist.get();
-    parallel_for(ne, OHX_LAMBDA(u32 i) {
-      if (cp[i] != DONT_COLLAPSE) cl[co[i]] = i;
+    parallel_for(nv0, OHX_LAMBDA(u32 v) {
+      u32 o = vo[v];
+      for (u32 k = uo[v]; k < uo[v + 1]; ++k)
+        if (collapses(cp[ua[k]], ud[k])) cl[o++] = 2 * ua[k] + ud[k];
     });
   }
   Buf<double> eq;
@@ -1240,8 +1247,6 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     a.codes = codes.get();
     a.out = cq.get();
     a.vmark = nullptr;
-    a.words = scratch_words();
-    a.tag = tag;
     a.st = screen_stats();
     tuned(tuner, ncand, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
       u32 g = var & 15u;
