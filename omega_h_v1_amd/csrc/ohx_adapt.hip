@@ -741,74 +741,82 @@ static void check_basic_class(Mesh& m, u32* codes) {
   });
 }
 
-// check_for_exposed_side (check_collapse_class.cpp:57-140)
+// check_for_exposed_side (check_collapse_class.cpp:57-140), from the
+// entities' side: every face (ED = 2) or tet (ED = 3) visits its edges that
+// still have a collapse direction and clears the directions it forbids.
+// The reference walks each edge's upward adjacency instead; the result is
+// an AND over the adjacent entities either way, so the order does not
+// matter, and the upward maps up(1,2), up(1,3) and the reflection
+// down(3,1) that the walk needs are not built here (a tet's edges come
+// from its faces' edges: the edge (a, b) lies in the face opposite a third
+// vertex).
+template <u32 ED>
+__global__ __launch_bounds__(BLOCK) void exposed_sides_k(u32 n, u32 const* __restrict__ vent,
+                                                         u32 const* __restrict__ sent,
+                                                         u32 const* __restrict__ e_of_f,
+                                                         u32 const* __restrict__ v_of_f,
+                                                         u32 const* __restrict__ ev, u32 const* __restrict__ ecd,
+                                                         u32 const* __restrict__ scd, u32* __restrict__ codes) {
+  u32 E = xcd_block() * BLOCK + threadIdx.x;
+  if (E >= n) return;
+  constexpr u32 VPE = ED + 1, SPE = ED + 1, NE = ED == 3 ? 6 : 3;
+  u32 v[VPE], sd[SPE];
+#pragma unroll
+  for (u32 k = 0; k < VPE; ++k) v[k] = vent[(size_t) E * VPE + k];
+#pragma unroll
+  for (u32 k = 0; k < SPE; ++k) sd[k] = sent[(size_t) E * SPE + k];
+  u32 const cE = ecd[E];
+  u32 const* so = opp(ED, 0);
+#pragma unroll
+  for (u32 le = 0; le < NE; ++le) {
+    u32 const* lv = canon(ED, 1, 0, le);
+    u32 edge;
+    if (ED == 2) {
+      edge = e_of_f[(size_t) E * 3 + le];
+    } else {
+      // the face opposite the first tet vertex off the edge holds it
+      u32 c3 = (lv[0] != 0 && lv[1] != 0) ? 0u : ((lv[0] != 1 && lv[1] != 1) ? 1u : 2u);
+      u32 f = sd[so[c3]];
+      u32 const* fv = v_of_f + 3 * (size_t) f;
+      u32 va = v[lv[0]], vb = v[lv[1]];
+      u32 m = 0;
+#pragma unroll
+      for (u32 q = 0; q < 3; ++q) {
+        u32 a = fv[q], b = fv[(q + 1) % 3];
+        if ((a == va && b == vb) || (a == vb && b == va)) m = q;
+      }
+      edge = e_of_f[3 * (size_t) f + m];
+    }
+    u32 c = codes[edge];
+    if (c == DONT_COLLAPSE) continue;
+    u32 mask = 0;
+    for (u32 k = 0; k < 2; ++k) {
+      if (!collapses(c, k)) continue;
+      u32 va = ev[2 * (size_t) edge + (1 - k)];
+      u32 l = v[lv[0]] == va ? 0u : 1u;
+      if (scd[sd[so[lv[l]]]] != cE) mask |= 1u << k;
+    }
+    if (mask) atomicAnd(&codes[edge], ~mask);
+  }
+}
+
 static void check_exposed_sides(Mesh& m, u32 ed, u32* codes) {
-  Up const& u = ask_up(m, 1, ed);
+  u32 n = count(m, ed);
+  if (!n) return;
   u32 const* vent = ask_down(m, ed, 0);
-  u32 const* ev = ask_down(m, 1, 0);
   u32 const* sent = ask_down(m, ed, ed - 1);
+  u32 const* e_of_f = ask_down(m, 2, 1);
+  u32 const* v_of_f = ask_down(m, 2, 0);
+  u32 const* ev = ask_down(m, 1, 0);
   u32 const* ecd = u32_tag(m, ed, "class_dim");
   u32 const* scd = u32_tag(m, ed - 1, "class_dim");
-  u32 vpe = ed + 1, spe = h_down_deg[ed][ed - 1];
-  u32 const* uo = u.offsets.get();
-  u32 const* ua = u.adj.get();
-  u32 const* ud = u.dirs.get();
-  u32* err = dev_error_word();
-  // the edge's adjacent entities are taken CH at a time with their reads
-  // issued level by level (entities, then their vertices and sides, then the
-  // class dimensions), so a thread waits on three round trips per chunk
-  // rather than on a chain per entity; the decisions then run in the
-  // reference's order on the loaded values
-  parallel_for(count(m, 1), OHX_LAMBDA(u32 i) {
-    u32 c = codes[i];
-    if (c == DONT_COLLAPSE) return;
-    u32 const* so = opp(ed, 0);
-    u32 const ve[2] = {ev[2 * (size_t) i], ev[2 * (size_t) i + 1]};
-    u32 const f = uo[i], e = uo[i + 1];
-    constexpr u32 CH = 8;
-    for (u32 j0 = f; j0 < e; j0 += CH) {
-      u32 ent[CH], lv[CH][2], vb[CH][2], sd[CH][2], sc[CH][2], ec[CH];
-#pragma unroll
-      for (u32 q = 0; q < CH; ++q) {
-        ent[q] = 0;
-        lv[q][0] = lv[q][1] = 0;
-        if (j0 + q < e) {
-          ent[q] = ua[j0 + q];
-          u32 const* eve = canon(ed, 1, 0, ud[j0 + q]);
-          lv[q][0] = eve[0];
-          lv[q][1] = eve[1];
-        }
-      }
-#pragma unroll
-      for (u32 q = 0; q < CH; ++q)
-#pragma unroll
-        for (u32 l = 0; l < 2; ++l) {
-          vb[q][l] = j0 + q < e ? vent[(size_t) ent[q] * vpe + lv[q][l]] : INVALID;
-          sd[q][l] = j0 + q < e ? sent[(size_t) ent[q] * spe + so[lv[q][l]]] : 0;
-        }
-#pragma unroll
-      for (u32 q = 0; q < CH; ++q) {
-        sc[q][0] = j0 + q < e ? scd[sd[q][0]] : 0;
-        sc[q][1] = j0 + q < e ? scd[sd[q][1]] : 0;
-        ec[q] = j0 + q < e ? ecd[ent[q]] : 0;
-      }
-#pragma unroll
-      for (u32 q = 0; q < CH; ++q) {
-        if (j0 + q >= e) break;
-        for (u32 k = 0; k < 2; ++k) {
-          if (!collapses(c, k)) continue;
-          u32 va = ve[1 - k];
-          u32 l = vb[q][0] == va ? 0u : (vb[q][1] == va ? 1u : 2u);
-          if (l == 2) {
-            *err = ERR_EXPOSED_SIDE;
-          } else if (sc[q][l] != ec[q]) {
-            c = dont_collapse(c, k);
-          }
-        }
-      }
-    }
-    codes[i] = c;
-  });
+  if (ed == 2)
+    hipLaunchKernelGGL(exposed_sides_k<2>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, vent, sent, e_of_f, v_of_f,
+                       ev, ecd, scd, codes);
+  else
+    hipLaunchKernelGGL(exposed_sides_k<3>, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), n, vent, sent, e_of_f, v_of_f,
+                       ev, ecd, scd, codes);
+  check_launch("exposed_sides");
 }
 
 // coarsen_quality (coarsen_qualities.cpp:12-79)
@@ -2285,6 +2293,8 @@ bool incr_op(Mesh& m, Ctx& c) {  // adapt.cpp:36-45
 enum Kind { REFINE, COARSEN, SWAP, COARSEN_SLIVERS };
 u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n) {
   double tin = count(m, m.dim);
+  static bool const trace = std::getenv("OHX_TRACE_DERIVE") && std::getenv("OHX_TRACE_DERIVE")[0] == '1';
+  if (trace) std::fprintf(stderr, "pass %s n=%.0f\n", k == REFINE ? "refine" : k == COARSEN ? "coarsen" : k == SWAP ? "swap" : "csliv", tin);
   hipEvent_t e0 = c.event(), e1 = c.event();
   OHX_CHECK(hipEventRecord(e0, stream()));
   u32 ok = 0;

@@ -963,10 +963,22 @@ void bridge_graph(u32 n, u32 const* offs, u32 const* adj, u32* p_nedges, Buf<u32
 }
 
 // ------------------------------------------------------------ lazy caches
+// OHX_TRACE_DERIVE=1 prints every derivation (kind, dimensions, size) to
+// stderr: which adjacencies each pass rebuilds
+static bool trace_derive() {
+  static int on = -1;
+  if (on < 0) {
+    char const* e = std::getenv("OHX_TRACE_DERIVE");
+    on = (e && e[0] == '1') ? 1 : 0;
+  }
+  return on == 1;
+}
+
 // mesh_ask_down (mesh.cpp:136-181)
 u32 const* ask_down(Mesh& m, u32 h, u32 l) {
   OHX_REQUIRE(h <= m.dim && l <= h, "ask_down: bad dimensions");
   if (m.down[h][l]) return m.down[h][l].get();
+  if (trace_derive()) std::fprintf(stderr, "derive down %u %u n=%u\n", h, l, m.counts[h]);
   if (l == h) {
     m.down[h][l] = iota(m.counts[h]);
   } else if (l > 0) {
@@ -1013,6 +1025,7 @@ u32 const* ask_down(Mesh& m, u32 h, u32 l) {
 // mesh_ask_up (mesh.cpp:190-217)
 Up const& ask_up(Mesh& m, u32 l, u32 h) {
   if (m.up[l][h]) return *m.up[l][h];
+  if (trace_derive()) std::fprintf(stderr, "derive up %u %u n=%u\n", l, h, m.counts[h]);
   auto u = std::make_unique<Up>();
   if (l == h) {
     u32 n = m.counts[h];
@@ -1030,6 +1043,7 @@ Up const& ask_up(Mesh& m, u32 l, u32 h) {
 // mesh_ask_star (mesh.cpp:226-244)
 Graph const& ask_star(Mesh& m, u32 l, u32 h) {
   if (m.star[l][h]) return *m.star[l][h];
+  if (trace_derive()) std::fprintf(stderr, "derive star %u %u n=%u\n", l, h, m.counts[l]);
   auto g = std::make_unique<Graph>();
   if (l == h) {
     g->offsets = filled(m.counts[l] + 1, 0);
