@@ -130,9 +130,9 @@ __global__ __launch_bounds__(BLOCK) void indset_round(u32 n, u32 const* __restri
 // host check, *last_flag then points at the device word that is nonzero if
 // entities were still undecided after the last round
 static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double const* good, u32 rounds,
-                              u32 const** last_flag) {
+                              u32 const** last_flag, Graph const* graph = nullptr) {
   u32 n = count(m, ent_dim);
-  Graph const& g = ask_star(m, ent_dim, m.dim);
+  Graph const& g = graph ? *graph : ask_star(m, ent_dim, m.dim);
   Buf<u32> a(n), b(n);
   constexpr u32 BATCH = 8;
   u32* flags = scratch_words() + W_FLAGS;
@@ -197,6 +197,39 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
 
 Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good) {
   return indset_rounds(m, ent_dim, filter, good, 0, nullptr);
+}
+
+// The edges of the tets around each edge, the edge itself left out (five
+// per use, repeats kept).  As a set this is the edge star star(1,3)
+// (get_edge_tet_star, star.cpp:202-278: the two other edges of every
+// triangle through the edge and the opposite edge of every tet through it,
+// which together are the tets' other edges), and an independent set depends
+// only on each entity's set of neighbours -- any neighbour in the set, any
+// live neighbour ahead of it -- not on their order or repeats.  It needs
+// only up(1,3) and down(3,1), which the swap pass has built already; the
+// star would also need up(1,2).
+static Graph edge_tet_graph(Mesh& m) {
+  Up const& u = ask_up(m, 1, 3);
+  u32 const* er = ask_down(m, 3, 1);
+  u32 ne = count(m, 1);
+  size_t nuse = u.adj.size();
+  if ((size_t) 5 * nuse >= ((size_t) 1 << 32)) fail("edge graph: more than 2^32 entries", __FILE__, __LINE__);
+  Graph g;
+  g.offsets = Buf<u32>(ne + 1);
+  g.adj = Buf<u32>(5 * nuse);
+  u32* go = g.offsets.get();
+  u32* ga = g.adj.get();
+  u32 const* uo = u.offsets.get();
+  u32 const* ua = u.adj.get();
+  u32 const* ud = u.dirs.get();
+  parallel_for(ne + 1, OHX_LAMBDA(u32 i) { go[i] = 5 * uo[i]; });
+  if (nuse)
+    parallel_for((u32) nuse, OHX_LAMBDA(u32 j) {
+      u32 t = ua[j], d = ud[j], k = 0;
+      for (u32 l = 0; l < 6; ++l)
+        if (l != d) ga[5 * (size_t) j + k++] = er[6 * (size_t) t + l];
+    });
+  return g;
 }
 
 // a first batch of rounds with no host check; if the set has not settled
@@ -2132,7 +2165,8 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   }
   eq.release();
   if (read_word(W_ANY) != tag) return 0;
-  Buf<u32> indset = find_indset(m, 1, cand.get(), q.get());
+  Graph eg = edge_tet_graph(m);
+  Buf<u32> indset = indset_rounds(m, 1, cand.get(), q.get(), 0, nullptr, &eg);
   if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();
   q.release();

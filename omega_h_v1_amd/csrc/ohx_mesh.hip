@@ -932,6 +932,70 @@ static Buf<u32> dual_from_sides(Mesh& m) {
   return out;
 }
 
+// The same dual, unique in a conforming mesh (at most two elements per
+// side), by pairing the elements' sides in a hash table keyed by the sorted
+// side vertices: the first of a pair takes a slot, the second finds it and
+// writes both entries; sides without a partner stay INVALID.  It needs
+// neither the sides' numbering (down(dim, dim - 1)) nor their upward map,
+// which a pass that only wants the dual would otherwise build.
+template <u32 DIM>
+__global__ __launch_bounds__(BLOCK) void dual_pairs_k(u32 nsides, u32 const* __restrict__ ev, u32 mask,
+                                                      u32* __restrict__ table, u32* __restrict__ dual) {
+  u32 t = xcd_block() * BLOCK + threadIdx.x;
+  if (t >= nsides) return;
+  constexpr u32 VPE = DIM + 1, SPE = DIM + 1;
+  u32 const i = t / SPE, j = t % SPE;
+  u32 const* sc = canon(DIM, DIM - 1, 0, j);
+  u32 key[DIM];
+#pragma unroll
+  for (u32 k = 0; k < DIM; ++k) key[k] = ev[(size_t) i * VPE + sc[k]];
+  sort_small(key);
+  u32 h = tuple_hash(key, mask);
+  for (u32 probe = 0; probe <= mask; ++probe) {
+    u32 prev = atomicCAS(&table[h], INVALID, t);
+    if (prev == INVALID) return;  // first of its pair: the partner writes both
+    u32 pi = prev / SPE;
+    u32 const* pc = canon(DIM, DIM - 1, 0, prev % SPE);
+    u32 pk[DIM];
+#pragma unroll
+    for (u32 k = 0; k < DIM; ++k) pk[k] = ev[(size_t) pi * VPE + pc[k]];
+    sort_small(pk);
+    bool same = true;
+#pragma unroll
+    for (u32 k = 0; k < DIM; ++k) same = same && pk[k] == key[k];
+    if (same) {
+      dual[t] = pi;
+      dual[prev] = i;
+      return;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+static Buf<u32> dual_by_pairs(Mesh& m) {
+  u32 dim = m.dim;
+  u32 ne = count(m, dim);
+  u32 spe = h_down_deg[dim][dim - 1];
+  size_t nsides = (size_t) ne * spe;
+  Buf<u32> out(nsides);
+  if (!nsides) return out;
+  OHX_REQUIRE(nsides < ((size_t) 1 << 31), "dual: too many element sides");
+  u32 cap = 1;
+  while (cap < 2 * nsides) cap <<= 1;
+  Buf<u32> table(cap);
+  fill(table.get(), cap, INVALID);
+  fill(out.get(), (u32) nsides, INVALID);
+  u32 const* ev = ask_down(m, dim, 0);
+  if (dim == 3)
+    hipLaunchKernelGGL(dual_pairs_k<3>, dim3(nblocks(nsides)), dim3(BLOCK), 0, stream(), (u32) nsides, ev, cap - 1,
+                       table.get(), out.get());
+  else
+    hipLaunchKernelGGL(dual_pairs_k<2>, dim3(nblocks(nsides)), dim3(BLOCK), 0, stream(), (u32) nsides, ev, cap - 1,
+                       table.get(), out.get());
+  check_launch("dual_pairs");
+  return out;
+}
+
 // bridge_graph_general (bridge_graph.cpp:12-80)
 void bridge_graph(u32 n, u32 const* offs, u32 const* adj, u32* p_nedges, Buf<u32>* ev, Buf<u32>* dirs) {
   Buf<u32> deg(n);
@@ -1058,10 +1122,13 @@ Graph const& ask_star(Mesh& m, u32 l, u32 h) {
 // mesh_ask_dual (mesh.cpp:252-261)
 u32 const* ask_dual(Mesh& m) {
   if (!m.dual) {
-    if (has_dim(m, m.dim - 1))
-      m.dual = dual_from_sides(m);
-    else
+    u32 d = m.dim;
+    if (!has_dim(m, d - 1))
       m.dual = dual_from_verts(m);
+    else if (m.down[d][d - 1] && m.up[d - 1][d])
+      m.dual = dual_from_sides(m);  // the side maps are there already
+    else
+      m.dual = dual_by_pairs(m);
   }
   return m.dual.get();
 }
