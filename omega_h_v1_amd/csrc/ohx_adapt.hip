@@ -1342,7 +1342,20 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   Buf<u32> indset = find_indset_bounded(m, 0, cand.get(), vq.get(), &flag);
   Buf<u32> gvv(nv);
   Buf<u32> goff(nv + 1);
-  u32 got[2];
+  // coarsen_all_ents (coarsen_common.cpp:91-107), top dimension first.
+  // Phase 1 decides every dimension's survivors and products on the device
+  // (a dimension's fused sides feed the next lower one, no host decision
+  // needed), right after the destinations; one read then brings the set's
+  // convergence flag, the collapse count and all the totals; phase 2
+  // writes the outputs.
+  bool has_class = find_tag(m, 0, "class_dim") != nullptr;
+  bool has_id = find_tag(m, 0, "class_id") != nullptr;
+  Buf<u32> fused[4];
+  struct DimPlan {
+    Buf<u32> gv, gd, egoff, esoff;
+  };
+  DimPlan plan[4];
+  u32 got[10];
   auto dest_and_total = [&] {
     // vertex_dest (collapses_to_verts.cpp:72-135)
     u32 const* uo = uv.offsets.get();
@@ -1378,8 +1391,74 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     });
     // coarsen_interior (coarsen_common.cpp:191-219)
     exscan(indset.get(), nv, goff.get());
-    u32 const* ptrs[2] = {flag, goff.get() + nv};
-    read_many(ptrs, 2, got);
+    for (u32 dd = 0; dd < dim; ++dd) {
+      u32 d = dim - dd;
+      u32 n = count(m, d);
+      u32 const* vent = ask_down(m, d, 0);
+      u32 const* sent = ask_down(m, d, d - 1);
+      u32 nsd = count(m, d - 1);
+      fused[d - 1] = filled(nsd, 0);
+      u32 vpe = d + 1, spe = h_down_deg[d][d - 1];
+      Buf<u32> will(n), gv(n), gd(n), same(n);
+      {
+        // collapse_to_ent (collapses_to_ents.cpp:11-82); also marks the
+        // entities with no collapsing vertex, which survive unchanged
+        u32 const* go = goff.get();
+        u32 const* g = gvv.get();
+        u32 const* fe = fused[d] ? fused[d].get() : nullptr;
+        u32* fs = fused[d - 1].get();
+        u32* wl = will.get();
+        u32* gvp = gv.get();
+        u32* gdp = gd.get();
+        u32* smp = same.get();
+        parallel_for(n, OHX_LAMBDA(u32 i) {
+          u32 const* v = vent + (size_t) i * vpe;
+          u32 col = INVALID, dir = INVALID;
+          for (u32 j = 0; j < vpe; ++j)
+            if (go[v[j]] != go[v[j] + 1]) {
+              col = v[j];
+              dir = j;
+            }
+          u32 w = 0, gvv_ = 0, gd_ = 0;
+          if (dir != INVALID) {
+            u32 target = g[col];
+            u32 gdir = INVALID;
+            for (u32 j = 0; j < vpe; ++j)
+              if (v[j] == target) gdir = j;
+            if (gdir != INVALID) {
+              fs[sent[(size_t) i * spe + opp(d, 0)[gdir]]] = 1;
+            } else if (!(fe && fe[i])) {
+              w = 1;
+              gvv_ = target;
+              gd_ = dir;
+            }
+          }
+          wl[i] = w;
+          gvp[i] = gvv_;
+          gdp[i] = gd_;
+          smp[i] = dir == INVALID ? 1u : 0u;
+        });
+      }
+      Buf<u32> egoff(n + 1), esoff(n + 1);
+      {
+        u32 const* wl = will.get();
+        u32 const* smp = same.get();
+        exscan_fn(n, OHX_LAMBDA(u32 i) { return wl[i]; }, egoff.get());
+        exscan_fn(n, OHX_LAMBDA(u32 i) { return smp[i]; }, esoff.get());
+      }
+      same.release();
+      plan[d] = DimPlan{std::move(gv), std::move(gd), std::move(egoff), std::move(esoff)};
+    }
+    u32 const* ptrs[10];
+    u32 k = 0;
+    ptrs[k++] = flag;
+    ptrs[k++] = goff.get() + nv;
+    for (u32 d = 1; d <= dim; ++d) {
+      u32 n = count(m, d);
+      ptrs[k++] = plan[d].egoff.get() + n;
+      ptrs[k++] = plan[d].esoff.get() + n;
+    }
+    read_many(ptrs, k, got);
   };
   dest_and_total();
   if (got[0]) {  // not settled in the first batch: the full loop
@@ -1416,92 +1495,11 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       check_launch("subset_tags");
     }
   }
-  // coarsen_all_ents (coarsen_common.cpp:91-107), top dimension first.
-  // Phase 1 decides every dimension's survivors and products on the device
-  // (a dimension's fused sides feed the next lower one, no host decision
-  // needed); one read then brings all the totals; phase 2 writes the
-  // outputs.
-  bool has_class = find_tag(m, 0, "class_dim") != nullptr;
-  bool has_id = find_tag(m, 0, "class_id") != nullptr;
-  Buf<u32> fused[4];
-  struct DimPlan {
-    Buf<u32> gv, gd, egoff, esoff;
-  };
-  DimPlan plan[4];
-  for (u32 dd = 0; dd < dim; ++dd) {
-    u32 d = dim - dd;
-    u32 n = count(m, d);
-    u32 const* vent = ask_down(m, d, 0);
-    u32 const* sent = ask_down(m, d, d - 1);
-    u32 nsd = count(m, d - 1);
-    fused[d - 1] = filled(nsd, 0);
-    u32 vpe = d + 1, spe = h_down_deg[d][d - 1];
-    Buf<u32> will(n), gv(n), gd(n), same(n);
-    {
-      // collapse_to_ent (collapses_to_ents.cpp:11-82); also marks the
-      // entities with no collapsing vertex, which survive unchanged
-      u32 const* go = goff.get();
-      u32 const* g = gvv.get();
-      u32 const* fe = fused[d] ? fused[d].get() : nullptr;
-      u32* fs = fused[d - 1].get();
-      u32* wl = will.get();
-      u32* gvp = gv.get();
-      u32* gdp = gd.get();
-      u32* smp = same.get();
-      parallel_for(n, OHX_LAMBDA(u32 i) {
-        u32 const* v = vent + (size_t) i * vpe;
-        u32 col = INVALID, dir = INVALID;
-        for (u32 j = 0; j < vpe; ++j)
-          if (go[v[j]] != go[v[j] + 1]) {
-            col = v[j];
-            dir = j;
-          }
-        u32 w = 0, gvv_ = 0, gd_ = 0;
-        if (dir != INVALID) {
-          u32 target = g[col];
-          u32 gdir = INVALID;
-          for (u32 j = 0; j < vpe; ++j)
-            if (v[j] == target) gdir = j;
-          if (gdir != INVALID) {
-            fs[sent[(size_t) i * spe + opp(d, 0)[gdir]]] = 1;
-          } else if (!(fe && fe[i])) {
-            w = 1;
-            gvv_ = target;
-            gd_ = dir;
-          }
-        }
-        wl[i] = w;
-        gvp[i] = gvv_;
-        gdp[i] = gd_;
-        smp[i] = dir == INVALID ? 1u : 0u;
-      });
-    }
-    Buf<u32> egoff(n + 1), esoff(n + 1);
-    {
-      u32 const* wl = will.get();
-      u32 const* smp = same.get();
-      exscan_fn(n, OHX_LAMBDA(u32 i) { return wl[i]; }, egoff.get());
-      exscan_fn(n, OHX_LAMBDA(u32 i) { return smp[i]; }, esoff.get());
-    }
-    same.release();
-    plan[d] = DimPlan{std::move(gv), std::move(gd), std::move(egoff), std::move(esoff)};
-  }
-  u32 tots[8];
-  {
-    u32 const* pt[8];
-    u32 k = 0;
-    for (u32 d = 1; d <= dim; ++d) {
-      u32 n = count(m, d);
-      pt[k++] = plan[d].egoff.get() + n;
-      pt[k++] = plan[d].esoff.get() + n;
-    }
-    read_many(pt, k, tots);
-  }
   for (u32 dd = 0; dd < dim; ++dd) {
     u32 d = dim - dd;
     u32 n = count(m, d);
     u32 vpe = d + 1;
-    u32 ngen = tots[2 * (d - 1)], nsame = tots[2 * (d - 1) + 1];
+    u32 ngen = got[2 + 2 * (d - 1)], nsame = got[2 + 2 * (d - 1) + 1];
     DimPlan& pl = plan[d];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     Buf<u32> cdo, cio;
@@ -2165,12 +2163,54 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   }
   eq.release();
   if (read_word(W_ANY) != tag) return 0;
+  // a first batch of indset rounds with no host check, every dimension's
+  // product and survivor offsets, then one read of the convergence flag
+  // and all the totals (a set not settled in the first batch starts over
+  // with the checked loop, same rounds, same result)
   Graph eg = edge_tet_graph(m);
-  Buf<u32> indset = indset_rounds(m, 1, cand.get(), q.get(), 0, nullptr, &eg);
+  u32 const* flag = nullptr;
+  u32 const rounds0 = g_indset_max_rounds < 8 ? g_indset_max_rounds : 8;
+  Buf<u32> indset = indset_rounds(m, 1, cand.get(), q.get(), rounds0, &flag, &eg);
+  u32 dlo = m.full ? 1 : m.dim;
+  Buf<u32> goffs[4], soffs[4];
+  u32 tots[8];
+  auto offsets_and_totals = [&] {
+    u32 const* is = indset.get();
+    u32 const* rs = rsz.get();
+    u32 const* pt[8];
+    u32 k = 0;
+    pt[k++] = flag;
+    for (u32 d = dlo; d <= m.dim; ++d) {
+      u32 n = count(m, d);
+      u32 lpe = h_down_deg[d][1];
+      u32 const* le = ask_down(m, d, 1);
+      // products per swapped edge (edge_topo_offset, swap_topology.cpp:16-44)
+      // and survivors: entities adjacent to no swapped edge (swap.cpp:41-45)
+      goffs[d] = Buf<u32>(ne + 1);
+      soffs[d] = Buf<u32>(n + 1);
+      exscan_fn(ne, OHX_LAMBDA(u32 i) {
+        return is[i] ? count_swap_ents(rs[i], d, k_sw_nint[rs[i]], k_sw_sizes[rs[i]]) : 0u;
+      }, goffs[d].get());
+      exscan_fn(n, OHX_LAMBDA(u32 i) {
+        for (u32 j = 0; j < lpe; ++j)
+          if (is[le[(size_t) i * lpe + j]]) return 0u;
+        return 1u;
+      }, soffs[d].get());
+      pt[k++] = goffs[d].get() + ne;
+      pt[k++] = soffs[d].get() + n;
+    }
+    read_many(pt, k, tots);
+  };
+  offsets_and_totals();
+  if (tots[0]) {
+    indset = indset_rounds(m, 1, cand.get(), q.get(), 0, nullptr, &eg);
+    flag = nullptr;
+    offsets_and_totals();
+  }
   if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();
   q.release();
-  u32 total = sum_u32(indset.get(), ne);
+  u32 total = g_verbose ? sum_u32(indset.get(), ne) : 0u;
   u32 nv = count(m, 0);
   Mesh mo(m.dim, m.full);
   set_ents(mo, 0, nv, Buf<u32>());
@@ -2192,30 +2232,14 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
     }
   }
   u32 const* is = indset.get();
-  u32 const* rs = rsz.get();
   bool has_class = m.full && find_tag(m, 0, "class_dim") != nullptr;
   bool has_id = has_class && find_tag(m, 0, "class_id") != nullptr;
-  u32 dlo = m.full ? 1 : m.dim;
   for (u32 d = dlo; d <= m.dim; ++d) {
     u32 n = count(m, d);
     u32 vpe = d + 1;
-    u32 lpe = h_down_deg[d][1];
-    u32 const* le = ask_down(m, d, 1);
-    // products per swapped edge (edge_topo_offset, swap_topology.cpp:16-44)
-    // and survivors: entities adjacent to no swapped edge (swap.cpp:41-45)
-    Buf<u32> goff(ne + 1), soff(n + 1);
-    exscan_fn(ne, OHX_LAMBDA(u32 i) {
-      return is[i] ? count_swap_ents(rs[i], d, k_sw_nint[rs[i]], k_sw_sizes[rs[i]]) : 0u;
-    }, goff.get());
-    exscan_fn(n, OHX_LAMBDA(u32 i) {
-      for (u32 j = 0; j < lpe; ++j)
-        if (is[le[(size_t) i * lpe + j]]) return 0u;
-      return 1u;
-    }, soff.get());
-    u32 tots[2];
-    u32 const* pt[2] = {goff.get() + ne, soff.get() + n};
-    read_many(pt, 2, tots);
-    u32 ngen = tots[0], nsame = tots[1];
+    Buf<u32>& goff = goffs[d];
+    Buf<u32>& soff = soffs[d];
+    u32 ngen = tots[1 + 2 * (d - dlo)], nsame = tots[2 + 2 * (d - dlo)];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     {
       double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
