@@ -125,14 +125,67 @@ __global__ __launch_bounds__(BLOCK) void indset_round(u32 n, u32 const* __restri
   wave_flag(flag, undecided);
 }
 
+// The same round over the edges of a 3D mesh with the neighbours taken
+// from the tets around each edge (up(1,3), down(3,1)): an edge's
+// neighbours are the other edges of its tets, which as a set is the edge
+// star star(1,3) (the two other edges of every triangle through the edge
+// and the opposite edge of every tet through it); the rule depends only on
+// that set, so the star itself, which would need up(1,2), is not built.
+__global__ __launch_bounds__(BLOCK) void indset_round_tets(u32 n, u32 const* __restrict__ uo,
+                                                           u32 const* __restrict__ ua, u32 const* __restrict__ ud,
+                                                           u32 const* __restrict__ er,
+                                                           double const* __restrict__ good,
+                                                           u32 const* __restrict__ old, u32* __restrict__ state,
+                                                           u32* __restrict__ flag) {
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
+  bool undecided = false;
+  if (i < n) {
+    u32 s = old[i];
+    if (s == UNKNOWN) {
+      double const myg = good[i];
+      bool wait = false, out = false;
+      for (u32 j = uo[i]; j < uo[i + 1] && !out; ++j) {
+        u32 t = ua[j], d = ud[j];
+        for (u32 l = 0; l < 6; ++l) {
+          if (l == d) continue;
+          u32 o = er[6 * (size_t) t + l];
+          u32 so = old[o];
+          if (so == IN_SET) {
+            out = true;
+            break;
+          }
+          if (so == NOT_IN_SET || wait) continue;
+          double og = good[o];
+          if ((myg == og && o < i) || myg < og) wait = true;
+        }
+      }
+      s = out ? (u32) NOT_IN_SET : (wait ? (u32) UNKNOWN : (u32) IN_SET);
+      undecided = !out && wait;
+    }
+    state[i] = s;
+  }
+  wave_flag(flag, undecided);
+}
+
 // rounds == 0: rounds in batches with a host check after each, up to the
 // limit (the reference's loop); rounds > 0: exactly that many rounds and no
 // host check, *last_flag then points at the device word that is nonzero if
 // entities were still undecided after the last round
 static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double const* good, u32 rounds,
-                              u32 const** last_flag, Graph const* graph = nullptr) {
+                              u32 const** last_flag, bool via_tets = false) {
   u32 n = count(m, ent_dim);
-  Graph const& g = graph ? *graph : ask_star(m, ent_dim, m.dim);
+  Graph empty;
+  Graph const& g = via_tets ? empty : ask_star(m, ent_dim, m.dim);
+  Up const* ut = via_tets ? &ask_up(m, 1, 3) : nullptr;
+  u32 const* er = via_tets ? ask_down(m, 3, 1) : nullptr;
+  auto round = [&](u32 const* in, u32* outs, u32* fl, double bytes) {
+    if (via_tets)
+      launch_timed("indset_round", bytes, indset_round_tets, dim3(nblocks(n)), dim3(BLOCK), n, ut->offsets.get(),
+                   ut->adj.get(), ut->dirs.get(), er, good, in, outs, fl);
+    else
+      launch_timed("indset_round", bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
+                   g.adj.get(), good, in, outs, fl);
+  };
   Buf<u32> a(n), b(n);
   constexpr u32 BATCH = 8;
   u32* flags = scratch_words() + W_FLAGS;
@@ -163,8 +216,7 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
     u32 nr = rounds < BATCH ? rounds : BATCH;
     if (n < nr) fill(flags, nr, 0);  // the set-up kernel cleared flags [0, n)
     for (u32 r = 0; r < nr; ++r) {
-      launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
-                   g.adj.get(), good, a.get(), b.get(), flags + r);
+      round(a.get(), b.get(), flags + r, round_bytes);
       std::swap(a, b);
     }
     *last_flag = flags + nr - 1;
@@ -175,8 +227,7 @@ static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double co
     u32 nr = MAX_ROUNDS - done < BATCH ? MAX_ROUNDS - done : BATCH;
     if (done || n < nr) fill(flags, nr, 0);
     for (u32 r = 0; r < nr; ++r) {
-      launch_timed("indset_round", round_bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
-                         g.adj.get(), good, a.get(), b.get(), flags + r);
+      round(a.get(), b.get(), flags + r, round_bytes);
       std::swap(a, b);
     }
     u32 h[BATCH];
@@ -199,38 +250,6 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
   return indset_rounds(m, ent_dim, filter, good, 0, nullptr);
 }
 
-// The edges of the tets around each edge, the edge itself left out (five
-// per use, repeats kept).  As a set this is the edge star star(1,3)
-// (get_edge_tet_star, star.cpp:202-278: the two other edges of every
-// triangle through the edge and the opposite edge of every tet through it,
-// which together are the tets' other edges), and an independent set depends
-// only on each entity's set of neighbours -- any neighbour in the set, any
-// live neighbour ahead of it -- not on their order or repeats.  It needs
-// only up(1,3) and down(3,1), which the swap pass has built already; the
-// star would also need up(1,2).
-static Graph edge_tet_graph(Mesh& m) {
-  Up const& u = ask_up(m, 1, 3);
-  u32 const* er = ask_down(m, 3, 1);
-  u32 ne = count(m, 1);
-  size_t nuse = u.adj.size();
-  if ((size_t) 5 * nuse >= ((size_t) 1 << 32)) fail("edge graph: more than 2^32 entries", __FILE__, __LINE__);
-  Graph g;
-  g.offsets = Buf<u32>(ne + 1);
-  g.adj = Buf<u32>(5 * nuse);
-  u32* go = g.offsets.get();
-  u32* ga = g.adj.get();
-  u32 const* uo = u.offsets.get();
-  u32 const* ua = u.adj.get();
-  u32 const* ud = u.dirs.get();
-  parallel_for(ne + 1, OHX_LAMBDA(u32 i) { go[i] = 5 * uo[i]; });
-  if (nuse)
-    parallel_for((u32) nuse, OHX_LAMBDA(u32 j) {
-      u32 t = ua[j], d = ud[j], k = 0;
-      for (u32 l = 0; l < 6; ++l)
-        if (l != d) ga[5 * (size_t) j + k++] = er[6 * (size_t) t + l];
-    });
-  return g;
-}
 
 // a first batch of rounds with no host check; if the set has not settled
 // the caller starts over with find_indset (same rounds, same result).  The
@@ -2167,10 +2186,9 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   // product and survivor offsets, then one read of the convergence flag
   // and all the totals (a set not settled in the first batch starts over
   // with the checked loop, same rounds, same result)
-  Graph eg = edge_tet_graph(m);
   u32 const* flag = nullptr;
   u32 const rounds0 = g_indset_max_rounds < 8 ? g_indset_max_rounds : 8;
-  Buf<u32> indset = indset_rounds(m, 1, cand.get(), q.get(), rounds0, &flag, &eg);
+  Buf<u32> indset = indset_rounds(m, 1, cand.get(), q.get(), rounds0, &flag, true);
   u32 dlo = m.full ? 1 : m.dim;
   Buf<u32> goffs[4], soffs[4];
   u32 tots[8];
@@ -2203,7 +2221,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   };
   offsets_and_totals();
   if (tots[0]) {
-    indset = indset_rounds(m, 1, cand.get(), q.get(), 0, nullptr, &eg);
+    indset = indset_rounds(m, 1, cand.get(), q.get(), 0, nullptr, true);
     flag = nullptr;
     offsets_and_totals();
   }

@@ -972,6 +972,39 @@ __global__ __launch_bounds__(BLOCK) void dual_pairs_k(u32 nsides, u32 const* __r
   }
 }
 
+// The same dual in 3D from the tets around an edge of each face: the tet
+// across face k of tet t is the other tet around the face's first edge
+// that holds the face's third vertex.  Used when up(1,3) and down(3,1)
+// are there already (a swap pass builds them): no table, no atomics.
+__global__ __launch_bounds__(BLOCK) void dual_edges_k(u32 nt, u32 const* __restrict__ tv, u32 const* __restrict__ te,
+                                                      u32 const* __restrict__ uo, u32 const* __restrict__ ua,
+                                                      u32* __restrict__ dual) {
+  u32 s = xcd_block() * BLOCK + threadIdx.x;
+  if (s >= 4 * nt) return;
+  u32 const t = s / 4, k = s % 4;
+  u32 const* fv = canon(3, 2, 0, k);
+  // the tet's local edge between the face's first two vertices
+  u32 le = 0;
+#pragma unroll
+  for (u32 l = 0; l < 6; ++l) {
+    u32 const* lv = canon(3, 1, 0, l);
+    if ((lv[0] == fv[0] && lv[1] == fv[1]) || (lv[0] == fv[1] && lv[1] == fv[0])) le = l;
+  }
+  u32 const edge = te[6 * (size_t) t + le];
+  u32 const c = tv[4 * (size_t) t + fv[2]];
+  u32 r = INVALID;
+  for (u32 j = uo[edge]; j < uo[edge + 1]; ++j) {
+    u32 o = ua[j];
+    if (o == t) continue;
+    u32 const* ov = tv + 4 * (size_t) o;
+    if (ov[0] == c || ov[1] == c || ov[2] == c || ov[3] == c) {
+      r = o;
+      break;
+    }
+  }
+  dual[s] = r;
+}
+
 static Buf<u32> dual_by_pairs(Mesh& m) {
   u32 dim = m.dim;
   u32 ne = count(m, dim);
@@ -1127,7 +1160,17 @@ u32 const* ask_dual(Mesh& m) {
       m.dual = dual_from_verts(m);
     else if (m.down[d][d - 1] && m.up[d - 1][d])
       m.dual = dual_from_sides(m);  // the side maps are there already
-    else
+    else if (d == 3 && m.down[3][1] && m.up[1][3]) {
+      u32 nt = m.counts[3];
+      Buf<u32> out((size_t) 4 * nt);
+      if (nt) {
+        Up const& u = *m.up[1][3];
+        hipLaunchKernelGGL(dual_edges_k, dim3(nblocks((size_t) 4 * nt)), dim3(BLOCK), 0, stream(), nt,
+                           m.down[3][0].get(), m.down[3][1].get(), u.offsets.get(), u.adj.get(), out.get());
+        check_launch("dual_edges");
+      }
+      m.dual = std::move(out);
+    } else
       m.dual = dual_by_pairs(m);
   }
   return m.dual.get();
