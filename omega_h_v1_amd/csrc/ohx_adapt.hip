@@ -1270,9 +1270,8 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   if (require_better) eq = element_qualities(dim, count(m, dim), vel, coords(m), count(m, 0), metric_of(m));
   Buf<double> cq((size_t) 2 * ne);
   u32 tag = next_any_tag();
-  u32 cq_mode = CQ_F64, cq_g = 1;
-  CqArgs cq_args{};
-  double cq_bytes = 0;
+  Buf<u32> cand;
+  Buf<double> vq;
   {
     double prof_bytes = (double) ne * 32 + (double) ue.adj.size() * 8 + (double) count(m, dim) * (dim + 1) * 4 + (double) count(m, 0) * 28;
     // variants, chosen online per size class: lanes per edge (1, 2, 4 or
@@ -1308,52 +1307,57 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     a.out = cq.get();
     a.vmark = nullptr;
     a.st = screen_stats();
+    // collapse_to_vert (collapses_to_verts.cpp:13-70) belongs to the timed
+    // variant: the certified pass adds its certification, the FP64
+    // measurement of what it could not certify and a second collapse_to_vert,
+    // so a trial times the whole sequence (events recorded on the stream
+    // around it) and the tuner weighs the variants by what they cost
+    u32 const nv = count(m, 0);
+    Up const& uv = ask_up(m, 0, 1);
+    Graph const* sg = (g_mixed && !require_better && nv) ? &ask_star(m, 0, dim) : nullptr;
+    cand = Buf<u32>(nv);
+    vq = Buf<double>(nv);
     tuned(tuner, ncand, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
       u32 g = var & 15u;
       u32 mode = (var & 32u) ? (u32) CQ_CERTIFIED : (var & 16u) ? (u32) CQ_SCREEN : (u32) CQ_F64;
-      cq_mode = mode;
-      cq_g = g;
-      dim == 3 ? cq_launch_mode<3>(mode, g, a, prof_bytes, ea, eb) : cq_launch_mode<2>(mode, g, a, prof_bytes, ea, eb);
+      bool const certified = mode == CQ_CERTIFIED && !require_better;
+      if (ea) OHX_CHECK(hipEventRecord(ea, stream()));
+      dim == 3 ? cq_launch_mode<3>(mode, g, a, prof_bytes, nullptr, nullptr)
+               : cq_launch_mode<2>(mode, g, a, prof_bytes, nullptr, nullptr);
+      Buf<u32> vinex, vmark;
+      if (certified) vinex = Buf<u32>(nv);
+      auto to_vert = [&] {
+        if (!nv) return;
+        hipLaunchKernelGGL(collapse_to_vert_k, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
+                           uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), cand.get(), vq.get(),
+                           certified ? vinex.get() : nullptr);
+        check_launch("collapse_to_vert");
+      };
+      to_vert();
+      if (certified && nv) {
+        // the comparisons the FP32 goodness takes part in: the vertices
+        // whose choices are not certified get exact values (their inexact
+        // directions measured in FP64), then their best again
+        vmark = Buf<u32>(nv);
+        hipLaunchKernelGGL(coarsen_certify, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
+                           uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), sg->offsets.get(), sg->adj.get(),
+                           cand.get(), vq.get(), vinex.get(), vmark.get());
+        check_launch("coarsen_certify");
+        CqArgs f = a;
+        f.vmark = vmark.get();
+        dim == 3 ? cq_launch_mode<3>(CQ_FORCE, g, f, prof_bytes, nullptr, nullptr)
+                 : cq_launch_mode<2>(CQ_FORCE, g, f, prof_bytes, nullptr, nullptr);
+        to_vert();
+      }
+      if (eb) OHX_CHECK(hipEventRecord(eb, stream()));
     });
-    cq_args = a;
-    cq_bytes = prof_bytes;
   }
   // no host check here: if the quality check left no collapse, the
   // candidates below are empty, the independent set settles at once and the
   // one read of its total returns 0 (check_coarsen_quality's early return)
   (void) tag;
-  // collapse_to_vert (collapses_to_verts.cpp:13-70)
   u32 nv = count(m, 0);
   Up const& uv = ask_up(m, 0, 1);
-  Buf<u32> cand(nv);
-  Buf<double> vq(nv);
-  bool const certified = cq_mode == CQ_CERTIFIED && !require_better;
-  Buf<u32> vinex, vmark;
-  if (certified) vinex = Buf<u32>(nv);
-  auto to_vert = [&] {
-    if (!nv) return;
-    hipLaunchKernelGGL(collapse_to_vert_k, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
-                       uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), cand.get(), vq.get(),
-                       certified ? vinex.get() : nullptr);
-    check_launch("collapse_to_vert");
-  };
-  to_vert();
-  if (certified && nv) {
-    // the comparisons the FP32 goodness takes part in: the vertices whose
-    // choices are not certified get exact values (their inexact
-    // directions measured in FP64), then their best again
-    Graph const& sg = ask_star(m, 0, dim);
-    vmark = Buf<u32>(nv);
-    hipLaunchKernelGGL(coarsen_certify, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
-                       uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), sg.offsets.get(), sg.adj.get(),
-                       cand.get(), vq.get(), vinex.get(), vmark.get());
-    check_launch("coarsen_certify");
-    CqArgs f = cq_args;
-    f.vmark = vmark.get();
-    dim == 3 ? cq_launch_mode<3>(CQ_FORCE, cq_g, f, cq_bytes, nullptr, nullptr)
-             : cq_launch_mode<2>(CQ_FORCE, cq_g, f, cq_bytes, nullptr, nullptr);
-    to_vert();
-  }
   eq.release();
   // a first batch of indset rounds, the collapse destinations and the
   // vertex offsets, then one read of the convergence flag and the total
