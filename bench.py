@@ -9,9 +9,10 @@ equal min quality, on one GPU.
   eps 0.01, sizes clamped to [0.002, 0.1].
   Run: 8 cycles of mesh_adapt (size floor 1/3, good quality 0.3, 4 sliver
   layers).  Every cycle runs to completion: the operation limit is set to
-  1000, above the most any cycle needs (the cycles take 70 to ~420
-  operations), so no cycle stops where the reference's mesh_adapt would
-  abort (adapt.cpp:36-45).  The independent sets of this workload need more
+  1000, above the most any cycle needs (the eight cycles take 95 to 872
+  operations; the counts are reported and a cycle at the limit is an
+  error), so no cycle stops where the reference's mesh_adapt would abort
+  (adapt.cpp:36-45).  The independent sets of this workload need more
   rounds than the reference's 100 (indset.cpp:33): its ties in goodness
   form long chains; the round limit is raised to 2000 (an extension; the
   rounds the workload needed are reported).
@@ -231,6 +232,10 @@ def c3_survey(lib, mixed):
                        "wall_seconds": wall, "counts": m.counts(), "min_quality": m.min_quality()})
     dll.ohx_synchronize()
     dll.ohx_profile_enable(0)
+    at_limit = [k for k, c in enumerate(cycles) if c["result"] < 0]
+    if at_limit:
+        raise RuntimeError("c3: cycles %r reached the operation limit %d, where the reference's mesh_adapt "
+                           "aborts (adapt.cpp:36-45)" % (at_limit, C3_MAX_OPS))
     tab, host_sync, host_malloc = kernel_table(dll)
     dom = max(tab, key=lambda k: tab[k][0]) if tab else None
     out = {
@@ -407,7 +412,9 @@ def run_c3(args, lib):
             "workload": "c3: 3D Kuhn box 64^3x6 = 1572864 tets, metric from the Hessian of tanh(50(x+y+z-1.5)) "
                         "(FP64 eigendecomposition, eps 0.01, h in [0.002, 0.1]), 8 cycles of mesh_adapt; a step "
                         "is one mesh_adapt cycle (cycle k mod 8 of the sequence, from its input mesh)",
-            "precision": "FP32 quality screen with FP64 re-evaluation within 1e-4 of each decision",
+            "precision": "FP32-mixed: split, collapse and swap qualities in FP32 (reject-only screen, or certified "
+                         "accept/reject and goodness comparisons), FP64 re-evaluation of every decision within 1e-4; "
+                         "the variant per kernel and size class is the fastest measured, FP64 included",
             "max_ops": C3_MAX_OPS,
             "ops_per_cycle": [c["passes"] for c in svf["cycles"]],
             "cycles_at_op_limit": sum(1 for c in svf["cycles"] if c["result"] < 0),
@@ -658,7 +665,7 @@ def run_c4(args, lib):
     ref_digests = {c: digest(singles[c]) for c in sample}
     dll.ohx_set_precision(1 if mixed else 0)
     runs = {}
-    for nthreads in (4, 8):
+    for nthreads in (4, 8, 16):
         clones = [p.clone() for p in parts]
         dll.ohx_synchronize()
         t0 = time.perf_counter()

@@ -2,6 +2,7 @@
 // kernel entry points.  Exceptions never cross the ABI: the seam and kernel
 // layers record ohx_last_error() and return a sentinel, the osh_* layer
 // aborts like the reference's asserts.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -590,20 +591,22 @@ struct Pool {
   std::vector<std::thread> workers;
   std::function<void(u32)> fn;
   u32 n = 0, next = 0, done = 0;
+  u32 active = 0;  // workers 0 .. active-1 take part in the current run
   unsigned long long gen = 0;
   int device = 0;
   void grow(u32 k) {
     while (workers.size() < k) {
-      workers.emplace_back([this] { loop(); });
+      u32 id = (u32) workers.size();
+      workers.emplace_back([this, id] { loop(id); });
       workers.back().detach();
     }
   }
-  void loop() {
+  void loop(u32 id) {
     (void) hipSetDevice(device);
     unsigned long long seen = 0;
     std::unique_lock<std::mutex> l(mu);
     for (;;) {
-      work.wait(l, [&] { return gen != seen && next < n; });
+      work.wait(l, [&] { return gen != seen && next < n && id < active; });
       while (next < n) {
         u32 i = next++;
         l.unlock();
@@ -621,6 +624,7 @@ struct Pool {
     n = count;
     next = 0;
     done = 0;
+    active = k;
     ++gen;
     work.notify_all();
     done_cv.wait(l, [&] { return done == n; });
@@ -641,8 +645,17 @@ int ohx_adapt_many(unsigned n, ohx_mesh* const* meshes, double sf, double gq, un
     std::vector<std::string> err(n);
     Pool& p = pool();
     p.device = rt().device;
+    // largest meshes first (by element count): the run ends when its last
+    // mesh does, so a large mesh started late would leave the other streams
+    // idle behind it
+    std::vector<u32> order(n);
+    for (u32 i = 0; i < n; ++i) order[i] = i;
+    std::vector<u32> size(n);
+    for (u32 i = 0; i < n; ++i) size[i] = meshes[i] ? count(meshes[i]->m, meshes[i]->m.dim) : 0;
+    std::stable_sort(order.begin(), order.end(), [&](u32 a, u32 b) { return size[a] > size[b]; });
     auto t0 = std::chrono::steady_clock::now();
-    p.run(nthreads ? nthreads : 1, n, [&](u32 i) {
+    p.run(nthreads ? nthreads : 1, n, [&](u32 k) {
+      u32 const i = order[k];
       try {
         results[i] = adapt(meshes[i]->m, sf, gq, nl, mo, &st[i]);
       } catch (std::exception const& e) {
