@@ -814,10 +814,17 @@ __global__ __launch_bounds__(BLOCK) void exposed_sides_k(u32 n, u32 const* __res
   constexpr u32 VPE = ED + 1, SPE = ED + 1, NE = ED == 3 ? 6 : 3;
   u32 v[VPE], sd[SPE];
 #pragma unroll
-  for (u32 k = 0; k < VPE; ++k) v[k] = vent[(size_t) E * VPE + k];
-#pragma unroll
   for (u32 k = 0; k < SPE; ++k) sd[k] = sent[(size_t) E * SPE + k];
   u32 const cE = ecd[E];
+  // a direction is forbidden only through a side classified on a lower
+  // dimension than the entity; an entity whose sides all share its class
+  // (every interior one) forbids nothing and stops here
+  bool exposed = false;
+#pragma unroll
+  for (u32 k = 0; k < SPE; ++k) exposed = exposed || scd[sd[k]] != cE;
+  if (!exposed) return;
+#pragma unroll
+  for (u32 k = 0; k < VPE; ++k) v[k] = vent[(size_t) E * VPE + k];
   u32 const* so = opp(ED, 0);
 #pragma unroll
   for (u32 le = 0; le < NE; ++le) {
@@ -1228,7 +1235,7 @@ static void cq_launch_mode(u32 mode, u32 g, CqArgs const& a, double bytes, hipEv
 }
 
 static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better,
-                          double const* qfe = nullptr) {
+                          double const* qfe = nullptr, double const* eq_given = nullptr) {
   check_index_space(m);
   u32 dim = m.dim;
   u32 ne = count(m, 1);
@@ -1267,7 +1274,13 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   u32 const* ev = ask_down(m, 1, 0);
   u32 const* vel = ask_down(m, dim, 0);
   Up const& ue = ask_up(m, 0, dim);
-  if (require_better) eq = element_qualities(dim, count(m, dim), vel, coords(m), count(m, 0), metric_of(m));
+  // the element qualities of this mesh: the caller's (the adapt loop
+  // computes them once per mesh) or evaluated here
+  double const* eqp = nullptr;
+  if (require_better) {
+    if (!eq_given) eq = element_qualities(dim, count(m, dim), vel, coords(m), count(m, 0), metric_of(m));
+    eqp = eq_given ? eq_given : eq.get();
+  }
   Buf<double> cq((size_t) 2 * ne);
   u32 tag = next_any_tag();
   Buf<u32> cand;
@@ -1300,7 +1313,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     a.vel = vel;
     a.x = coords(m);
     a.mt = metric_of(m);
-    a.eq = eq.get();
+    a.eq = eqp;
     a.qual_floor = qual_floor;
     a.qfe = qfe;
     a.codes = codes.get();
@@ -1582,9 +1595,9 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
 }
 
 // coarsen_slivers (coarsen.cpp:38-70)
-u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers) {
+u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers, double const* eq) {
   u32 dim = m.dim;
-  Buf<u32> sl = mark_slivers(m, qual_floor, nlayers);
+  Buf<u32> sl = mark_slivers(m, qual_floor, nlayers, eq);
   Buf<u32> mv = mark_down(m, dim, 0, sl.get());
   sl.release();
   u32 ne = count(m, 1);
@@ -1600,7 +1613,7 @@ u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers) {
   });
   if (g_batch) batch_mask(m, 1, c);
   if (max_u32(c, ne) == DONT_COLLAPSE) return 0;
-  return coarsen_common(m, codes, 0.0, true);
+  return coarsen_common(m, codes, 0.0, true, nullptr, eq);
 }
 
 // ----------------------------------------------------------------- swap
@@ -2114,7 +2127,7 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
 }
 
 // swap_common + swap_interior + swap_ents (swap.cpp:27-136)
-static u32 swap_common(Mesh& m, Buf<u32>& cand) {
+static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr) {
   check_index_space(m);
   u32 ne = count(m, 1);
   unmark_boundary(m, 1, cand.get());
@@ -2141,7 +2154,9 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   Up const& u = ask_up(m, 1, 3);
   u32 const* ev = ask_down(m, 1, 0);
   u32 const* vt = ask_down(m, 3, 0);
-  Buf<double> eq = mesh_qualities(m);
+  Buf<double> eq;
+  if (!eq_given) eq = mesh_qualities(m);
+  double const* eqp = eq_given ? eq_given : eq.get();
   Buf<double> q(ne);
   Buf<u32> rsz = filled(ne, 0);
   Buf<u32> codes(ne);
@@ -2158,12 +2173,12 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
     auto go = [&](auto kernel, char const* kn, hipEvent_t ea, hipEvent_t eb) {
       if (ea) {
         hipExtLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, stream(), ea, eb, 0, ncand, clist.get(), u.offsets.get(),
-                              u.adj.get(), u.dirs.get(), ev, vt, coords(m), mts, eq.get(), cand.get(), q.get(),
+                              u.adj.get(), u.dirs.get(), ev, vt, coords(m), mts, eqp, cand.get(), q.get(),
                               rsz.get(), codes.get(), scratch_words(), tag, dev_error_word(), sst);
         check_launch(kn);
       } else {
         launch_timed(kn, prof_bytes, kernel, grid, dim3(BLOCK), ncand, clist.get(), u.offsets.get(), u.adj.get(),
-                     u.dirs.get(), ev, vt, coords(m), mts, eq.get(), cand.get(), q.get(), rsz.get(), codes.get(),
+                     u.dirs.get(), ev, vt, coords(m), mts, eqp, cand.get(), q.get(), rsz.get(), codes.get(),
                      scratch_words(), tag, dev_error_word(), sst);
       }
     };
@@ -2304,13 +2319,17 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand) {
   return 1;
 }
 
-u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers) {
+u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq) {
   OHX_REQUIRE(m.dim == 3, "swap_slivers needs a 3D mesh");
-  Buf<u32> sl = mark_slivers(m, good_qual, nlayers);
+  // the pass reads up(1,3) (the edge marks below, the rings, the quality
+  // kernel); derived first, it also gives the sliver layers' element dual
+  // from the rings (dual_edges_k) instead of a hash table of the faces
+  (void) ask_up(m, 1, 3);
+  Buf<u32> sl = mark_slivers(m, good_qual, nlayers, eq);
   Buf<u32> c = mark_down(m, m.dim, 1, sl.get());
   sl.release();
   if (g_batch) batch_mask(m, 1, c.get());
-  return swap_common(m, c);
+  return swap_common(m, c, eq);
 }
 
 // ---------------------------------------------------------------- adapt
@@ -2371,7 +2390,7 @@ bool incr_op(Mesh& m, Ctx& c) {  // adapt.cpp:36-45
   return true;
 }
 enum Kind { REFINE, COARSEN, SWAP, COARSEN_SLIVERS };
-u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n) {
+u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n, double const* eq = nullptr) {
   double tin = count(m, m.dim);
   static bool const trace = std::getenv("OHX_TRACE_DERIVE") && std::getenv("OHX_TRACE_DERIVE")[0] == '1';
   if (trace) std::fprintf(stderr, "pass %s n=%.0f\n", k == REFINE ? "refine" : k == COARSEN ? "coarsen" : k == SWAP ? "swap" : "csliv", tin);
@@ -2381,8 +2400,8 @@ u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n) {
   switch (k) {
     case REFINE: ok = refine_by_size(m, a); break;
     case COARSEN: ok = coarsen_by_size(m, a, b); break;
-    case SWAP: ok = swap_slivers(m, a, n); break;
-    case COARSEN_SLIVERS: ok = coarsen_slivers(m, a, n); break;
+    case SWAP: ok = swap_slivers(m, a, n, eq); break;
+    case COARSEN_SLIVERS: ok = coarsen_slivers(m, a, n, eq); break;
   }
   OHX_CHECK(hipEventRecord(e1, stream()));
   if (ok)
@@ -2409,13 +2428,18 @@ int adapt(Mesh& m, double size_floor, double good_qual, u32 nlayers, u32 max_ops
   while (!aborted && run_pass(m, c, COARSEN, qf, size_floor, 0))
     if (!incr_op(m, c)) aborted = true;
   while (!aborted) {  // satisfy_shape (adapt.cpp:58-80)
-    double prev = mesh_min_quality(m);
+    // the element qualities of this mesh, evaluated once: its minimum, the
+    // sliver marks and the comparisons of the swap and collapse passes all
+    // read them (a pass that changes nothing leaves them valid for the next)
+    u32 const nel = count(m, m.dim);
+    Buf<double> eq = mesh_qualities(m);
+    double prev = nel > 1 ? min_f64(eq.get(), nel, 1) : 1.7976931348623157e308;
     if (prev >= good_qual) break;
-    if (m.dim == 3 && run_pass(m, c, SWAP, good_qual, 0, nlayers)) {
+    if (m.dim == 3 && run_pass(m, c, SWAP, good_qual, 0, nlayers, eq.get())) {
       if (!incr_op(m, c)) aborted = true;
       continue;
     }
-    if (run_pass(m, c, COARSEN_SLIVERS, good_qual, 0, nlayers)) {
+    if (run_pass(m, c, COARSEN_SLIVERS, good_qual, 0, nlayers, eq.get())) {
       if (!incr_op(m, c)) aborted = true;
       continue;
     }

@@ -12,8 +12,9 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
 u32 refine_by_size(Mesh& m, double qual_floor);
 void uniformly_refine(Mesh& m);
 u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor);
-u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers);
-u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers);
+// eq: the mesh's element qualities when the caller has them (else computed)
+u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers, double const* eq = nullptr);
+u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr);
 
 struct AdaptStats {
   double npass = 0;   // passes that changed the mesh
