@@ -1327,7 +1327,6 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     // around it) and the tuner weighs the variants by what they cost
     u32 const nv = count(m, 0);
     Up const& uv = ask_up(m, 0, 1);
-    Graph const* sg = (g_mixed && !require_better && nv) ? &ask_star(m, 0, dim) : nullptr;
     cand = Buf<u32>(nv);
     vq = Buf<double>(nv);
     tuned(tuner, ncand, [&](u32 var, hipEvent_t ea, hipEvent_t eb) {
@@ -1350,7 +1349,9 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       if (certified && nv) {
         // the comparisons the FP32 goodness takes part in: the vertices
         // whose choices are not certified get exact values (their inexact
-        // directions measured in FP64), then their best again
+        // directions measured in FP64), then their best again; the vertex
+        // star is derived here, for this variant only
+        Graph const* sg = &ask_star(m, 0, dim);
         vmark = Buf<u32>(nv);
         hipLaunchKernelGGL(coarsen_certify, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
                            uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), sg->offsets.get(), sg->adj.get(),
@@ -2126,6 +2127,18 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
   }
 }
 
+// a vertex pair as one key (smaller vertex in the high word) and its home
+// slot in a table of mask + 1 slots
+__host__ __device__ inline unsigned long long swap_pair_key(u32 a, u32 b) {
+  return a < b ? ((unsigned long long) a << 32) | b : ((unsigned long long) b << 32) | a;
+}
+__host__ __device__ inline u32 swap_pair_slot(unsigned long long key, u32 mask) {
+  key ^= key >> 29;
+  key *= 0xBF58476D1CE4E5B9ull;
+  key ^= key >> 32;
+  return (u32) key & mask;
+}
+
 // swap_common + swap_interior + swap_ents (swap.cpp:27-136)
 static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr) {
   check_index_space(m);
@@ -2211,16 +2224,40 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   u32 dlo = m.full ? 1 : m.dim;
   Buf<u32> goffs[4], soffs[4];
   u32 tots[8];
+  // the faces that survive are those holding no swapped edge; a face holds
+  // an edge exactly when it holds both of the edge's vertices, so when the
+  // face-to-edge map is not derived (it is needed for nothing else here) the
+  // swapped edges' vertex pairs go into a small hash table and each face
+  // looks its three vertex pairs up, instead of hashing every edge of the
+  // mesh to build down(2,1)
+  bool const faces_by_pairs = m.dim == 3 && m.full && !m.down[2][1];
+  u32 pcap = 64;
+  while (pcap < 4 * (size_t) ncand) pcap *= 2;
+  Buf<unsigned long long> pairs(faces_by_pairs ? pcap : 0);
   auto offsets_and_totals = [&] {
     u32 const* is = indset.get();
     u32 const* rs = rsz.get();
     u32 const* pt[8];
     u32 k = 0;
     pt[k++] = flag;
+    if (faces_by_pairs) {
+      unsigned long long* tb = pairs.get();
+      u32 const mask = pcap - 1;
+      parallel_for(pcap, OHX_LAMBDA(u32 i) { tb[i] = ~0ull; });
+      u32 const* cl = clist.get();
+      u32* err = dev_error_word();
+      parallel_for(ncand, OHX_LAMBDA(u32 c) {
+        u32 e = cl[c];
+        if (!is[e]) return;
+        unsigned long long key = swap_pair_key(ev[2 * (size_t) e], ev[2 * (size_t) e + 1]);
+        u32 h = swap_pair_slot(key, mask);
+        for (u32 probe = 0; probe <= mask; ++probe, h = (h + 1) & mask)
+          if (atomicCAS(&tb[h], ~0ull, key) == ~0ull) return;
+        *err = ERR_REFLECT_NOT_FOUND;
+      });
+    }
     for (u32 d = dlo; d <= m.dim; ++d) {
       u32 n = count(m, d);
-      u32 lpe = h_down_deg[d][1];
-      u32 const* le = ask_down(m, d, 1);
       // products per swapped edge (edge_topo_offset, swap_topology.cpp:16-44)
       // and survivors: entities adjacent to no swapped edge (swap.cpp:41-45)
       goffs[d] = Buf<u32>(ne + 1);
@@ -2228,11 +2265,31 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       exscan_fn(ne, OHX_LAMBDA(u32 i) {
         return is[i] ? count_swap_ents(rs[i], d, k_sw_nint[rs[i]], k_sw_sizes[rs[i]]) : 0u;
       }, goffs[d].get());
-      exscan_fn(n, OHX_LAMBDA(u32 i) {
-        for (u32 j = 0; j < lpe; ++j)
-          if (is[le[(size_t) i * lpe + j]]) return 0u;
-        return 1u;
-      }, soffs[d].get());
+      if (d == 2 && faces_by_pairs) {
+        u32 const* fv = ask_down(m, 2, 0);
+        unsigned long long const* tb = pairs.get();
+        u32 const mask = pcap - 1;
+        exscan_fn(n, OHX_LAMBDA(u32 i) {
+          u32 const* v = fv + 3 * (size_t) i;
+          for (u32 j = 0; j < 3; ++j) {
+            unsigned long long key = swap_pair_key(v[j], v[(j + 1) % 3]);
+            for (u32 probe = 0, h = swap_pair_slot(key, mask); probe <= mask; ++probe, h = (h + 1) & mask) {
+              unsigned long long t = tb[h];
+              if (t == key) return 0u;
+              if (t == ~0ull) break;
+            }
+          }
+          return 1u;
+        }, soffs[d].get());
+      } else {
+        u32 lpe = h_down_deg[d][1];
+        u32 const* le = ask_down(m, d, 1);
+        exscan_fn(n, OHX_LAMBDA(u32 i) {
+          for (u32 j = 0; j < lpe; ++j)
+            if (is[le[(size_t) i * lpe + j]]) return 0u;
+          return 1u;
+        }, soffs[d].get());
+      }
       pt[k++] = goffs[d].get() + ne;
       pt[k++] = soffs[d].get() + n;
     }
