@@ -74,7 +74,14 @@ void mixed_stats_reset() {
   fill(scratch_words() + W_STATS, 1, 0);
 }
 
-static u32 read_word(u32 k) { return read_at(scratch_words(), k); }
+// one scratch word, read through the mapped-memory gather (one small kernel
+// and a wait, cheaper than a device-to-host copy)
+static u32 read_word(u32 k) {
+  u32 const* p = scratch_words() + k;
+  u32 v = 0;
+  read_many(&p, 1, &v);
+  return v;
+}
 // a new value for the tagged "any" word W_ANY (never 0, the cleared value)
 static u32 next_any_tag() {
   static thread_local u32 t = 0;
