@@ -90,10 +90,20 @@ H = 1.0 / 200.0
 WARP_FLOOR = 0.2
 ADAPT_ARGS = (1.0 / 3.0, 0.3, 4, 50)
 
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-# scaled by the same counters read for a copy kernel of known size
-# (FETCH_SIZE x 2, WRITE_SIZE x 1); per workload and kernel
-TRAFFIC = {("c2", "scan"): 2499499}
+# Bytes per launch beyond L2 measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one run each), scaled by the same counters read for a copy
+# kernel of known size (FETCH_SIZE x 2, WRITE_SIZE x 1), averaged over the
+# profiled run's launches of the kernel; per workload and kernel (the
+# kernels that can be a workload's dominant one)
+TRAFFIC = {
+    ("c2", "scan"): 2994598,
+    ("c2", "indset_round"): 2158592,
+    ("c2", "reflect_down"): 55894426,
+    ("c3", "reflect_down"): 871991724,
+    ("c3", "scan"): 30890107,
+    ("c5", "lows_hash_insert"): 4283247102,
+    ("c5", "reflect_down"): 17995302541,
+}
 
 
 def _checker(which):
