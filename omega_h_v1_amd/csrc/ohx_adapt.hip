@@ -174,6 +174,42 @@ __global__ __launch_bounds__(BLOCK) void indset_round_tets(u32 n, u32 const* __r
   wave_flag(flag, undecided);
 }
 
+// The same round over the vertices, with the neighbours taken from the
+// edges (up(0,1), down(1,0)): in a simplicial mesh two vertices share an
+// element exactly when they share an edge, so the other ends of a vertex's
+// edges are, as a set, its star star(0,dim), which is then not built.
+__global__ __launch_bounds__(BLOCK) void indset_round_verts(u32 n, u32 const* __restrict__ uo,
+                                                            u32 const* __restrict__ ua, u32 const* __restrict__ ud,
+                                                            u32 const* __restrict__ ev,
+                                                            double const* __restrict__ good,
+                                                            u32 const* __restrict__ old, u32* __restrict__ state,
+                                                            u32* __restrict__ flag) {
+  u32 i = xcd_block() * BLOCK + threadIdx.x;
+  bool undecided = false;
+  if (i < n) {
+    u32 s = old[i];
+    if (s == UNKNOWN) {
+      double const myg = good[i];
+      bool wait = false, out = false;
+      for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
+        u32 o = ev[2 * (size_t) ua[j] + (1 - ud[j])];
+        u32 so = old[o];
+        if (so == IN_SET) {
+          out = true;
+          break;
+        }
+        if (so == NOT_IN_SET || wait) continue;
+        double og = good[o];
+        if ((myg == og && o < i) || myg < og) wait = true;
+      }
+      s = out ? (u32) NOT_IN_SET : (wait ? (u32) UNKNOWN : (u32) IN_SET);
+      undecided = !out && wait;
+    }
+    state[i] = s;
+  }
+  wave_flag(flag, undecided);
+}
+
 // rounds == 0: rounds in batches with a host check after each, up to the
 // limit (the reference's loop); rounds > 0: exactly that many rounds and no
 // host check, *last_flag then points at the device word that is nonzero if
@@ -181,13 +217,18 @@ __global__ __launch_bounds__(BLOCK) void indset_round_tets(u32 n, u32 const* __r
 static Buf<u32> indset_rounds(Mesh& m, u32 ent_dim, u32 const* filter, double const* good, u32 rounds,
                               u32 const** last_flag, bool via_tets = false) {
   u32 n = count(m, ent_dim);
+  // vertices: neighbours through the edges (indset_round_verts)
+  bool const via_edges = ent_dim == 0 && m.dim >= 2;
   Graph empty;
-  Graph const& g = via_tets ? empty : ask_star(m, ent_dim, m.dim);
-  Up const* ut = via_tets ? &ask_up(m, 1, 3) : nullptr;
-  u32 const* er = via_tets ? ask_down(m, 3, 1) : nullptr;
+  Graph const& g = (via_tets || via_edges) ? empty : ask_star(m, ent_dim, m.dim);
+  Up const* ut = via_tets ? &ask_up(m, 1, 3) : via_edges ? &ask_up(m, 0, 1) : nullptr;
+  u32 const* er = via_tets ? ask_down(m, 3, 1) : via_edges ? ask_down(m, 1, 0) : nullptr;
   auto round = [&](u32 const* in, u32* outs, u32* fl, double bytes) {
     if (via_tets)
       launch_timed("indset_round", bytes, indset_round_tets, dim3(nblocks(n)), dim3(BLOCK), n, ut->offsets.get(),
+                   ut->adj.get(), ut->dirs.get(), er, good, in, outs, fl);
+    else if (via_edges)
+      launch_timed("indset_round", bytes, indset_round_verts, dim3(nblocks(n)), dim3(BLOCK), n, ut->offsets.get(),
                    ut->adj.get(), ut->dirs.get(), er, good, in, outs, fl);
     else
       launch_timed("indset_round", bytes, indset_round, dim3(nblocks(n)), dim3(BLOCK), n, g.offsets.get(),
@@ -1061,13 +1102,16 @@ __global__ __launch_bounds__(BLOCK) void coarsen_quality_f64(CqArgs A) {
 __global__ __launch_bounds__(BLOCK) void collapse_to_vert_k(u32 nv, u32 const* __restrict__ uo,
                                                             u32 const* __restrict__ ua, u32 const* __restrict__ ud,
                                                             u32 const* __restrict__ cd, double const* __restrict__ q,
-                                                            u32* __restrict__ cand, double* __restrict__ vq,
-                                                            u32* __restrict__ inex) {
+                                                            u32 const* __restrict__ vo, u32* __restrict__ cand,
+                                                            double* __restrict__ vq, u32* __restrict__ inex) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= nv) return;
   double maxq = 0;
   u32 is = 0, mi = 0;
-  for (u32 j = uo[i]; j < uo[i + 1]; ++j) {
+  // vo: the candidate directions' offsets per vertex; a vertex that had
+  // none has none left and reads nothing
+  u32 const jb = (vo && vo[i] == vo[i + 1]) ? uo[i + 1] : uo[i];
+  for (u32 j = jb; j < uo[i + 1]; ++j) {
     u32 e = ua[j], d = ud[j];
     u32 c = cd[e];
     if (!collapses(c, d)) continue;
@@ -1098,7 +1142,7 @@ __global__ __launch_bounds__(BLOCK) void collapse_to_vert_k(u32 nv, u32 const* _
 __global__ __launch_bounds__(BLOCK) void coarsen_certify(u32 nv, u32 const* __restrict__ uo,
                                                          u32 const* __restrict__ ua, u32 const* __restrict__ ud,
                                                          u32 const* __restrict__ cd, double const* __restrict__ q,
-                                                         u32 const* __restrict__ so, u32 const* __restrict__ sa,
+                                                         u32 const* __restrict__ ev,
                                                          u32 const* __restrict__ cand, double const* __restrict__ vq,
                                                          u32 const* __restrict__ inex, u32* __restrict__ mark) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
@@ -1119,9 +1163,11 @@ __global__ __launch_bounds__(BLOCK) void coarsen_certify(u32 nv, u32 const* __re
       }
     }
     if (near > 1 && near_inex) mk = 1;
+    // the neighbours the independent set compares: the other ends of the
+    // vertex's edges (the vertex star, as a set)
     u32 const mine = inex[i];
-    for (u32 k = so[i]; !mk && k < so[i + 1]; ++k) {
-      u32 o = sa[k];
+    for (u32 k = uo[i]; !mk && k < uo[i + 1]; ++k) {
+      u32 o = ev[2 * (size_t) ua[k] + (1 - ud[k])];
       if (!cand[o]) continue;
       double dd = vq[o] - best;
       if (!(dd > m || dd < -m) && (mine || inex[o])) mk = 1;
@@ -1253,7 +1299,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   // work, and neighbouring groups share a vertex's star (the count, read
   // once, is also the early return when the class checks leave nothing)
   u32 ncand;
-  Buf<u32> clist;
+  Buf<u32> clist, voff;
   u32 const nv0 = count(m, 0);
   {
     Up const& uv = ask_up(m, 0, 1);
@@ -1261,7 +1307,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32 const* ua = uv.adj.get();
     u32 const* ud = uv.dirs.get();
     u32 const* cp = codes.get();
-    Buf<u32> voff(nv0 + 1);
+    voff = Buf<u32>(nv0 + 1);
     ncand = exscan_fn_total(nv0, OHX_LAMBDA(u32 v) {
       u32 c = 0;
       for (u32 k = uo[v]; k < uo[v + 1]; ++k) c += collapses(cp[ua[k]], ud[k]);
@@ -1273,8 +1319,12 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32* cl = clist.get();
     parallel_for(nv0, OHX_LAMBDA(u32 v) {
       u32 o = vo[v];
-      for (u32 k = uo[v]; k < uo[v + 1]; ++k)
-        if (collapses(cp[ua[k]], ud[k])) cl[o++] = 2 * ua[k] + ud[k];
+      if (o == vo[v + 1]) return;  // no direction from this vertex: nothing to read
+      u32 const b = uo[v], e = uo[v + 1];
+      for (u32 k = b; k < e; ++k) {
+        u32 const ed = ua[k], d = ud[k];
+        if (collapses(cp[ed], d)) cl[o++] = 2 * ed + d;
+      }
     });
   }
   Buf<double> eq;
@@ -1348,7 +1398,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       auto to_vert = [&] {
         if (!nv) return;
         hipLaunchKernelGGL(collapse_to_vert_k, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
-                           uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), cand.get(), vq.get(),
+                           uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), voff.get(), cand.get(), vq.get(),
                            certified ? vinex.get() : nullptr);
         check_launch("collapse_to_vert");
       };
@@ -1356,13 +1406,11 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       if (certified && nv) {
         // the comparisons the FP32 goodness takes part in: the vertices
         // whose choices are not certified get exact values (their inexact
-        // directions measured in FP64), then their best again; the vertex
-        // star is derived here, for this variant only
-        Graph const* sg = &ask_star(m, 0, dim);
+        // directions measured in FP64), then their best again
         vmark = Buf<u32>(nv);
         hipLaunchKernelGGL(coarsen_certify, dim3(nblocks(nv)), dim3(BLOCK), 0, stream(), nv, uv.offsets.get(),
-                           uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), sg->offsets.get(), sg->adj.get(),
-                           cand.get(), vq.get(), vinex.get(), vmark.get());
+                           uv.adj.get(), uv.dirs.get(), codes.get(), cq.get(), ev, cand.get(), vq.get(), vinex.get(),
+                           vmark.get());
         check_launch("coarsen_certify");
         CqArgs f = a;
         f.vmark = vmark.get();
