@@ -99,10 +99,10 @@ TRAFFIC = {
     ("c2", "scan"): 2994598,
     ("c2", "indset_round"): 2158592,
     ("c2", "reflect_down"): 55894426,
-    ("c3", "reflect_down"): 871991724,
-    ("c3", "scan"): 30890107,
-    ("c5", "lows_hash_insert"): 4283247102,
-    ("c5", "reflect_down"): 17995302541,
+    ("c3", "reflect_down"): 871959533,
+    ("c3", "scan"): 30494655,
+    ("c5", "lows_hash_insert"): 4283248041,
+    ("c5", "reflect_down"): 17986620014,
 }
 
 
