@@ -1658,7 +1658,11 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
     launch_timed("coarsen_size_code_f64", prof_bytes, coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
                        ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words(), tag);
   }
-  if (read_word(W_ANY) != tag) return 0;
+  // check_coarsen_noop's and check_coarsen_class's early returns
+  // (coarsen_common.cpp:109-139) are the one count of the directions left
+  // after the class checks (coarsen_common's first host read): an empty
+  // count returns 0 there, so a productive pass waits on the host once less
+  (void) tag;
   if (g_batch) {
     batch_mask(m, 1, codes.get());
     Buf<double> qfe = batch_floors(m, 1);
@@ -1685,7 +1689,8 @@ u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers, double const* eq) {
     c[i] = r;
   });
   if (g_batch) batch_mask(m, 1, c);
-  if (max_u32(c, ne) == DONT_COLLAPSE) return 0;
+  // check_coarsen_noop (coarsen_common.cpp:109-118) is coarsen_common's
+  // count of the directions left (its first host read)
   return coarsen_common(m, codes, 0.0, true, nullptr, eq);
 }
 
