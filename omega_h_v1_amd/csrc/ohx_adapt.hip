@@ -2293,7 +2293,10 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       run(0, nullptr, nullptr);
   }
   eq.release();
-  if (read_word(W_ANY) != tag) return 0;
+  // the early return after the qualities (swap.cpp:119-124) is the totals
+  // read below: the quality kernel clears every rejected candidate, so
+  // with none accepted the set is empty and no element is generated
+  (void) tag;
   // a first batch of indset rounds with no host check, every dimension's
   // product and survivor offsets, then one read of the convergence flag
   // and all the totals (a set not settled in the first batch goes on from
@@ -2383,6 +2386,8 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   if (g_batch) batch_note_changed(m, 1, indset.get());
   cand.release();
   q.release();
+  // every swapped edge generates elements, so none generated: nothing swapped
+  if (!tots[1 + 2 * (m.dim - dlo)]) return 0;
   u32 total = g_verbose ? sum_u32(indset.get(), ne) : 0u;
   u32 nv = count(m, 0);
   Mesh mo(m.dim, m.full);
