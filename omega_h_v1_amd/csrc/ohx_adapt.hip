@@ -1658,11 +1658,11 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
     launch_timed("coarsen_size_code_f64", prof_bytes, coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
                        ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words(), tag);
   }
-  // check_coarsen_noop's and check_coarsen_class's early returns
-  // (coarsen_common.cpp:109-139) are the one count of the directions left
-  // after the class checks (coarsen_common's first host read): an empty
-  // count returns 0 there, so a productive pass waits on the host once less
-  (void) tag;
+  // check_coarsen_noop (coarsen_common.cpp:109-118), kept as its own read:
+  // without it the last, unproductive pass of every coarsening loop would
+  // derive the maps of the class checks for nothing, which on small meshes
+  // costs more than the host wait it saves on each productive pass
+  if (read_word(W_ANY) != tag) return 0;
   if (g_batch) {
     batch_mask(m, 1, codes.get());
     Buf<double> qfe = batch_floors(m, 1);
