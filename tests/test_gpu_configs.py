@@ -55,6 +55,8 @@ def product():
     d.ohx_tuning_force.argtypes = [C.c_char_p, C.c_int]
     d.ohx_mesh_diff.restype = C.c_uint
     d.ohx_mesh_diff.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_uint]
+    d.ohx_indset_rounds_max.argtypes = [C.c_int]
+    d.ohx_indset_rounds_max.restype = C.c_uint
     d.ohx_set_indset_max_rounds(ROUNDS)
     yield lib
     d.ohx_set_indset_max_rounds(0)
@@ -106,6 +108,7 @@ def test_c3_parameters_parity_fp64_and_mixed(product, oracle):
     """C3's field on an 8^3 box: FP64 and FP32-mixed (screen pinned on) both
     equal the oracle bit for bit after every step."""
     d = product.dll
+    d.ohx_indset_rounds_max(1)
     gens = {"fp64": _c3_steps(product, 8, 2, 50, True), "mixed": _c3_steps(product, 8, 2, 50, True)}
     go = _c3_steps(oracle, 8, 2, 50, False)
     d.ohx_mixed_stats_reset()
@@ -128,6 +131,9 @@ def test_c3_parameters_parity_fp64_and_mixed(product, oracle):
     err, screened, reeval = _stats(d)
     assert err < MARGIN / 10
     assert screened > 0 and reeval > 0
+    # some independent sets here outlast the passes' first batch of four
+    # rounds, so their continuation is compared with the oracle too
+    assert d.ohx_indset_rounds_max(1) > 4
 
 
 def _digest(m):
@@ -160,6 +166,7 @@ def test_c3_full_size_mixed_takes_fp64_decisions(product):
     cycle, and the identical final mesh, with the screen settling decisions
     in FP32 and its error far inside the margin."""
     d = product.dll
+    d.ohx_indset_rounds_max(1)
     m64, c64 = _c3_full(product, False)
     d.ohx_mixed_stats_reset()
     m32, c32 = _c3_full(product, True)
@@ -169,6 +176,9 @@ def test_c3_full_size_mixed_takes_fp64_decisions(product):
     assert _digest(m64) == _digest(m32)
     assert err < MARGIN / 10
     assert screened > 0
+    # sets that outlast the passes' first batch of rounds (and a checked
+    # batch after it) are part of this run: their continuation is exercised
+    assert d.ohx_indset_rounds_max(1) > 12
 
 
 # ------------------------------------------------------------------ C4
