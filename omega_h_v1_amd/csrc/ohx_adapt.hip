@@ -2460,13 +2460,13 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   return 1;
 }
 
-u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq) {
+u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq, double const* gate) {
   OHX_REQUIRE(m.dim == 3, "swap_slivers needs a 3D mesh");
   // the pass reads up(1,3) (the edge marks below, the rings, the quality
   // kernel); derived first, it also gives the sliver layers' element dual
   // from the rings (dual_edges_k) instead of a hash table of the faces
   (void) ask_up(m, 1, 3);
-  Buf<u32> sl = mark_slivers(m, good_qual, nlayers, eq);
+  Buf<u32> sl = mark_slivers(m, good_qual, nlayers, eq, gate);
   Buf<u32> c = mark_down(m, m.dim, 1, sl.get());
   sl.release();
   if (g_batch) batch_mask(m, 1, c.get());
@@ -2531,7 +2531,8 @@ bool incr_op(Mesh& m, Ctx& c) {  // adapt.cpp:36-45
   return true;
 }
 enum Kind { REFINE, COARSEN, SWAP, COARSEN_SLIVERS };
-u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n, double const* eq = nullptr) {
+u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n, double const* eq = nullptr,
+             double const* gate = nullptr) {
   double tin = count(m, m.dim);
   static bool const trace = std::getenv("OHX_TRACE_DERIVE") && std::getenv("OHX_TRACE_DERIVE")[0] == '1';
   if (trace) std::fprintf(stderr, "pass %s n=%.0f\n", k == REFINE ? "refine" : k == COARSEN ? "coarsen" : k == SWAP ? "swap" : "csliv", tin);
@@ -2541,7 +2542,7 @@ u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n, double const* e
   switch (k) {
     case REFINE: ok = refine_by_size(m, a); break;
     case COARSEN: ok = coarsen_by_size(m, a, b); break;
-    case SWAP: ok = swap_slivers(m, a, n, eq); break;
+    case SWAP: ok = swap_slivers(m, a, n, eq, gate); break;
     case COARSEN_SLIVERS: ok = coarsen_slivers(m, a, n, eq); break;
   }
   OHX_CHECK(hipEventRecord(e1, stream()));
@@ -2574,11 +2575,27 @@ int adapt(Mesh& m, double size_floor, double good_qual, u32 nlayers, u32 max_ops
     // read them (a pass that changes nothing leaves them valid for the next)
     u32 const nel = count(m, m.dim);
     Buf<double> eq = mesh_qualities(m);
-    double prev = nel > 1 ? min_f64(eq.get(), nel, 1) : 1.7976931348623157e308;
-    if (prev >= good_qual) break;
-    if (m.dim == 3 && run_pass(m, c, SWAP, good_qual, 0, nlayers, eq.get())) {
-      if (!incr_op(m, c)) aborted = true;
-      continue;
+    if (m.dim == 3) {
+      // the minimum (over elements 1..n-1, as the serial doubles_min)
+      // reaches the host with the swap pass's first wait instead of a wait
+      // of its own: the pass's sliver marks are gated on it on the device,
+      // so when it is at least good_qual the pass marks nothing, changes
+      // nothing, and the loop stops where the reference's stops
+      Buf<double> mq(1);
+      min_f64_async(eq.get(), nel, 1, mq.get());
+      u32 const swapped = run_pass(m, c, SWAP, good_qual, 0, nlayers, eq.get(), mq.get());
+      double const prev = min_f64_async_result();
+      if (prev >= good_qual) {
+        OHX_REQUIRE(!swapped, "shape loop: a swap pass ran on a mesh of good quality");
+        break;
+      }
+      if (swapped) {
+        if (!incr_op(m, c)) aborted = true;
+        continue;
+      }
+    } else {
+      double prev = nel > 1 ? min_f64(eq.get(), nel, 1) : 1.7976931348623157e308;
+      if (prev >= good_qual) break;
     }
     if (run_pass(m, c, COARSEN_SLIVERS, good_qual, 0, nlayers, eq.get())) {
       if (!incr_op(m, c)) aborted = true;

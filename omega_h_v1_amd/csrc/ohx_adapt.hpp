@@ -14,7 +14,7 @@ void uniformly_refine(Mesh& m);
 u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor);
 // eq: the mesh's element qualities when the caller has them (else computed)
 u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers, double const* eq = nullptr);
-u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr);
+u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr, double const* gate = nullptr);
 
 struct AdaptStats {
   double npass = 0;   // passes that changed the mesh

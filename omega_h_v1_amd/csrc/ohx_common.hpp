@@ -79,7 +79,7 @@ void sync();
 // the few scalars the host needs (scan totals, reductions, error codes)
 // straight into them, so a read costs one stream synchronisation and no
 // copy.  Slot 0 is the device error word, checked by every sync().
-enum Slot : unsigned { SLOT_ERROR = 0, SLOT_SCAN = 1, SLOT_REDUCE = 2, SLOT_GATHER = 8, SLOT_STAGE = 32 };
+enum Slot : unsigned { SLOT_ERROR = 0, SLOT_SCAN = 1, SLOT_REDUCE = 2, SLOT_MIN_ASYNC = 3, SLOT_GATHER = 8, SLOT_STAGE = 32 };
 template <class T>
 inline T* host_slot(unsigned k) { return reinterpret_cast<T*>(static_cast<char*>(rt().pinned) + 8 * k); }
 template <class T>

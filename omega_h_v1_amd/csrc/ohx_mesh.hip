@@ -1270,12 +1270,12 @@ __global__ __launch_bounds__(BLOCK) void mark_sliver_f64(u32 n, u32 const* __res
   out[i] = (elem_q<DIM>(i, ev, x, mt) < good) ? 1u : 0u;
 }
 
-Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq) {
+Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq, double const* gate) {
   u32 ne = count(m, m.dim);
   Buf<u32> s(ne);
   if (ne && eq) {  // the qualities are given: the threshold alone
     u32* o = s.get();
-    parallel_for(ne, OHX_LAMBDA(u32 i) { o[i] = eq[i] < good_qual ? 1u : 0u; });
+    parallel_for(ne, OHX_LAMBDA(u32 i) { o[i] = (eq[i] < good_qual && (!gate || *gate < good_qual)) ? 1u : 0u; });
   } else if (ne) {
     double prof_bytes = (double) ne * (m.dim + 1) * 4 + (double) count(m, 0) * 24 + (double) ne * 4;
     if (m.dim == 3)

@@ -93,7 +93,10 @@ Buf<u32> mark_up(Mesh& m, u32 ld, u32 hd, u32 const* marked_lows);
 void mark_dual_layers(Mesh& m, Buf<u32>& marked, u32 nlayers);
 Buf<u32> mark_class(Mesh& m, u32 ent_dim, u32 target_dim, u32 target_id);
 void unmark_boundary(Mesh& m, u32 ent_dim, u32* marked);
-Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr);
+// eq: the element qualities when the caller has them; gate (device, with
+// eq): nothing is marked unless *gate < good_qual
+Buf<u32> mark_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr,
+                      double const* gate = nullptr);
 Buf<u32> mark_part_boundary(Mesh& m);
 
 // topology primitives (exposed through the kernel C ABI too)

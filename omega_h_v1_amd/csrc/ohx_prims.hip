@@ -140,6 +140,34 @@ u32 sum_u32(u32 const* a, u32 n) { return reduce<u32, SumOp<u32>>(a, n, 0, 0u); 
 double min_f64(double const* a, u32 n, u32 begin) {
   return reduce<double, MinOp<double>>(a, n, begin, DBL_MAX);
 }
+
+namespace {
+__global__ __launch_bounds__(256) void min_final_both(double const* part, u32 np, double* dev_out,
+                                                      double* host_out) {
+  __shared__ double lds[4];
+  double v = DBL_MAX;
+  for (u32 i = threadIdx.x; i < np; i += 256) v = MinOp<double>::apply(v, part[i]);
+  v = block_reduce<double, MinOp<double>>(v, lds);
+  if (threadIdx.x == 0) {
+    *dev_out = v;
+    *host_out = v;
+  }
+}
+}  // namespace
+
+void min_f64_async(double const* a, u32 n, u32 begin, double* dev_out) {
+  u32 nb = n > begin ? nblocks(n - begin, 256) : 0;
+  if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+  Buf<double> part(nb ? nb : 1);
+  if (nb)
+    hipLaunchKernelGGL((reduce_partial<double, MinOp<double>>), dim3(nb), dim3(256), 0, stream(), a, begin, n,
+                       DBL_MAX, part.get());
+  hipLaunchKernelGGL(min_final_both, dim3(1), dim3(256), 0, stream(), part.get(), nb, dev_out,
+                     dev_slot<double>(SLOT_MIN_ASYNC));
+  check_launch("min_f64_async");
+}
+
+double min_f64_async_result() { return *host_slot<double>(SLOT_MIN_ASYNC); }
 double max_f64(double const* a, u32 n, u32 begin) {
   return reduce<double, MaxOp<double>>(a, n, begin, -DBL_MAX);
 }

@@ -18,6 +18,11 @@ u32 sum_u32(u32 const* a, u32 n);
 // min / max over a[1..n-1]: the serial loop build starts at index 1
 // (doubles.cpp:100-118); begin selects the first index
 double min_f64(double const* a, u32 n, u32 begin);
+// the minimum of a[begin, n) (DBL_MAX if empty) into dev_out and into a
+// mapped host slot, with no wait: min_f64_async_result() holds it after the
+// stream's next synchronisation
+void min_f64_async(double const* a, u32 n, u32 begin, double* dev_out);
+double min_f64_async_result();
 double max_f64(double const* a, u32 n, u32 begin);
 
 Buf<u32> filled(u32 n, u32 v);
