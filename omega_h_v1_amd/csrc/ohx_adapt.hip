@@ -1356,7 +1356,6 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     eqp = eq_given ? eq_given : eq.get();
   }
   Buf<double> cq((size_t) 2 * ne);
-  u32 tag = next_any_tag();
   Buf<u32> cand;
   Buf<double> vq;
   {
@@ -1441,7 +1440,6 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   // no host check here: if the quality check left no collapse, the
   // candidates below are empty, the independent set settles at once and the
   // one read of its total returns 0 (check_coarsen_quality's early return)
-  (void) tag;
   u32 nv = count(m, 0);
   Up const& uv = ask_up(m, 0, 1);
   eq.release();
@@ -2296,7 +2294,6 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   // the early return after the qualities (swap.cpp:119-124) is the totals
   // read below: the quality kernel clears every rejected candidate, so
   // with none accepted the set is empty and no element is generated
-  (void) tag;
   // a first batch of indset rounds with no host check, every dimension's
   // product and survivor offsets, then one read of the convergence flag
   // and all the totals (a set not settled in the first batch goes on from
