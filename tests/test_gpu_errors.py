@@ -1,6 +1,6 @@
 """Error returns at the reference's abort points, through the product's C ABI.
 
-The reference aborts where an adapt cannot go on (SURVEY's error paths):
+The reference aborts where an adapt cannot go on:
 the independent set not settling within its round limit
 (indset.cpp:107-116, LOOP_NORETURN after MAX_ITERATIONS = 100 rounds) and a
 split child of non-positive quality (refine_qualities.cpp:72,
