@@ -271,14 +271,34 @@ __global__ __launch_bounds__(BLOCK) void upfd_rank_split(u32 nlows, u32 deg, u32
   }
 }
 
+// Ascending sort of one value per lane across a segment of S lanes
+// (bitonic network over lane permutes within the segment).
+template <u32 S>
+__device__ inline u32 segment_sort(u32 v, u32 lane) {
+#pragma unroll
+  for (u32 k = 2; k <= S; k <<= 1)
+#pragma unroll
+    for (u32 j = k >> 1; j > 0; j >>= 1) {
+      u32 o = (u32) __shfl_xor((int) v, (int) j, (int) S);
+      bool const keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+      v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
+    }
+  return v;
+}
+
 // S lanes per low entity, the bucket held in the lanes: lane j of a segment
 // holds the bucket's j-th use, and each use's rank is counted against the
 // whole bucket read across the segment with lane permutes (no memory reads
-// in the loop).  Buckets longer than S fall back to the loop over the fill
-// buffer.  Every lane of the wave takes part in the permutes.
+// in the loop).  Every lane of the wave takes part in those permutes.  A
+// bucket longer than S is sorted in chunks of S uses in place in the fill
+// buffer (each chunk by the segment's lanes), and a use's rank is then its
+// place in its own chunk plus, for every other chunk, the count of smaller
+// uses found by binary search: k (k/S) log S reads instead of the k^2 of
+// counting against the whole bucket.  (The fill buffer only holds each
+// bucket's uses in some order, so sorting it in place changes no result.)
 template <u32 S>
 __global__ __launch_bounds__(BLOCK) void upfd_rank_seg(u32 nlows, u32 deg, u32 const* __restrict__ offs,
-                                                       u32 const* __restrict__ tmp, u32* __restrict__ adj,
+                                                       u32* __restrict__ tmp, u32* __restrict__ adj,
                                                        u32* __restrict__ dirs) {
   size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
   size_t grp = gid / S;
@@ -308,10 +328,30 @@ __global__ __launch_bounds__(BLOCK) void upfd_rank_seg(u32 nlows, u32 deg, u32 c
     }
     return;
   }
-  for (u32 j = f + lane; j < e; j += S) {
-    u32 w = tmp[j];
-    u32 rr = 0;
-    for (u32 q = f; q < e; ++q) rr += tmp[q] < w ? 1u : 0u;
+  // the segment's lanes run these loops together (same bucket, same trip
+  // counts), so the permutes of segment_sort stay within active lanes
+  for (u32 c = f; c < e; c += S) {
+    u32 const idx = c + lane;
+    u32 w = idx < e ? tmp[idx] : INVALID;
+    w = segment_sort<S>(w, lane);
+    if (idx < e) tmp[idx] = w;
+  }
+  __threadfence_block();
+  for (u32 c = f; c < e; c += S) {
+    u32 const idx = c + lane;
+    if (idx >= e) continue;
+    u32 const w = tmp[idx];
+    u32 rr = lane;
+    for (u32 c2 = f; c2 < e; c2 += S) {
+      if (c2 == c) continue;
+      u32 lo = c2, hi = c2 + S < e ? c2 + S : e;
+      while (lo < hi) {
+        u32 mid = (lo + hi) >> 1;
+        if (tmp[mid] < w) lo = mid + 1;
+        else hi = mid;
+      }
+      rr += lo - c2;
+    }
     u32 h = w / deg;
     adj[f + rr] = h;
     dirs[f + rr] = w - h * deg;
