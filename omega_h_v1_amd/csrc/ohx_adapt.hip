@@ -670,7 +670,7 @@ static void refine_dom_offsets(Mesh& m, u32 const* gen_off, Buf<u32>* off) {
     u32 const* ed = ask_down(m, d, 1);
     u32 spd = h_down_deg[d][1];
     off[d] = Buf<u32>(n + 1);
-    exscan_fn(n, OHX_LAMBDA(u32 i) {
+    exscan_fn_heavy(n, OHX_LAMBDA(u32 i) {
       for (u32 j = 0; j < spd; ++j) {
         u32 e = ed[(size_t) i * spd + j];
         if (gen_off[e] != gen_off[e + 1]) return 1u;
@@ -1314,7 +1314,10 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   // the collapse directions left, listed vertex by vertex (each vertex's
   // edges in up(0,1) order): the quality kernel's groups then all have
   // work, and neighbouring groups share a vertex's star (the count, read
-  // once, is also the early return when the class checks leave nothing)
+  // once, is also the early return when the class checks leave nothing).
+  // The list is the uses of up(0,1) that collapse, in use order, so one
+  // flat scan over the uses numbers them (no per-vertex loop), and a
+  // vertex's first direction is the scan at its first use.
   u32 ncand;
   Buf<u32> clist, voff;
   u32 const nv0 = count(m, 0);
@@ -1324,24 +1327,18 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     u32 const* ua = uv.adj.get();
     u32 const* ud = uv.dirs.get();
     u32 const* cp = codes.get();
-    voff = Buf<u32>(nv0 + 1);
-    ncand = exscan_fn_total(nv0, OHX_LAMBDA(u32 v) {
-      u32 c = 0;
-      for (u32 k = uo[v]; k < uo[v + 1]; ++k) c += collapses(cp[ua[k]], ud[k]);
-      return c;
-    }, voff.get());
+    u32 const nuse = 2 * ne;
+    Buf<u32> uoff(nuse + 1);
+    ncand = exscan_fn_total(nuse, OHX_LAMBDA(u32 k) { return collapses(cp[ua[k]], ud[k]); }, uoff.get());
     if (!ncand) return 0;
     clist = Buf<u32>(ncand);
-    u32 const* vo = voff.get();
+    voff = Buf<u32>(nv0 + 1);
+    u32 const* uf = uoff.get();
     u32* cl = clist.get();
-    parallel_for(nv0, OHX_LAMBDA(u32 v) {
-      u32 o = vo[v];
-      if (o == vo[v + 1]) return;  // no direction from this vertex: nothing to read
-      u32 const b = uo[v], e = uo[v + 1];
-      for (u32 k = b; k < e; ++k) {
-        u32 const ed = ua[k], d = ud[k];
-        if (collapses(cp[ed], d)) cl[o++] = 2 * ed + d;
-      }
+    u32* vo = voff.get();
+    parallel_for(nuse > nv0 ? nuse : nv0 + 1, OHX_LAMBDA(u32 k) {
+      if (k < nuse && uf[k] != uf[k + 1]) cl[uf[k]] = 2 * ua[k] + ud[k];
+      if (k <= nv0) vo[k] = uf[uo[k]];
     });
   }
   Buf<double> eq;
@@ -2348,7 +2345,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
         u32 const* fv = ask_down(m, 2, 0);
         unsigned long long const* tb = pairs.get();
         u32 const mask = pcap - 1;
-        exscan_fn(n, OHX_LAMBDA(u32 i) {
+        exscan_fn_heavy(n, OHX_LAMBDA(u32 i) {
           u32 const* v = fv + 3 * (size_t) i;
           for (u32 j = 0; j < 3; ++j) {
             unsigned long long key = swap_pair_key(v[j], v[(j + 1) % 3]);
@@ -2363,7 +2360,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       } else {
         u32 lpe = h_down_deg[d][1];
         u32 const* le = ask_down(m, d, 1);
-        exscan_fn(n, OHX_LAMBDA(u32 i) {
+        exscan_fn_heavy(n, OHX_LAMBDA(u32 i) {
           for (u32 j = 0; j < lpe; ++j)
             if (is[le[(size_t) i * lpe + j]]) return 0u;
           return 1u;

@@ -256,4 +256,23 @@ u32 exscan_fn_total(u32 n, F f, u32* out) {
   return *host_slot<u32>(SLOT_SCAN);
 }
 
+// exscan_fn for a functor whose items each take several dependent memory
+// round trips (a loop over an entity's edges, hash probes).  Inside the scan
+// kernel a thread evaluates its 4 or 16 items one after another, so those
+// round trips add up per thread; here the items are first evaluated one per
+// thread into a buffer, where the whole grid overlaps them, and the buffer
+// is then scanned: one more launch and 8 bytes per item of traffic.  Below
+// one scan tile the functor is scanned as it is.
+template <class F>
+void exscan_fn_heavy(u32 n, F f, u32* out) {
+  if (n < (1u << 12)) {
+    exscan_fn(n, f, out);
+    return;
+  }
+  Buf<u32> tmp(n);
+  u32* t = tmp.get();
+  parallel_for(n, OHX_LAMBDA(u32 i) { t[i] = f(i); });
+  exscan_fn(n, OHX_LAMBDA(u32 i) { return t[i]; }, out);
+}
+
 }  // namespace ohx
