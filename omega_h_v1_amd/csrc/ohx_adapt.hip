@@ -1809,16 +1809,18 @@ void swap_int_edges_host(u32 rs, u32 code, u32* out) {
 }
 
 // find_edge_ring (edge_ring.cpp:12-74): ring vertices ordered by chaining
-// each tet's opposite edge (a -> b), insertion style, <= 7 entries
+// each tet's opposite edge (a -> b), insertion style, <= 7 entries; with
+// ring_u, also the use (index into up(1,3)) of the tet at each ring slot k,
+// the tet holding ring vertices k and k + 1
 __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 const* ud, u32 const* ev,
-                                u32 const* vt, u32* edge_v, u32* ring_v) {
+                                u32 const* vt, u32* edge_v, u32* ring_v, u32* ring_u = nullptr) {
   u32 f = uo[edge], e = uo[edge + 1];
   u32 rs = e - f;
   if (rs > MAX_EDGE_SWAP || rs < 3) return rs;
   u32 const* oe = opp(3, 1);
   edge_v[0] = ev[2 * (size_t) edge];
   edge_v[1] = ev[2 * (size_t) edge + 1];
-  u32 ra[MAX_EDGE_SWAP], rb[MAX_EDGE_SWAP];
+  u32 ra[MAX_EDGE_SWAP], rb[MAX_EDGE_SWAP], ru[MAX_EDGE_SWAP];
   for (u32 i = f; i < e; ++i) {
     u32 const* tv = vt + 4 * (size_t) ua[i];
     u32 te = ud[i];
@@ -1827,19 +1829,24 @@ __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 cons
     u32 const* tvo = canon(3, 1, 0, oe[te]);
     ra[i - f] = tv[tvo[al]];
     rb[i - f] = tv[tvo[al ^ 1u]];
+    ru[i - f] = i;
   }
   for (u32 i = 0; i + 1 < rs; ++i) {
     u32 j;
     for (j = i + 1; j < rs; ++j)
       if (ra[j] == rb[i]) break;
     if (j == rs) return 0;  // ring does not close: reported by the caller
-    u32 ta = ra[j], tb = rb[j];
+    u32 ta = ra[j], tb = rb[j], tu = ru[j];
     ra[j] = ra[i + 1];
     rb[j] = rb[i + 1];
+    ru[j] = ru[i + 1];
     ra[i + 1] = ta;
     rb[i + 1] = tb;
+    ru[i + 1] = tu;
   }
   for (u32 i = 0; i < rs; ++i) ring_v[i] = ra[i];
+  if (ring_u)
+    for (u32 i = 0; i < rs; ++i) ring_u[i] = ru[i];
   return rs;
 }
 
@@ -2101,17 +2108,42 @@ static inline __host__ __device__ u32 count_swap_ents(u32 rs, u32 d, u32 nint, u
   return d == 1 ? nint : (d == 2 ? tpm + 2 * nint : 2 * tpm);
 }
 
+// The swapped mesh's tet-to-edge map, built from the old one instead of by
+// reflection (reflect_down.cpp:13-81 gives the same map: an entity's lows
+// are unique in a conforming mesh).  A surviving tet is adjacent to no
+// swapped edge, so its six edges survive too and it keeps them, renumbered.
+// A product tet's edges are its cavity's: ring edges and spokes (ring
+// vertex to an end of the swapped edge), both edges of the old tets around
+// the swapped edge, and the new interior edges.  out null: not built.
+struct SwapEdgesOf {
+  u32* out;            // (surviving + product tets) x 6 new edge ids
+  u32 const* old;      // the old tets' edges
+  u32 const* soff1;    // old edge -> its index among the surviving edges
+  u32 const* goff1;    // interior edges generated per swapped edge
+  u32 nsame1, nsame3;  // surviving edges, surviving tets
+  u32* err;
+};
+
+// local edge of a tet between local vertices a and b
+__device__ inline u32 tet_edge_between(u32 a, u32 b) {
+  for (u32 j = 0; j < 6; ++j) {
+    u32 const* c = canon(3, 1, 0, j);
+    if ((c[0] == a && c[1] == b) || (c[0] == b && c[1] == a)) return j;
+  }
+  return 0;
+}
+
 // swap_cavity (swap_topology.cpp:46-75) with get_swap_ents
 // (edge_swap.cpp:282-373): the ring again (integer work) and the
 // triangulation code that swap_quality chose, then the products
 __global__ __launch_bounds__(BLOCK) void swap_cavity_k(
     u32 ne, u32 d, u32 const* __restrict__ indset, u32 const* __restrict__ goff, u32 const* __restrict__ uo,
     u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
-    u32 const* __restrict__ vt, u32 const* __restrict__ codes, u32* __restrict__ out) {
+    u32 const* __restrict__ vt, u32 const* __restrict__ codes, u32* __restrict__ out, SwapEdgesOf E) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ne || !indset[i]) return;
-  u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
-  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
+  u32 edge_v[2], ring_v[MAX_EDGE_SWAP], ring_u[MAX_EDGE_SWAP];
+  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v, ring_u);
   if (rs < 3 || rs > MAX_EDGE_SWAP) return;  // cannot happen for an indset edge
   u32 const code = codes[i];
   u32 tpm = k_sw_sizes[rs];
@@ -2121,15 +2153,66 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_k(
   unsigned char const* ie = k_sw_int + k_sw_int_off[rs] + code * nint * 2;
   u32* o = out + (size_t) goff[i] * (d + 1);
   if (d == 3) {
+    // the cavity's surviving edges, new numbering: ring edge k joins ring
+    // vertices k and k + 1, spoke (k, j) ring vertex k and edge end j; the
+    // tet at ring slot k holds all of them
+    u32 ring_e[MAX_EDGE_SWAP], spoke_e[MAX_EDGE_SWAP][2];
+    if (E.out) {
+      for (u32 k = 0; k < rs; ++k) {
+        u32 const use = ring_u[k];
+        u32 const tet = ua[use];
+        u32 const* tv = vt + 4 * (size_t) tet;
+        u32 const* te = E.old + 6 * (size_t) tet;
+        ring_e[k] = E.soff1[te[opp(3, 1)[ud[use]]]];
+        u32 lk = 0, l0 = 0, l1 = 0;
+        for (u32 c = 0; c < 4; ++c) {
+          if (tv[c] == ring_v[k]) lk = c;
+          if (tv[c] == edge_v[0]) l0 = c;
+          if (tv[c] == edge_v[1]) l1 = c;
+        }
+        spoke_e[k][0] = E.soff1[te[tet_edge_between(lk, l0)]];
+        spoke_e[k][1] = E.soff1[te[tet_edge_between(lk, l1)]];
+      }
+    }
+    u32 const first = E.nsame3 + goff[i];
     for (u32 t = 0; t < tpm; ++t) {
-      u32 tv[4];
-      for (u32 j = 0; j < 3; ++j) tv[j] = ring_v[tris[mesh[t]][j]];
+      u32 tv[4], lab[4];  // vertices; labels: ring slot, or 8 + edge end
+      for (u32 j = 0; j < 3; ++j) {
+        lab[j] = tris[mesh[t]][j];
+        tv[j] = ring_v[lab[j]];
+      }
       for (u32 j = 0; j < 2; ++j) {
         tv[3] = edge_v[1 - j];
+        lab[3] = 8 + (1 - j);
         for (u32 k = 0; k < 4; ++k) *o++ = tv[k];
+        if (E.out) {
+          u32* eo = E.out + 6 * (size_t) (first + 2 * t + j);
+          for (u32 le = 0; le < 6; ++le) {
+            u32 const* c = canon(3, 1, 0, le);
+            u32 a = lab[c[0]], b = lab[c[1]];
+            u32 id = INVALID;
+            if (a >= 8 || b >= 8) {
+              if (a < 8) id = spoke_e[a][b - 8];
+              else if (b < 8) id = spoke_e[b][a - 8];
+            } else if (a + 1 == b || a == b + rs - 1) {
+              id = ring_e[a];
+            } else if (b + 1 == a || b == a + rs - 1) {
+              id = ring_e[b];
+            } else {
+              for (u32 k = 0; k < nint; ++k)
+                if ((ie[2 * k] == a && ie[2 * k + 1] == b) || (ie[2 * k] == b && ie[2 * k + 1] == a))
+                  id = E.nsame1 + E.goff1[i] + k;
+            }
+            if (id == INVALID) *E.err = ERR_REFLECT_NOT_FOUND;
+            eo[le] = id;
+          }
+        }
         u32 tmp = tv[0];
         tv[0] = tv[1];
         tv[1] = tmp;
+        tmp = lab[0];
+        lab[0] = lab[1];
+        lab[1] = tmp;
       }
     }
   } else if (d == 1) {
@@ -2173,6 +2256,7 @@ struct SwapKeepArgs {
   u32 const* goff;
   u32 const* vent;
   u32* out;
+  SwapEdgesOf E;  // tets: the survivors' edges (E.out null: not built)
   u32 const* cd;
   u32 const* ci;
   u32 const* ecd;
@@ -2190,6 +2274,8 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
     u32 o = a.soff[t];
     if (o == a.soff[t + 1]) return;
     for (u32 j = 0; j < a.vpe; ++j) a.out[(size_t) o * a.vpe + j] = a.vent[(size_t) t * a.vpe + j];
+    if (a.E.out)
+      for (u32 j = 0; j < 6; ++j) a.E.out[(size_t) o * 6 + j] = a.E.soff1[a.E.old[(size_t) t * 6 + j]];
     if (a.out_cd) a.out_cd[o] = a.cd[t];
     if (a.out_ci) a.out_ci[o] = a.ci[t];
     return;
@@ -2200,6 +2286,107 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
     if (a.out_cd) a.out_cd[a.nsame + k] = a.ecd[e];
     if (a.out_ci) a.out_ci[a.nsame + k] = a.eci[e];
   }
+}
+
+// The swapped mesh's edge-to-tet adjacency, built from the old one
+// (up_from_down.cpp:9-47 gives the same lists: per edge, its uses in
+// increasing use index tet * 6 + local edge).  Surviving tets keep their
+// relative order and their vertex order, and every product tet comes after
+// them, so a surviving edge's list is its old list without the removed
+// tets, renumbered, followed by its uses in product tets; a product edge
+// has product uses only.  Only the product uses (a few per swapped edge)
+// take atomics and a sort.
+struct SwapUpArgs {
+  u32 ne_old, nsame1, ngen1, nsame3, ngen3;
+  u32 const* uo;     // old up(1,3): offsets, tets, local edges
+  u32 const* ua;
+  u32 const* ud;
+  u32 const* soff1;  // old edge / tet -> index among the survivors
+  u32 const* soff3;
+  u32 const* d31;    // the new tets' edges
+  u32* cnt;          // per new edge: its uses (surviving, then + product)
+  u32* cnt_s;        // per new edge: its uses in surviving tets
+  u32* slot;         // per product use: its place in its edge's list
+  u32 const* off;    // the new offsets
+  u32* key;          // product use indices at their unsorted places
+  u32* adj;
+  u32* dirs;
+};
+
+// threads [0, ne_old): a surviving edge counts its surviving tets;
+// [ne_old, ne_old + ngen1): a product edge starts at zero
+__global__ __launch_bounds__(BLOCK) void swap_up_count_k(SwapUpArgs a) {
+  u32 t = xcd_block() * BLOCK + threadIdx.x;
+  if (t < a.ne_old) {
+    u32 n = a.soff1[t];
+    if (n == a.soff1[t + 1]) return;
+    u32 c = 0;
+    for (u32 k = a.uo[t]; k < a.uo[t + 1]; ++k) {
+      u32 h = a.ua[k];
+      c += a.soff3[h] != a.soff3[h + 1] ? 1u : 0u;
+    }
+    a.cnt[n] = c;
+    a.cnt_s[n] = c;
+  } else if (t - a.ne_old < a.ngen1) {
+    u32 n = a.nsame1 + (t - a.ne_old);
+    a.cnt[n] = 0;
+    a.cnt_s[n] = 0;
+  }
+}
+
+// a product use's edge, or INVALID where the tet's edges were not found
+// (reported by swap_cavity_k)
+__device__ inline u32 swap_up_edge(SwapUpArgs const& a, u32 u) {
+  u32 e = a.d31[(size_t) a.nsame3 * 6 + u];
+  return e < a.nsame1 + a.ngen1 ? e : INVALID;
+}
+
+// each product use takes a place after its edge's surviving uses
+__global__ __launch_bounds__(BLOCK) void swap_up_slot_k(SwapUpArgs a) {
+  u32 u = xcd_block() * BLOCK + threadIdx.x;
+  if (u >= a.ngen3 * 6) return;
+  u32 e = swap_up_edge(a, u);
+  if (e != INVALID) a.slot[u] = atomicAdd(&a.cnt[e], 1u);
+}
+
+// after the scan of the counts: threads [0, ne_old) write the surviving
+// uses of a surviving edge in order; the next ngen3 * 6 put each product
+// use's index at its place
+__global__ __launch_bounds__(BLOCK) void swap_up_fill_k(SwapUpArgs a) {
+  u32 t = xcd_block() * BLOCK + threadIdx.x;
+  if (t < a.ne_old) {
+    u32 n = a.soff1[t];
+    if (n == a.soff1[t + 1]) return;
+    u32 p = a.off[n];
+    for (u32 k = a.uo[t]; k < a.uo[t + 1]; ++k) {
+      u32 h = a.ua[k];
+      u32 s = a.soff3[h];
+      if (s == a.soff3[h + 1]) continue;
+      a.adj[p] = s;
+      a.dirs[p] = a.ud[k];
+      ++p;
+    }
+  } else if (t - a.ne_old < a.ngen3 * 6) {
+    u32 u = t - a.ne_old;
+    u32 e = swap_up_edge(a, u);
+    if (e != INVALID) a.key[a.off[e] + a.slot[u]] = a.nsame3 * 6 + u;
+  }
+}
+
+// each product use's final place: its rank by use index among its edge's
+// product uses
+__global__ __launch_bounds__(BLOCK) void swap_up_sort_k(SwapUpArgs a) {
+  u32 u = xcd_block() * BLOCK + threadIdx.x;
+  if (u >= a.ngen3 * 6) return;
+  u32 e = swap_up_edge(a, u);
+  if (e == INVALID) return;
+  u32 const b = a.off[e] + a.cnt_s[e], end = a.off[e + 1];
+  u32 const me = a.nsame3 * 6 + u;
+  u32 r = 0;
+  for (u32 k = b; k < end; ++k) r += a.key[k] < me ? 1u : 0u;
+  u32 const h = me / 6;
+  a.adj[b + r] = h;
+  a.dirs[b + r] = me - 6 * h;
 }
 
 // a vertex pair as one key (smaller vertex in the high word) and its home
@@ -2406,6 +2593,10 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   u32 const* is = indset.get();
   bool has_class = m.full && find_tag(m, 0, "class_dim") != nullptr;
   bool has_id = has_class && find_tag(m, 0, "class_id") != nullptr;
+  // the tets' edges of the new mesh, from the old map (the old one exists:
+  // the rings above read up(1,3)); the next pass asks for them first
+  bool const edges_of_tets = m.dim == 3 && dlo == 1;
+  Buf<u32> d31;
   for (u32 d = dlo; d <= m.dim; ++d) {
     u32 n = count(m, d);
     u32 vpe = d + 1;
@@ -2413,13 +2604,26 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
     Buf<u32>& soff = soffs[d];
     u32 ngen = tots[1 + 2 * (d - dlo)], nsame = tots[2 + 2 * (d - dlo)];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
+    SwapEdgesOf E{};
+    if (d == 3 && edges_of_tets) {
+      d31 = Buf<u32>((size_t) (nsame + ngen) * 6);
+      E.out = d31.get();
+      E.old = ask_down(m, 3, 1);
+      E.soff1 = soffs[1].get();
+      E.goff1 = goffs[1].get();
+      E.nsame1 = tots[2];
+      E.nsame3 = nsame;
+      E.err = dev_error_word();
+    }
     {
       double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
       launch_timed("swap_cavity", prof_bytes, swap_cavity_k, dim3(nblocks(ne)), dim3(BLOCK), ne, d, is, goff.get(),
-                   u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), vout.get() + (size_t) nsame * vpe);
+                   u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), vout.get() + (size_t) nsame * vpe,
+                   E);
     }
     Buf<u32> cdo, cio;
     SwapKeepArgs a{};
+    a.E = E;
     a.n = n;
     a.ne = ne;
     a.vpe = vpe;
@@ -2448,6 +2652,50 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
     }
     if (d == m.dim) swap_transfer(m, mo, goff.get(), soff.get(), nsame, ngen);
+  }
+  if (d31) {
+    u32 const nsame1 = tots[2], ngen1 = tots[1], nsame3 = tots[6], ngen3 = tots[5];
+    u32 const ne_new = nsame1 + ngen1, nuse = (nsame3 + ngen3) * 6;
+    auto up = std::make_unique<Up>();
+    up->offsets = Buf<u32>(ne_new + 1);
+    up->adj = Buf<u32>(nuse);
+    up->dirs = Buf<u32>(nuse);
+    Buf<u32> cnt(ne_new), cnt_s(ne_new), slot((size_t) ngen3 * 6), key(nuse);
+    SwapUpArgs a{};
+    a.ne_old = ne;
+    a.nsame1 = nsame1;
+    a.ngen1 = ngen1;
+    a.nsame3 = nsame3;
+    a.ngen3 = ngen3;
+    a.uo = u.offsets.get();
+    a.ua = u.adj.get();
+    a.ud = u.dirs.get();
+    a.soff1 = soffs[1].get();
+    a.soff3 = soffs[3].get();
+    a.d31 = d31.get();
+    a.cnt = cnt.get();
+    a.cnt_s = cnt_s.get();
+    a.slot = slot.get();
+    a.off = up->offsets.get();
+    a.key = key.get();
+    a.adj = up->adj.get();
+    a.dirs = up->dirs.get();
+    hipLaunchKernelGGL(swap_up_count_k, dim3(nblocks((size_t) ne + ngen1)), dim3(BLOCK), 0, stream(), a);
+    check_launch("swap_up_count");
+    if (ngen3) {
+      hipLaunchKernelGGL(swap_up_slot_k, dim3(nblocks((size_t) ngen3 * 6)), dim3(BLOCK), 0, stream(), a);
+      check_launch("swap_up_slot");
+    }
+    exscan(cnt.get(), ne_new, up->offsets.get());
+    hipLaunchKernelGGL(swap_up_fill_k, dim3(nblocks((size_t) ne + (size_t) ngen3 * 6)), dim3(BLOCK), 0, stream(),
+                       a);
+    check_launch("swap_up_fill");
+    if (ngen3) {
+      hipLaunchKernelGGL(swap_up_sort_k, dim3(nblocks((size_t) ngen3 * 6)), dim3(BLOCK), 0, stream(), a);
+      check_launch("swap_up_sort");
+    }
+    mo.down[3][1] = std::move(d31);
+    mo.up[1][3] = std::move(up);
   }
   say("swapped", total, 1);
   overwrite(m, std::move(mo));

@@ -66,3 +66,28 @@ def test_full_mesh_stars_and_dual(product, oracle):
             for h in range(l + 1, dim + 1):
                 _compare(mp.star(l, h), mo.star(l, h), "%s star(%d,%d)" % (name, l, h))
         assert np.array_equal(mp.dual(), mo.dual()), name + " dual"
+
+
+def _maps_match_fresh(m, what):
+    """The mesh's tet-to-edge map and edge-to-tet adjacency equal the ones a
+    copy with only the entities' vertices derives."""
+    fresh = m.clone()
+    assert np.array_equal(m.down(3, 1), fresh.down(3, 1)), what + " down(3,1)"
+    for a, b, part in zip(m.up(1, 3), fresh.up(1, 3), ("offsets", "adjacency", "directions")):
+        assert np.array_equal(a, b), what + " up(1,3) " + part
+
+
+def test_swapped_mesh_maps_match_derivation(product):
+    """A swap pass hands the next mesh its tets' edges built from the old
+    mesh's maps instead of reflecting them; on the warped 4^3 box, after
+    every swap pass that changes the mesh, they equal the derived ones."""
+    swaps = 0
+    for label, m in P.CASES["warp3d_rot_n4"](product):
+        if not label.startswith("warp"):
+            continue
+        for _ in range(4):
+            if not m.swap_slivers(0.3, 4):
+                break
+            swaps += 1
+            _maps_match_fresh(m, "%s, swap %d" % (label, swaps))
+    assert swaps >= 3
