@@ -1229,6 +1229,24 @@ __global__ __launch_bounds__(BLOCK) void subset_tags(u32 n, u32 const* __restric
   }
 }
 
+// The new entities' sides of dimension l (0 < l < d), carried from the old
+// map instead of reflected (reflect_down.cpp:13-81 gives the same map: an
+// entity's lows are unique in a conforming mesh).  A survivor keeps its
+// sides, renumbered.  A product is the old entity with its collapsing vertex
+// replaced by the target; a side without that vertex survives, and one with
+// it is either rebuilt on the target itself (a product of dimension l) or
+// fused into the side that already joins the target to the same vertices.
+struct CoarsenSides {
+  u32* out;            // new entities x sides (null: not built)
+  u32 const* old;      // the old entities' sides
+  u32 const* soff;     // old side -> its index among the surviving sides
+  u32 const* egoff;    // old side -> its index among the product sides
+  u32 const* fused_to; // fused old side -> the old side it merges into
+  u32 nsame, l;        // surviving sides, their dimension
+  u32 nold;            // old sides
+  u32* err;
+};
+
 struct CoarsenArgs {
   u32 d, n, nsame;
   u32 const* vent;
@@ -1242,7 +1260,41 @@ struct CoarsenArgs {
   u32 const* ci;
   u32* out_cd;
   u32* out_ci;
+  CoarsenSides sides;
 };
+
+// sides of dimension l of a d-simplex
+__host__ __device__ inline u32 simplex_sides(u32 d, u32 l) { return d == 3 ? (l == 1 ? 6u : 4u) : 3u; }
+
+// the new id of side c (dimension s.l) of a product: the old entity's
+// sides in row, its collapsing vertex at old local index gd; the product's
+// local vertices are the base opposite gd in canonical order, then the
+// target in gd's place
+__device__ inline u32 coarsen_product_side(u32 d, u32 c, u32 gd, u32 const* row, CoarsenSides const& s) {
+  u32 const* bv = canon(d, d - 1, 0, opp(d, 0)[gd]);
+  u32 const* cs = canon(d, s.l, 0, c);
+  u32 mask = 0;
+  bool on_target = false;
+  for (u32 q = 0; q <= s.l; ++q) {
+    u32 const nl = cs[q];
+    on_target = on_target || nl == d;
+    mask |= 1u << (nl < d ? bv[nl] : gd);
+  }
+  u32 k = 0;
+  for (u32 j = 0; j < simplex_sides(d, s.l); ++j) {
+    u32 const* os = canon(d, s.l, 0, j);
+    u32 mj = 0;
+    for (u32 q = 0; q <= s.l; ++q) mj |= 1u << os[q];
+    if (mj == mask) k = j;
+  }
+  u32 const side = row[k];
+  if (!on_target) return s.soff[side];
+  if (s.egoff[side] != s.egoff[side + 1]) return s.nsame + s.egoff[side];
+  u32 const into = s.fused_to[side];
+  if (into < s.nold) return s.soff[into];
+  *s.err = ERR_REFLECT_NOT_FOUND;  // a side on the target that is neither rebuilt nor fused
+  return INVALID;
+}
 
 // concat_verts_of_ents (inherit.cpp:234-255) + coarsen_elem
 // (coarsen_topology.cpp:9-44) + remap_conn (coarsen_common.cpp:29-33) +
@@ -1258,11 +1310,20 @@ __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
   if (a.esoff[i] != a.esoff[i + 1]) {
     o = a.esoff[i];
     for (u32 j = 0; j < vpe; ++j) a.out[(size_t) o * vpe + j] = a.vsoff[v[j]];
+    if (CoarsenSides const& s = a.sides; s.out) {
+      u32 const ns = simplex_sides(d, s.l);
+      for (u32 c = 0; c < ns; ++c) s.out[(size_t) o * ns + c] = s.soff[s.old[(size_t) i * ns + c]];
+    }
   } else if (a.egoff[i] != a.egoff[i + 1]) {
     o = a.nsame + a.egoff[i];
     u32 const* bv = canon(d, d - 1, 0, opp(d, 0)[a.gd[i]]);
     for (u32 j = 0; j + 1 < vpe; ++j) a.out[(size_t) o * vpe + j] = a.vsoff[v[bv[j]]];
     a.out[(size_t) o * vpe + vpe - 1] = a.vsoff[a.gv[i]];
+    if (CoarsenSides const& s = a.sides; s.out) {
+      u32 const ns = simplex_sides(d, s.l);
+      for (u32 c = 0; c < ns; ++c)
+        s.out[(size_t) o * ns + c] = coarsen_product_side(d, c, a.gd[i], s.old + (size_t) i * ns, s);
+    }
   } else {
     return;
   }
@@ -1302,6 +1363,194 @@ static void cq_launch_mode(u32 mode, u32 g, CqArgs const& a, double bytes, hipEv
     case CQ_CERTIFIED: cq_launch<DIM, CQ_CERTIFIED>(g, a, bytes, ea, eb); break;
     default: cq_launch<DIM, CQ_FORCE>(g, a, bytes, ea, eb); break;
   }
+}
+
+// ------------------------------------------------ maps carried across a pass
+// A pass numbers the surviving entities of every dimension first, in their
+// old relative order and vertex order, and appends its products; the lows
+// of a surviving entity survive too.  So the new mesh's upward adjacency of
+// a (low, high) pair follows from the old one (up_from_down.cpp:9-47 gives
+// the same lists: per low, its uses in increasing use index high * deg +
+// local index): a surviving low's list is its old list without the removed
+// highs, renumbered, followed by its uses in product highs; a product low
+// has product uses only.  Only the product uses (a few per cavity) take
+// atomics and a sort.
+struct UpPatch {
+  u32 deg, nl_old, nsame_l, ngen_l, nsame_h, ngen_h;
+  u32 nuse;           // (nsame_h + ngen_h) * deg: writes stay below it
+  u32 const* uo;      // old up(l,h): offsets, highs, local indices
+  u32 const* ua;
+  u32 const* ud;
+  u32 const* soff_l;  // old low / high -> index among the survivors
+  u32 const* soff_h;
+  u32 const* dn;      // the new down(h,l)
+  u32* cnt;           // per new low: its uses (surviving, then + product)
+  u32* cnt_s;         // per new low: its uses in surviving highs
+  u32* slot;          // per product use: its place in its low's list
+  u32 const* off;     // the new offsets
+  u32* key;           // product use indices at their unsorted places
+  u32* adj;
+  u32* dirs;
+};
+
+// G lanes per old low (a group): a surviving low counts its surviving
+// highs, G uses at a time; groups past nl_old start the product lows at
+// zero (lane 0 of each)
+template <u32 G>
+__global__ __launch_bounds__(BLOCK) void up_patch_count_k(UpPatch a) {
+  size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
+  u32 const t = (u32) (gid / G), lane = (u32) (gid % G);
+  if (t < a.nl_old) {
+    u32 n = a.soff_l[t];
+    if (n == a.soff_l[t + 1]) return;
+    u32 c = 0;
+    for (u32 k = a.uo[t] + lane; k < a.uo[t + 1]; k += G) {
+      u32 h = a.ua[k];
+      c += a.soff_h[h] != a.soff_h[h + 1] ? 1u : 0u;
+    }
+#pragma unroll
+    for (u32 d = G / 2; d >= 1; d >>= 1) c += (u32) __shfl_xor((int) c, (int) d, (int) G);
+    if (lane == 0) {
+      a.cnt[n] = c;
+      a.cnt_s[n] = c;
+    }
+  } else if (t - a.nl_old < a.ngen_l && lane == 0) {
+    u32 n = a.nsame_l + (t - a.nl_old);
+    a.cnt[n] = 0;
+    a.cnt_s[n] = 0;
+  }
+}
+
+// a product use's low, or INVALID where the product's lows were not found
+// (reported where they were built)
+__device__ inline u32 up_patch_low(UpPatch const& a, u32 u) {
+  u32 e = a.dn[(size_t) a.nsame_h * a.deg + u];
+  return e < a.nsame_l + a.ngen_l ? e : INVALID;
+}
+
+// each product use takes a place after its low's surviving uses
+__global__ __launch_bounds__(BLOCK) void up_patch_slot_k(UpPatch a) {
+  u32 u = xcd_block() * BLOCK + threadIdx.x;
+  if (u >= a.ngen_h * a.deg) return;
+  u32 e = up_patch_low(a, u);
+  if (e != INVALID) a.slot[u] = atomicAdd(&a.cnt[e], 1u);
+}
+
+// after the scan of the counts: groups of G lanes over the old lows write
+// a surviving low's surviving uses in order (G at a time, placed by a
+// prefix count over the group's ballot); the ngen_h * deg threads after
+// them put each product use's index at its place
+template <u32 G>
+__global__ __launch_bounds__(BLOCK) void up_patch_fill_k(UpPatch a) {
+  size_t gid = (size_t) xcd_block() * BLOCK + threadIdx.x;
+  size_t const ngroups_lanes = (size_t) a.nl_old * G;
+  if (gid < ngroups_lanes) {
+    u32 const t = (u32) (gid / G), lane = (u32) (gid % G);
+    u32 n = a.soff_l[t];
+    if (n == a.soff_l[t + 1]) return;
+    u32 const first = (u32) (threadIdx.x & 63u) & ~(G - 1u);  // the group's first lane in the wave
+    unsigned long long const below = ((1ull << lane) - 1ull) << first;
+    u32 p = a.off[n];
+    for (u32 k0 = a.uo[t]; k0 < a.uo[t + 1]; k0 += G) {
+      u32 const k = k0 + lane;
+      bool keep = false;
+      u32 s = 0;
+      if (k < a.uo[t + 1]) {
+        u32 h = a.ua[k];
+        s = a.soff_h[h];
+        keep = s != a.soff_h[h + 1];
+      }
+      unsigned long long const m = __ballot(keep);
+      u32 const q = p + (u32) __popcll(m & below);
+      if (keep && q < a.nuse) {
+        a.adj[q] = s;
+        a.dirs[q] = a.ud[k];
+      }
+      p += (u32) __popcll((m >> first) & ((G == 64 ? ~0ull : (1ull << G) - 1ull)));
+    }
+  } else if (gid - ngroups_lanes < (size_t) a.ngen_h * a.deg) {
+    u32 u = (u32) (gid - ngroups_lanes);
+    u32 e = up_patch_low(a, u);
+    if (e == INVALID) return;
+    u32 const p = a.off[e] + a.slot[u];
+    if (p < a.nuse) a.key[p] = a.nsame_h * a.deg + u;
+  }
+}
+
+// each product use's final place: its rank by use index among its low's
+// product uses
+__global__ __launch_bounds__(BLOCK) void up_patch_sort_k(UpPatch a) {
+  u32 u = xcd_block() * BLOCK + threadIdx.x;
+  if (u >= a.ngen_h * a.deg) return;
+  u32 e = up_patch_low(a, u);
+  if (e == INVALID) return;
+  u32 const b = a.off[e] + a.cnt_s[e], end = a.off[e + 1] < a.nuse ? a.off[e + 1] : a.nuse;
+  u32 const me = a.nsame_h * a.deg + u;
+  u32 r = 0;
+  for (u32 k = b; k < end; ++k) r += a.key[k] < me ? 1u : 0u;
+  if (b + r >= a.nuse) return;
+  u32 const h = me / a.deg;
+  a.adj[b + r] = h;
+  a.dirs[b + r] = me - a.deg * h;
+}
+
+// the new mesh's up(l,h) from the old mesh's: nl_old old lows, survivors
+// of both dimensions given by their offsets over the old entities, dn the
+// new down(h,l) (its product rows are read)
+static std::unique_ptr<Up> up_patch(u32 deg, Up const& old, u32 nl_old, u32 const* soff_l, u32 nsame_l,
+                                    u32 ngen_l, u32 const* soff_h, u32 nsame_h, u32 ngen_h, u32 const* dn) {
+  u32 const nl = nsame_l + ngen_l, nuse = (nsame_h + ngen_h) * deg, nprod = ngen_h * deg;
+  auto up = std::make_unique<Up>();
+  up->offsets = Buf<u32>(nl + 1);
+  up->adj = Buf<u32>(nuse);
+  up->dirs = Buf<u32>(nuse);
+  Buf<u32> cnt(nl), cnt_s(nl), slot(nprod), key(nuse);
+  UpPatch a{};
+  a.deg = deg;
+  a.nuse = nuse;
+  a.nl_old = nl_old;
+  a.nsame_l = nsame_l;
+  a.ngen_l = ngen_l;
+  a.nsame_h = nsame_h;
+  a.ngen_h = ngen_h;
+  a.uo = old.offsets.get();
+  a.ua = old.adj.get();
+  a.ud = old.dirs.get();
+  a.soff_l = soff_l;
+  a.soff_h = soff_h;
+  a.dn = dn;
+  a.cnt = cnt.get();
+  a.cnt_s = cnt_s.get();
+  a.slot = slot.get();
+  a.off = up->offsets.get();
+  a.key = key.get();
+  a.adj = up->adj.get();
+  a.dirs = up->dirs.get();
+  // compulsory bytes: the old lists and the highs' survivor flags read
+  double const old_bytes = (double) nl_old * 12 + (double) old.adj.size() * 12;
+  // lanes per old low: about its mean list length (edges in tets ~5,
+  // vertices in edges ~14, vertices in tets ~24)
+  size_t const mean = nl_old ? old.adj.size() / nl_old : 0;
+  u32 const G = mean <= 6 ? 4u : mean <= 16 ? 8u : 16u;
+  auto groups = [&](auto count_k, auto fill_k) {
+    size_t const nc = ((size_t) nl_old + ngen_l) * G;
+    if (nc) launch_timed("up_patch_count", old_bytes, count_k, dim3(nblocks(nc)), dim3(BLOCK), a);
+    if (nprod) {
+      hipLaunchKernelGGL(up_patch_slot_k, dim3(nblocks(nprod)), dim3(BLOCK), 0, stream(), a);
+      check_launch("up_patch_slot");
+    }
+    exscan(cnt.get(), nl, up->offsets.get());
+    size_t const nf = (size_t) nl_old * G + nprod;
+    if (nf) launch_timed("up_patch_fill", old_bytes + (double) nuse * 8, fill_k, dim3(nblocks(nf)), dim3(BLOCK), a);
+  };
+  if (G == 4) groups(up_patch_count_k<4>, up_patch_fill_k<4>);
+  else if (G == 8) groups(up_patch_count_k<8>, up_patch_fill_k<8>);
+  else groups(up_patch_count_k<16>, up_patch_fill_k<16>);
+  if (nprod) {
+    hipLaunchKernelGGL(up_patch_sort_k, dim3(nblocks(nprod)), dim3(BLOCK), 0, stream(), a);
+    check_launch("up_patch_sort");
+  }
+  return up;
 }
 
 static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better,
@@ -1454,7 +1703,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   // writes the outputs.
   bool has_class = find_tag(m, 0, "class_dim") != nullptr;
   bool has_id = find_tag(m, 0, "class_id") != nullptr;
-  Buf<u32> fused[4];
+  Buf<u32> fused[4], fused_to[4];
   struct DimPlan {
     Buf<u32> gv, gd, egoff, esoff;
   };
@@ -1502,15 +1751,20 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       u32 const* sent = ask_down(m, d, d - 1);
       u32 nsd = count(m, d - 1);
       fused[d - 1] = filled(nsd, 0);
+      if (d >= 2) fused_to[d - 1] = Buf<u32>(nsd);
       u32 vpe = d + 1, spe = h_down_deg[d][d - 1];
       Buf<u32> will(n), gv(n), gd(n), same(n);
       {
         // collapse_to_ent (collapses_to_ents.cpp:11-82); also marks the
-        // entities with no collapsing vertex, which survive unchanged
+        // entities with no collapsing vertex, which survive unchanged, and
+        // notes for each fused side the side it merges into (the one
+        // opposite the collapsing vertex), which the maps carried to the
+        // new mesh read
         u32 const* go = goff.get();
         u32 const* g = gvv.get();
         u32 const* fe = fused[d] ? fused[d].get() : nullptr;
         u32* fs = fused[d - 1].get();
+        u32* ft = d >= 2 ? fused_to[d - 1].get() : nullptr;
         u32* wl = will.get();
         u32* gvp = gv.get();
         u32* gdp = gd.get();
@@ -1530,7 +1784,9 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
             for (u32 j = 0; j < vpe; ++j)
               if (v[j] == target) gdir = j;
             if (gdir != INVALID) {
-              fs[sent[(size_t) i * spe + opp(d, 0)[gdir]]] = 1;
+              u32 const side = sent[(size_t) i * spe + opp(d, 0)[gdir]];
+              fs[side] = 1;
+              if (ft) ft[side] = sent[(size_t) i * spe + opp(d, 0)[dir]];
             } else if (!(fe && fe[i])) {
               w = 1;
               gvv_ = target;
@@ -1599,6 +1855,13 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       check_launch("subset_tags");
     }
   }
+  // the maps this pass read and the next collapse pass reads again,
+  // carried to the new mesh instead of derived again: each entity's sides
+  // (down(d, d-1)) here, the vertices' upward adjacencies below.  The tets'
+  // edges are not carried: only a swap pass reads them, and carrying them
+  // through every collapse pass cost more than one reflection before the
+  // swap (C3: 2.87 against 2.82 s per 8 cycles)
+  Buf<u32> side_maps[4][4];
   for (u32 dd = 0; dd < dim; ++dd) {
     u32 d = dim - dd;
     u32 n = count(m, d);
@@ -1608,6 +1871,20 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     Buf<u32> cdo, cio;
     CoarsenArgs a{};
+    if (d >= 2 && m.down[d][d - 1]) {
+      u32 const l = d - 1, ns = simplex_sides(d, l);
+      side_maps[d][l] = Buf<u32>((size_t) (nsame + ngen) * ns);
+      CoarsenSides& cs = a.sides;
+      cs.out = side_maps[d][l].get();
+      cs.old = m.down[d][l].get();
+      cs.soff = plan[l].esoff.get();
+      cs.egoff = plan[l].egoff.get();
+      cs.fused_to = fused_to[l].get();
+      cs.nsame = got[2 + 2 * (l - 1) + 1];
+      cs.l = l;
+      cs.nold = m.counts[l];
+      cs.err = dev_error_word();
+    }
     a.d = d;
     a.n = n;
     a.nsame = nsame;
@@ -1628,8 +1905,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
         a.out_ci = cio.get();
       }
     }
-    hipLaunchKernelGGL(coarsen_outputs, dim3(nblocks(n)), dim3(BLOCK), 0, stream(), a);
-    check_launch("coarsen_outputs");
+    launch_timed("coarsen_outputs", (double) n * vpe * 8, coarsen_outputs, dim3(nblocks(n)), dim3(BLOCK), a);
     set_ents(mo, d, nsame + ngen, std::move(vout));
     if (has_class) {
       add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
@@ -1637,6 +1913,18 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     }
     if (d == dim) coarsen_transfer(m, mo, goff.get(), total, pl.egoff.get(), pl.esoff.get(), nsame, ngen);
   }
+  for (u32 h = 2; h <= dim; ++h)
+    for (u32 l = 1; l < h; ++l)
+      if (side_maps[h][l]) mo.down[h][l] = std::move(side_maps[h][l]);
+  // the vertices' upward adjacencies this pass read (to edges, to
+  // elements), carried too (vertices survive or are removed, none is new)
+  auto carry_up = [&](u32 h) {
+    if (!m.up[0][h]) return;
+    mo.up[0][h] = up_patch(h_down_deg[h][0], *m.up[0][h], m.counts[0], soff.get(), nv_out, 0, plan[h].esoff.get(),
+                           got[2 + 2 * (h - 1) + 1], got[2 + 2 * (h - 1)], mo.down[h][0].get());
+  };
+  carry_up(1);
+  carry_up(dim);
   say("collapsed", total, 1);
   overwrite(m, std::move(mo));
   return 1;
@@ -2288,107 +2576,6 @@ __global__ __launch_bounds__(BLOCK) void swap_keep(SwapKeepArgs a) {
   }
 }
 
-// The swapped mesh's edge-to-tet adjacency, built from the old one
-// (up_from_down.cpp:9-47 gives the same lists: per edge, its uses in
-// increasing use index tet * 6 + local edge).  Surviving tets keep their
-// relative order and their vertex order, and every product tet comes after
-// them, so a surviving edge's list is its old list without the removed
-// tets, renumbered, followed by its uses in product tets; a product edge
-// has product uses only.  Only the product uses (a few per swapped edge)
-// take atomics and a sort.
-struct SwapUpArgs {
-  u32 ne_old, nsame1, ngen1, nsame3, ngen3;
-  u32 const* uo;     // old up(1,3): offsets, tets, local edges
-  u32 const* ua;
-  u32 const* ud;
-  u32 const* soff1;  // old edge / tet -> index among the survivors
-  u32 const* soff3;
-  u32 const* d31;    // the new tets' edges
-  u32* cnt;          // per new edge: its uses (surviving, then + product)
-  u32* cnt_s;        // per new edge: its uses in surviving tets
-  u32* slot;         // per product use: its place in its edge's list
-  u32 const* off;    // the new offsets
-  u32* key;          // product use indices at their unsorted places
-  u32* adj;
-  u32* dirs;
-};
-
-// threads [0, ne_old): a surviving edge counts its surviving tets;
-// [ne_old, ne_old + ngen1): a product edge starts at zero
-__global__ __launch_bounds__(BLOCK) void swap_up_count_k(SwapUpArgs a) {
-  u32 t = xcd_block() * BLOCK + threadIdx.x;
-  if (t < a.ne_old) {
-    u32 n = a.soff1[t];
-    if (n == a.soff1[t + 1]) return;
-    u32 c = 0;
-    for (u32 k = a.uo[t]; k < a.uo[t + 1]; ++k) {
-      u32 h = a.ua[k];
-      c += a.soff3[h] != a.soff3[h + 1] ? 1u : 0u;
-    }
-    a.cnt[n] = c;
-    a.cnt_s[n] = c;
-  } else if (t - a.ne_old < a.ngen1) {
-    u32 n = a.nsame1 + (t - a.ne_old);
-    a.cnt[n] = 0;
-    a.cnt_s[n] = 0;
-  }
-}
-
-// a product use's edge, or INVALID where the tet's edges were not found
-// (reported by swap_cavity_k)
-__device__ inline u32 swap_up_edge(SwapUpArgs const& a, u32 u) {
-  u32 e = a.d31[(size_t) a.nsame3 * 6 + u];
-  return e < a.nsame1 + a.ngen1 ? e : INVALID;
-}
-
-// each product use takes a place after its edge's surviving uses
-__global__ __launch_bounds__(BLOCK) void swap_up_slot_k(SwapUpArgs a) {
-  u32 u = xcd_block() * BLOCK + threadIdx.x;
-  if (u >= a.ngen3 * 6) return;
-  u32 e = swap_up_edge(a, u);
-  if (e != INVALID) a.slot[u] = atomicAdd(&a.cnt[e], 1u);
-}
-
-// after the scan of the counts: threads [0, ne_old) write the surviving
-// uses of a surviving edge in order; the next ngen3 * 6 put each product
-// use's index at its place
-__global__ __launch_bounds__(BLOCK) void swap_up_fill_k(SwapUpArgs a) {
-  u32 t = xcd_block() * BLOCK + threadIdx.x;
-  if (t < a.ne_old) {
-    u32 n = a.soff1[t];
-    if (n == a.soff1[t + 1]) return;
-    u32 p = a.off[n];
-    for (u32 k = a.uo[t]; k < a.uo[t + 1]; ++k) {
-      u32 h = a.ua[k];
-      u32 s = a.soff3[h];
-      if (s == a.soff3[h + 1]) continue;
-      a.adj[p] = s;
-      a.dirs[p] = a.ud[k];
-      ++p;
-    }
-  } else if (t - a.ne_old < a.ngen3 * 6) {
-    u32 u = t - a.ne_old;
-    u32 e = swap_up_edge(a, u);
-    if (e != INVALID) a.key[a.off[e] + a.slot[u]] = a.nsame3 * 6 + u;
-  }
-}
-
-// each product use's final place: its rank by use index among its edge's
-// product uses
-__global__ __launch_bounds__(BLOCK) void swap_up_sort_k(SwapUpArgs a) {
-  u32 u = xcd_block() * BLOCK + threadIdx.x;
-  if (u >= a.ngen3 * 6) return;
-  u32 e = swap_up_edge(a, u);
-  if (e == INVALID) return;
-  u32 const b = a.off[e] + a.cnt_s[e], end = a.off[e + 1];
-  u32 const me = a.nsame3 * 6 + u;
-  u32 r = 0;
-  for (u32 k = b; k < end; ++k) r += a.key[k] < me ? 1u : 0u;
-  u32 const h = me / 6;
-  a.adj[b + r] = h;
-  a.dirs[b + r] = me - 6 * h;
-}
-
 // a vertex pair as one key (smaller vertex in the high word) and its home
 // slot in a table of mask + 1 slots
 __host__ __device__ inline unsigned long long swap_pair_key(u32 a, u32 b) {
@@ -2654,48 +2841,9 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
     if (d == m.dim) swap_transfer(m, mo, goff.get(), soff.get(), nsame, ngen);
   }
   if (d31) {
-    u32 const nsame1 = tots[2], ngen1 = tots[1], nsame3 = tots[6], ngen3 = tots[5];
-    u32 const ne_new = nsame1 + ngen1, nuse = (nsame3 + ngen3) * 6;
-    auto up = std::make_unique<Up>();
-    up->offsets = Buf<u32>(ne_new + 1);
-    up->adj = Buf<u32>(nuse);
-    up->dirs = Buf<u32>(nuse);
-    Buf<u32> cnt(ne_new), cnt_s(ne_new), slot((size_t) ngen3 * 6), key(nuse);
-    SwapUpArgs a{};
-    a.ne_old = ne;
-    a.nsame1 = nsame1;
-    a.ngen1 = ngen1;
-    a.nsame3 = nsame3;
-    a.ngen3 = ngen3;
-    a.uo = u.offsets.get();
-    a.ua = u.adj.get();
-    a.ud = u.dirs.get();
-    a.soff1 = soffs[1].get();
-    a.soff3 = soffs[3].get();
-    a.d31 = d31.get();
-    a.cnt = cnt.get();
-    a.cnt_s = cnt_s.get();
-    a.slot = slot.get();
-    a.off = up->offsets.get();
-    a.key = key.get();
-    a.adj = up->adj.get();
-    a.dirs = up->dirs.get();
-    hipLaunchKernelGGL(swap_up_count_k, dim3(nblocks((size_t) ne + ngen1)), dim3(BLOCK), 0, stream(), a);
-    check_launch("swap_up_count");
-    if (ngen3) {
-      hipLaunchKernelGGL(swap_up_slot_k, dim3(nblocks((size_t) ngen3 * 6)), dim3(BLOCK), 0, stream(), a);
-      check_launch("swap_up_slot");
-    }
-    exscan(cnt.get(), ne_new, up->offsets.get());
-    hipLaunchKernelGGL(swap_up_fill_k, dim3(nblocks((size_t) ne + (size_t) ngen3 * 6)), dim3(BLOCK), 0, stream(),
-                       a);
-    check_launch("swap_up_fill");
-    if (ngen3) {
-      hipLaunchKernelGGL(swap_up_sort_k, dim3(nblocks((size_t) ngen3 * 6)), dim3(BLOCK), 0, stream(), a);
-      check_launch("swap_up_sort");
-    }
+    // the edge-to-tet adjacency from the old one (the rings above read it)
+    mo.up[1][3] = up_patch(6, u, ne, soffs[1].get(), tots[2], tots[1], soffs[3].get(), tots[6], tots[5], d31.get());
     mo.down[3][1] = std::move(d31);
-    mo.up[1][3] = std::move(up);
   }
   say("swapped", total, 1);
   overwrite(m, std::move(mo));

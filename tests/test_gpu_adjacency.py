@@ -69,18 +69,24 @@ def test_full_mesh_stars_and_dual(product, oracle):
 
 
 def _maps_match_fresh(m, what):
-    """The mesh's tet-to-edge map and edge-to-tet adjacency equal the ones a
-    copy with only the entities' vertices derives."""
+    """The mesh's side maps and upward adjacencies equal the ones a copy
+    with only the entities' vertices derives (asking for a map the mesh
+    lacks derives it, so later passes carry it too)."""
     fresh = m.clone()
-    assert np.array_equal(m.down(3, 1), fresh.down(3, 1)), what + " down(3,1)"
-    for a, b, part in zip(m.up(1, 3), fresh.up(1, 3), ("offsets", "adjacency", "directions")):
-        assert np.array_equal(a, b), what + " up(1,3) " + part
+    downs = ((3, 1), (3, 2), (2, 1)) if m.dim == 3 else ((2, 1),)
+    ups = ((0, 1), (0, 3), (1, 3)) if m.dim == 3 else ((0, 1), (0, 2))
+    for h, l in downs:
+        assert np.array_equal(m.down(h, l), fresh.down(h, l)), "%s down(%d,%d)" % (what, h, l)
+    for l, h in ups:
+        for a, b, part in zip(m.up(l, h), fresh.up(l, h), ("offsets", "adjacency", "directions")):
+            assert np.array_equal(a, b), "%s up(%d,%d) %s" % (what, l, h, part)
 
 
 def test_swapped_mesh_maps_match_derivation(product):
-    """A swap pass hands the next mesh its tets' edges built from the old
-    mesh's maps instead of reflecting them; on the warped 4^3 box, after
-    every swap pass that changes the mesh, they equal the derived ones."""
+    """A swap pass hands the next mesh its tets' edges and the edges' tets
+    built from the old mesh's maps instead of deriving them again; on the
+    warped 4^3 box, after every swap pass that changes the mesh, they equal
+    the derived ones."""
     swaps = 0
     for label, m in P.CASES["warp3d_rot_n4"](product):
         if not label.startswith("warp"):
@@ -91,3 +97,25 @@ def test_swapped_mesh_maps_match_derivation(product):
             swaps += 1
             _maps_match_fresh(m, "%s, swap %d" % (label, swaps))
     assert swaps >= 3
+
+
+@pytest.mark.parametrize("case", ["warp3d_rot_n4", "warp2d_sine_n16"])
+def test_coarsened_mesh_maps_match_derivation(product, case):
+    """A collapse pass carries the side maps and upward adjacencies it read
+    to the next mesh (survivors renumbered, products from their old entity,
+    fused sides to the side they merge into); after every sliver collapse
+    pass that changes the mesh, and after every mesh_adapt of the case,
+    they equal the derived ones."""
+    passes = 0
+    for label, m in P.CASES[case](product):
+        if label.startswith("adapt"):
+            _maps_match_fresh(m, label)
+            continue
+        if not label.startswith("warp"):
+            continue
+        for _ in range(3):
+            if not m.coarsen_slivers(0.3, 4):
+                break
+            passes += 1
+            _maps_match_fresh(m, "%s, collapse %d" % (label, passes))
+    assert passes >= 1
