@@ -29,7 +29,7 @@ static double now_s() {
 }
 
 // scratch word slots (zeroed by a fill kernel before each use)
-enum { W_ANY = 0, W_FLAGS = 8, W_STATS = 40 };
+enum { W_ANY = 0, W_CACHED = 1, W_FLAGS = 8, W_STATS = 40 };
 // FP32-mixed statistics: two slotted u64 counters (decisions settled in
 // FP32, FP64 re-evaluations) in a buffer of their own, float bits of the
 // max |q32 - q64| at scratch word 40
@@ -1386,6 +1386,7 @@ struct UpPatch {
   u32 const* dn;      // the new down(h,l)
   u32* cnt;           // per new low: its uses (surviving, then + product)
   u32* cnt_s;         // per new low: its uses in surviving highs
+  u32* full;          // per new low: 1 if it kept every old use (null: not kept)
   u32* slot;          // per product use: its place in its low's list
   u32 const* off;     // the new offsets
   u32* key;           // product use indices at their unsorted places
@@ -1413,11 +1414,13 @@ __global__ __launch_bounds__(BLOCK) void up_patch_count_k(UpPatch a) {
     if (lane == 0) {
       a.cnt[n] = c;
       a.cnt_s[n] = c;
+      if (a.full) a.full[n] = c == a.uo[t + 1] - a.uo[t] ? 1u : 0u;
     }
   } else if (t - a.nl_old < a.ngen_l && lane == 0) {
     u32 n = a.nsame_l + (t - a.nl_old);
     a.cnt[n] = 0;
     a.cnt_s[n] = 0;
+    if (a.full) a.full[n] = 0;
   }
 }
 
@@ -1494,17 +1497,26 @@ __global__ __launch_bounds__(BLOCK) void up_patch_sort_k(UpPatch a) {
   a.dirs[b + r] = me - a.deg * h;
 }
 
+// per new low, kept for the caller: all its uses (cnt), those in surviving
+// highs (cnt_s), and whether those are all its old ones (full); a low with
+// full set and cnt == cnt_s has exactly its old highs
+struct UpPatchCounts {
+  Buf<u32> cnt, cnt_s, full;
+};
+
 // the new mesh's up(l,h) from the old mesh's: nl_old old lows, survivors
 // of both dimensions given by their offsets over the old entities, dn the
 // new down(h,l) (its product rows are read)
 static std::unique_ptr<Up> up_patch(u32 deg, Up const& old, u32 nl_old, u32 const* soff_l, u32 nsame_l,
-                                    u32 ngen_l, u32 const* soff_h, u32 nsame_h, u32 ngen_h, u32 const* dn) {
+                                    u32 ngen_l, u32 const* soff_h, u32 nsame_h, u32 ngen_h, u32 const* dn,
+                                    UpPatchCounts* keep = nullptr) {
   u32 const nl = nsame_l + ngen_l, nuse = (nsame_h + ngen_h) * deg, nprod = ngen_h * deg;
   auto up = std::make_unique<Up>();
   up->offsets = Buf<u32>(nl + 1);
   up->adj = Buf<u32>(nuse);
   up->dirs = Buf<u32>(nuse);
-  Buf<u32> cnt(nl), cnt_s(nl), slot(nprod), key(nuse);
+  Buf<u32> cnt(nl), cnt_s(nl), slot(nprod), key(nuse), full;
+  if (keep) full = Buf<u32>(nl);
   UpPatch a{};
   a.deg = deg;
   a.nuse = nuse;
@@ -1521,6 +1533,7 @@ static std::unique_ptr<Up> up_patch(u32 deg, Up const& old, u32 nl_old, u32 cons
   a.dn = dn;
   a.cnt = cnt.get();
   a.cnt_s = cnt_s.get();
+  a.full = full.get();
   a.slot = slot.get();
   a.off = up->offsets.get();
   a.key = key.get();
@@ -1549,6 +1562,11 @@ static std::unique_ptr<Up> up_patch(u32 deg, Up const& old, u32 nl_old, u32 cons
   if (nprod) {
     hipLaunchKernelGGL(up_patch_sort_k, dim3(nblocks(nprod)), dim3(BLOCK), 0, stream(), a);
     check_launch("up_patch_sort");
+  }
+  if (keep) {
+    keep->cnt = std::move(cnt);
+    keep->cnt_s = std::move(cnt_s);
+    keep->full = std::move(full);
   }
   return up;
 }
@@ -2589,26 +2607,71 @@ __host__ __device__ inline u32 swap_pair_slot(unsigned long long key, u32 mask) 
 }
 
 // swap_common + swap_interior + swap_ents (swap.cpp:27-136)
-static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr) {
+static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr, SwapCache* cache = nullptr) {
   check_index_space(m);
   u32 ne = count(m, 1);
   unmark_boundary(m, 1, cand.get());
-  // the candidates, compacted (their count is the reference's early
-  // return, swap.cpp:103-106, in one read)
-  Buf<u32> coff(ne + 1);
-  u32 ncand;
-  {
-    u32 const* cp = cand.get();
-    ncand = exscan_fn_total(ne, OHX_LAMBDA(u32 i) { return cp[i] ? 1u : 0u; }, coff.get());
+  Buf<double> q(ne);
+  Buf<u32> rsz = filled(ne, 0);
+  Buf<u32> codes(ne);
+  // candidates with a cached choice (their edge's tets unchanged since an
+  // earlier swap pass of this shape loop evaluated it) take it: a rejected
+  // one leaves the candidates, an accepted one keeps its value, ring size
+  // and triangulation; they are counted on the device
+  if (cache && cache->ne != ne) cache->clear();
+  SwapCache* const cc = cache && cache->ne ? cache : nullptr;
+  u32 const* inf = cc ? cc->info.get() : nullptr;
+  if (cc) {
+    fill(scratch_words() + W_CACHED, 1, 0);
+    u32* cp = cand.get();
+    double const* cq = cc->q.get();
+    double* qo = q.get();
+    u32* rs = rsz.get();
+    u32* cd = codes.get();
+    u32* nacc = scratch_words() + W_CACHED;
+    parallel_for(ne, OHX_LAMBDA(u32 i) {
+      bool hit = false;
+      u32 const f = inf[i];
+      if (cp[i] && (f >> 31)) {
+        if ((f >> 30) & 1u) {
+          qo[i] = cq[i];
+          rs[i] = (f >> 16) & 0xffu;
+          cd[i] = f & 0xffffu;
+          hit = true;
+        } else {
+          cp[i] = 0;
+        }
+      }
+      unsigned long long const b = __ballot(hit), act = __ballot(1);
+      if (b && (u32) __lane_id() == (u32) (__ffsll((long long) act) - 1)) atomicAdd(nacc, (u32) __popcll(b));
+    });
   }
-  if (!ncand) return 0;
-  Buf<u32> clist(ncand);
+  // the candidates to evaluate, compacted (with the cached ones accepted,
+  // their count is the reference's early return, swap.cpp:103-106, in one
+  // read)
+  Buf<u32> coff(ne + 1);
+  u32 ncand, ncached = 0;
   {
     u32 const* cp = cand.get();
+    auto f = OHX_LAMBDA(u32 i) { return cp[i] && !(inf && (inf[i] >> 31)) ? 1u : 0u; };
+    if (cc) {
+      exscan_fn(ne, f, coff.get());
+      u32 const* ptrs[2] = {coff.get() + ne, scratch_words() + W_CACHED};
+      u32 got[2];
+      read_many(ptrs, 2, got);
+      ncand = got[0];
+      ncached = got[1];
+    } else {
+      ncand = exscan_fn_total(ne, f, coff.get());
+    }
+  }
+  if (!ncand && !ncached) return 0;
+  Buf<u32> clist(ncand);
+  if (ncand) {
     u32 const* co = coff.get();
     u32* cl = clist.get();
     parallel_for(ne, OHX_LAMBDA(u32 i) {
-      if (cp[i]) cl[co[i]] = i;
+      if (co[i] != co[i + 1]) cl[co[i]] = i;
     });
   }
   coff.release();
@@ -2619,11 +2682,8 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   Buf<double> eq;
   if (!eq_given) eq = mesh_qualities(m);
   double const* eqp = eq_given ? eq_given : eq.get();
-  Buf<double> q(ne);
-  Buf<u32> rsz = filled(ne, 0);
-  Buf<u32> codes(ne);
   u32 tag = next_any_tag();
-  {
+  if (ncand) {
     // compulsory bytes: the edges' candidate flags, vertices and outputs,
     // the tets around them (uses + directions), their qualities and
     // vertices, and the vertex coordinates (and metrics)
@@ -2662,6 +2722,29 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       run(0, nullptr, nullptr);
   }
   eq.release();
+  // the evaluated candidates' choices go into the cache (this mesh's
+  // numbering; the swap below carries the cache to the new mesh)
+  if (cache) {
+    if (!cc) {
+      cache->ne = ne;
+      cache->q = Buf<double>(ne);
+      cache->info = filled(ne, 0);
+    }
+    if (ncand) {
+      u32 const* cl = clist.get();
+      u32 const* cp = cand.get();
+      double const* qi = q.get();
+      u32 const* rs = rsz.get();
+      u32 const* cd = codes.get();
+      double* cq = cache->q.get();
+      u32* ci = cache->info.get();
+      parallel_for(ncand, OHX_LAMBDA(u32 k) {
+        u32 const i = cl[k];
+        ci[i] = (1u << 31) | (cp[i] ? (1u << 30) | (rs[i] << 16) | cd[i] : 0u);
+        cq[i] = qi[i];
+      });
+    }
+  }
   // the early return after the qualities (swap.cpp:119-124) is the totals
   // read below: the quality kernel clears every rejected candidate, so
   // with none accepted the set is empty and no element is generated
@@ -2682,7 +2765,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   // mesh to build down(2,1)
   bool const faces_by_pairs = m.dim == 3 && m.full && !m.down[2][1];
   u32 pcap = 64;
-  while (pcap < 4 * (size_t) ncand) pcap *= 2;
+  while (pcap < 4 * ((size_t) ncand + ncached)) pcap *= 2;
   Buf<unsigned long long> pairs(faces_by_pairs ? pcap : 0);
   auto offsets_and_totals = [&] {
     u32 const* is = indset.get();
@@ -2694,10 +2777,8 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       unsigned long long* tb = pairs.get();
       u32 const mask = pcap - 1;
       parallel_for(pcap, OHX_LAMBDA(u32 i) { tb[i] = ~0ull; });
-      u32 const* cl = clist.get();
       u32* err = dev_error_word();
-      parallel_for(ncand, OHX_LAMBDA(u32 c) {
-        u32 e = cl[c];
+      parallel_for(ne, OHX_LAMBDA(u32 e) {
         if (!is[e]) return;
         unsigned long long key = swap_pair_key(ev[2 * (size_t) e], ev[2 * (size_t) e + 1]);
         u32 h = swap_pair_slot(key, mask);
@@ -2842,15 +2923,43 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   }
   if (d31) {
     // the edge-to-tet adjacency from the old one (the rings above read it)
-    mo.up[1][3] = up_patch(6, u, ne, soffs[1].get(), tots[2], tots[1], soffs[3].get(), tots[6], tots[5], d31.get());
+    UpPatchCounts pc;
+    mo.up[1][3] = up_patch(6, u, ne, soffs[1].get(), tots[2], tots[1], soffs[3].get(), tots[6], tots[5], d31.get(),
+                           cache ? &pc : nullptr);
     mo.down[3][1] = std::move(d31);
+    if (cache) {
+      // the cache in the new numbering: a surviving edge that kept all its
+      // tets and gained none keeps its entry
+      u32 const ne_new = tots[2] + tots[1];
+      Buf<double> nq(ne_new);
+      Buf<u32> ninfo = filled(ne_new, 0);
+      u32 const* so = soffs[1].get();
+      u32 const* full = pc.full.get();
+      u32 const* cnt = pc.cnt.get();
+      u32 const* cnt_s = pc.cnt_s.get();
+      u32 const* ci = cache->info.get();
+      double const* cq = cache->q.get();
+      u32* ni = ninfo.get();
+      double* nqp = nq.get();
+      parallel_for(ne, OHX_LAMBDA(u32 e) {
+        u32 const n = so[e];
+        if (n == so[e + 1] || !full[n] || cnt[n] != cnt_s[n]) return;
+        ni[n] = ci[e];
+        nqp[n] = cq[e];
+      });
+      cache->ne = ne_new;
+      cache->q = std::move(nq);
+      cache->info = std::move(ninfo);
+    }
+  } else if (cache) {
+    cache->clear();
   }
   say("swapped", total, 1);
   overwrite(m, std::move(mo));
   return 1;
 }
 
-u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq, double const* gate) {
+u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq, double const* gate, SwapCache* cache) {
   OHX_REQUIRE(m.dim == 3, "swap_slivers needs a 3D mesh");
   // the pass reads up(1,3) (the edge marks below, the rings, the quality
   // kernel); derived first, it also gives the sliver layers' element dual
@@ -2860,7 +2969,7 @@ u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq, doubl
   Buf<u32> c = mark_down(m, m.dim, 1, sl.get());
   sl.release();
   if (g_batch) batch_mask(m, 1, c.get());
-  return swap_common(m, c, eq);
+  return swap_common(m, c, eq, cache);
 }
 
 // ---------------------------------------------------------------- adapt
@@ -2869,6 +2978,7 @@ namespace {
 struct Ctx {
   u32 ops = 0, max_ops = 0;
   AdaptStats st;
+  SwapCache swap_cache;  // the shape loop's swap choices (cleared by other passes)
   // passes that changed the mesh: events around them on the stream, read
   // once at the end (a pass's time is the stream's, from the end of the work
   // before it to the end of its own, host waits included)
@@ -2929,10 +3039,11 @@ u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n, double const* e
   hipEvent_t e0 = c.event(), e1 = c.event();
   OHX_CHECK(hipEventRecord(e0, stream()));
   u32 ok = 0;
+  if (k != SWAP) c.swap_cache.clear();
   switch (k) {
     case REFINE: ok = refine_by_size(m, a); break;
     case COARSEN: ok = coarsen_by_size(m, a, b); break;
-    case SWAP: ok = swap_slivers(m, a, n, eq, gate); break;
+    case SWAP: ok = swap_slivers(m, a, n, eq, gate, &c.swap_cache); break;
     case COARSEN_SLIVERS: ok = coarsen_slivers(m, a, n, eq); break;
   }
   OHX_CHECK(hipEventRecord(e1, stream()));

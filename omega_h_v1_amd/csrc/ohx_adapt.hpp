@@ -14,7 +14,24 @@ void uniformly_refine(Mesh& m);
 u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor);
 // eq: the mesh's element qualities when the caller has them (else computed)
 u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers, double const* eq = nullptr);
-u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr, double const* gate = nullptr);
+// The swap choices of the edges evaluated by earlier swap passes of one
+// shape loop, per edge of the current mesh: an edge whose tets are the
+// same keeps its ring, so its swap quality, ring size, triangulation and
+// acceptance stay what they were (info: bit 31 valid, bit 30 accepted,
+// bits 16-23 ring size, bits 0-15 triangulation).  Any other pass that
+// changes the mesh clears it.
+struct SwapCache {
+  u32 ne = 0;  // edges of the mesh it describes (0: empty)
+  Buf<double> q;
+  Buf<u32> info;
+  void clear() {
+    ne = 0;
+    q.release();
+    info.release();
+  }
+};
+u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr, double const* gate = nullptr,
+                 SwapCache* cache = nullptr);
 
 struct AdaptStats {
   double npass = 0;   // passes that changed the mesh
