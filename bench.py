@@ -96,13 +96,18 @@ ADAPT_ARGS = (1.0 / 3.0, 0.3, 4, 50)
 # profiled run's launches of the kernel; per workload and kernel (the
 # kernels that can be a workload's dominant one)
 TRAFFIC = {
-    ("c2", "scan"): 2994598,
-    ("c2", "indset_round"): 2158592,
-    ("c2", "reflect_down"): 55894426,
-    ("c3", "reflect_down"): 871959533,
-    ("c3", "scan"): 30494655,
-    ("c5", "lows_hash_insert"): 4283248041,
-    ("c5", "reflect_down"): 17986620014,
+    ("c2", "scan"): 2764823,
+    ("c2", "indset_round"): 3225804,
+    ("c2", "reflect_down"): 53090816,
+    ("c3", "coarsen_quality_mixed"): 610625883,
+    ("c3", "scan"): 12556348,
+    ("c3", "reflect_down"): 715727174,
+    ("c5", "coarsen_outputs"): 4469509427,
+    ("c5", "up_patch_fill"): 3443524823,
+    ("c5", "scan"): 389051928,
+    ("c5", "coarsen_quality_mixed"): 13300132659,
+    ("c5", "lows_hash_insert"): 2519497975,
+    ("c5", "reflect_down"): 11939056067,
 }
 
 

@@ -1923,7 +1923,13 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
         a.out_ci = cio.get();
       }
     }
-    launch_timed("coarsen_outputs", (double) n * vpe * 8, coarsen_outputs, dim3(nblocks(n)), dim3(BLOCK), a);
+    // compulsory bytes per old entity: its vertices in, their new numbers
+    // gathered and written, its survivor and product offsets, target and
+    // direction, class tags in and out, and the carried sides (old ids in,
+    // new ids gathered and written)
+    u32 const ns = a.sides.out ? simplex_sides(d, d - 1) : 0u;
+    launch_timed("coarsen_outputs", (double) n * (vpe * 12 + 32 + ns * 12), coarsen_outputs, dim3(nblocks(n)),
+                 dim3(BLOCK), a);
     set_ents(mo, d, nsame + ngen, std::move(vout));
     if (has_class) {
       add_tag_u32(mo, d, "class_dim", 1, std::move(cdo));
