@@ -2121,18 +2121,16 @@ void swap_int_edges_host(u32 rs, u32 code, u32* out) {
 }
 
 // find_edge_ring (edge_ring.cpp:12-74): ring vertices ordered by chaining
-// each tet's opposite edge (a -> b), insertion style, <= 7 entries; with
-// ring_u, also the use (index into up(1,3)) of the tet at each ring slot k,
-// the tet holding ring vertices k and k + 1
+// each tet's opposite edge (a -> b), insertion style, <= 7 entries
 __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 const* ud, u32 const* ev,
-                                u32 const* vt, u32* edge_v, u32* ring_v, u32* ring_u = nullptr) {
+                                u32 const* vt, u32* edge_v, u32* ring_v) {
   u32 f = uo[edge], e = uo[edge + 1];
   u32 rs = e - f;
   if (rs > MAX_EDGE_SWAP || rs < 3) return rs;
   u32 const* oe = opp(3, 1);
   edge_v[0] = ev[2 * (size_t) edge];
   edge_v[1] = ev[2 * (size_t) edge + 1];
-  u32 ra[MAX_EDGE_SWAP], rb[MAX_EDGE_SWAP], ru[MAX_EDGE_SWAP];
+  u32 ra[MAX_EDGE_SWAP], rb[MAX_EDGE_SWAP];
   for (u32 i = f; i < e; ++i) {
     u32 const* tv = vt + 4 * (size_t) ua[i];
     u32 te = ud[i];
@@ -2141,24 +2139,19 @@ __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 cons
     u32 const* tvo = canon(3, 1, 0, oe[te]);
     ra[i - f] = tv[tvo[al]];
     rb[i - f] = tv[tvo[al ^ 1u]];
-    ru[i - f] = i;
   }
   for (u32 i = 0; i + 1 < rs; ++i) {
     u32 j;
     for (j = i + 1; j < rs; ++j)
       if (ra[j] == rb[i]) break;
     if (j == rs) return 0;  // ring does not close: reported by the caller
-    u32 ta = ra[j], tb = rb[j], tu = ru[j];
+    u32 ta = ra[j], tb = rb[j];
     ra[j] = ra[i + 1];
     rb[j] = rb[i + 1];
-    ru[j] = ru[i + 1];
     ra[i + 1] = ta;
     rb[i + 1] = tb;
-    ru[i + 1] = tu;
   }
   for (u32 i = 0; i < rs; ++i) ring_v[i] = ra[i];
-  if (ring_u)
-    for (u32 i = 0; i < rs; ++i) ring_u[i] = ru[i];
   return rs;
 }
 
@@ -2451,11 +2444,11 @@ __device__ inline u32 tet_edge_between(u32 a, u32 b) {
 __global__ __launch_bounds__(BLOCK) void swap_cavity_k(
     u32 ne, u32 d, u32 const* __restrict__ indset, u32 const* __restrict__ goff, u32 const* __restrict__ uo,
     u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
-    u32 const* __restrict__ vt, u32 const* __restrict__ codes, u32* __restrict__ out, SwapEdgesOf E) {
+    u32 const* __restrict__ vt, u32 const* __restrict__ codes, u32* __restrict__ out) {
   u32 i = xcd_block() * BLOCK + threadIdx.x;
   if (i >= ne || !indset[i]) return;
-  u32 edge_v[2], ring_v[MAX_EDGE_SWAP], ring_u[MAX_EDGE_SWAP];
-  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v, ring_u);
+  u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
+  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
   if (rs < 3 || rs > MAX_EDGE_SWAP) return;  // cannot happen for an indset edge
   u32 const code = codes[i];
   u32 tpm = k_sw_sizes[rs];
@@ -2465,66 +2458,15 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_k(
   unsigned char const* ie = k_sw_int + k_sw_int_off[rs] + code * nint * 2;
   u32* o = out + (size_t) goff[i] * (d + 1);
   if (d == 3) {
-    // the cavity's surviving edges, new numbering: ring edge k joins ring
-    // vertices k and k + 1, spoke (k, j) ring vertex k and edge end j; the
-    // tet at ring slot k holds all of them
-    u32 ring_e[MAX_EDGE_SWAP], spoke_e[MAX_EDGE_SWAP][2];
-    if (E.out) {
-      for (u32 k = 0; k < rs; ++k) {
-        u32 const use = ring_u[k];
-        u32 const tet = ua[use];
-        u32 const* tv = vt + 4 * (size_t) tet;
-        u32 const* te = E.old + 6 * (size_t) tet;
-        ring_e[k] = E.soff1[te[opp(3, 1)[ud[use]]]];
-        u32 lk = 0, l0 = 0, l1 = 0;
-        for (u32 c = 0; c < 4; ++c) {
-          if (tv[c] == ring_v[k]) lk = c;
-          if (tv[c] == edge_v[0]) l0 = c;
-          if (tv[c] == edge_v[1]) l1 = c;
-        }
-        spoke_e[k][0] = E.soff1[te[tet_edge_between(lk, l0)]];
-        spoke_e[k][1] = E.soff1[te[tet_edge_between(lk, l1)]];
-      }
-    }
-    u32 const first = E.nsame3 + goff[i];
     for (u32 t = 0; t < tpm; ++t) {
-      u32 tv[4], lab[4];  // vertices; labels: ring slot, or 8 + edge end
-      for (u32 j = 0; j < 3; ++j) {
-        lab[j] = tris[mesh[t]][j];
-        tv[j] = ring_v[lab[j]];
-      }
+      u32 tv[4];
+      for (u32 j = 0; j < 3; ++j) tv[j] = ring_v[tris[mesh[t]][j]];
       for (u32 j = 0; j < 2; ++j) {
         tv[3] = edge_v[1 - j];
-        lab[3] = 8 + (1 - j);
         for (u32 k = 0; k < 4; ++k) *o++ = tv[k];
-        if (E.out) {
-          u32* eo = E.out + 6 * (size_t) (first + 2 * t + j);
-          for (u32 le = 0; le < 6; ++le) {
-            u32 const* c = canon(3, 1, 0, le);
-            u32 a = lab[c[0]], b = lab[c[1]];
-            u32 id = INVALID;
-            if (a >= 8 || b >= 8) {
-              if (a < 8) id = spoke_e[a][b - 8];
-              else if (b < 8) id = spoke_e[b][a - 8];
-            } else if (a + 1 == b || a == b + rs - 1) {
-              id = ring_e[a];
-            } else if (b + 1 == a || b == a + rs - 1) {
-              id = ring_e[b];
-            } else {
-              for (u32 k = 0; k < nint; ++k)
-                if ((ie[2 * k] == a && ie[2 * k + 1] == b) || (ie[2 * k] == b && ie[2 * k + 1] == a))
-                  id = E.nsame1 + E.goff1[i] + k;
-            }
-            if (id == INVALID) *E.err = ERR_REFLECT_NOT_FOUND;
-            eo[le] = id;
-          }
-        }
         u32 tmp = tv[0];
         tv[0] = tv[1];
         tv[1] = tmp;
-        tmp = lab[0];
-        lab[0] = lab[1];
-        lab[1] = tmp;
       }
     }
   } else if (d == 1) {
@@ -2542,6 +2484,96 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_k(
     o += nint * 6;
     for (u32 t = 0; t < tpm; ++t)
       for (u32 j = 0; j < 3; ++j) o[t * 3 + j] = ring_v[tris[mesh[t]][j]];
+  }
+}
+
+// The product tets' edges of each swapped edge, in the order swap_cavity_k
+// writes the tets (triangle t of the chosen triangulation with edge end 1,
+// then with its first two corners exchanged and edge end 0).  A product
+// tet's vertices are labelled by where they come from (ring slot, or edge
+// end); an edge between adjacent ring slots is a ring edge, one between a
+// slot and an edge end a spoke, both edges of the old tet at that ring slot
+// (the one holding ring vertices k and k + 1), and any other is one of the
+// new interior edges.  The cavity's ring edges and spokes are staged in
+// LDS per thread (indexed by slot).
+__global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
+    u32 ne, u32 const* __restrict__ indset, u32 const* __restrict__ goff3, u32 const* __restrict__ uo,
+    u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
+    u32 const* __restrict__ vt, u32 const* __restrict__ codes, SwapEdgesOf E) {
+  __shared__ u32 s_ring[MAX_EDGE_SWAP][BLOCK];
+  __shared__ u32 s_spoke[2][MAX_EDGE_SWAP][BLOCK];
+  u32 const tid = threadIdx.x;
+  u32 i = xcd_block() * BLOCK + tid;
+  if (i >= ne || !indset[i]) return;  // no barrier follows
+  u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
+  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
+  if (rs < 3 || rs > MAX_EDGE_SWAP) return;
+  u32 const f = uo[i], e = uo[i + 1];
+  for (u32 k = 0; k < rs; ++k) {
+    u32 const a = ring_v[k], b = ring_v[k + 1 < rs ? k + 1 : 0];
+    u32 tet = INVALID;
+    for (u32 q = f; q < e && tet == INVALID; ++q) {
+      u32 const* tv = vt + 4 * (size_t) ua[q];
+      bool ha = false, hb = false;
+      for (u32 c = 0; c < 4; ++c) {
+        ha = ha || tv[c] == a;
+        hb = hb || tv[c] == b;
+      }
+      if (ha && hb) tet = ua[q];
+    }
+    if (tet == INVALID) {
+      *E.err = ERR_REFLECT_NOT_FOUND;
+      return;
+    }
+    u32 const* tv = vt + 4 * (size_t) tet;
+    u32 const* te = E.old + 6 * (size_t) tet;
+    u32 la = 0, lb = 0, l0 = 0, l1 = 0;
+    for (u32 c = 0; c < 4; ++c) {
+      if (tv[c] == a) la = c;
+      if (tv[c] == b) lb = c;
+      if (tv[c] == edge_v[0]) l0 = c;
+      if (tv[c] == edge_v[1]) l1 = c;
+    }
+    s_ring[k][tid] = E.soff1[te[tet_edge_between(la, lb)]];
+    s_spoke[0][k][tid] = E.soff1[te[tet_edge_between(la, l0)]];
+    s_spoke[1][k][tid] = E.soff1[te[tet_edge_between(la, l1)]];
+  }
+  u32 const code = codes[i];
+  u32 const tpm = k_sw_sizes[rs];
+  unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs] + code * tpm;
+  unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
+  u32 const nint = k_sw_nint[rs];
+  unsigned char const* ie = k_sw_int + k_sw_int_off[rs] + code * nint * 2;
+  u32 const first = E.nsame3 + goff3[i];
+  for (u32 t = 0; t < tpm; ++t) {
+    u32 lab[4];  // ring slot, or 8 + edge end
+    for (u32 j = 0; j < 3; ++j) lab[j] = tris[mesh[t]][j];
+    for (u32 j = 0; j < 2; ++j) {
+      lab[3] = 8 + (1 - j);
+      u32* eo = E.out + 6 * (size_t) (first + 2 * t + j);
+      for (u32 le = 0; le < 6; ++le) {
+        u32 const* c = canon(3, 1, 0, le);
+        u32 a = lab[c[0]], b = lab[c[1]];
+        u32 id = INVALID;
+        if (a >= 8 || b >= 8) {
+          if (a < 8) id = s_spoke[b - 8][a][tid];
+          else if (b < 8) id = s_spoke[a - 8][b][tid];
+        } else if (a + 1 == b || a == b + rs - 1) {
+          id = s_ring[a][tid];
+        } else if (b + 1 == a || b == a + rs - 1) {
+          id = s_ring[b][tid];
+        } else {
+          for (u32 k = 0; k < nint; ++k)
+            if ((ie[2 * k] == a && ie[2 * k + 1] == b) || (ie[2 * k] == b && ie[2 * k + 1] == a))
+              id = E.nsame1 + E.goff1[i] + k;
+        }
+        if (id == INVALID) *E.err = ERR_REFLECT_NOT_FOUND;
+        eo[le] = id;
+      }
+      u32 tmp = lab[0];
+      lab[0] = lab[1];
+      lab[1] = tmp;
+    }
   }
 }
 
@@ -2892,8 +2924,12 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
     {
       double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
       launch_timed("swap_cavity", prof_bytes, swap_cavity_k, dim3(nblocks(ne)), dim3(BLOCK), ne, d, is, goff.get(),
-                   u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), vout.get() + (size_t) nsame * vpe,
-                   E);
+                   u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), vout.get() + (size_t) nsame * vpe);
+      if (E.out) {
+        hipLaunchKernelGGL(swap_product_edges_k, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, is, goff.get(),
+                           u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), E);
+        check_launch("swap_product_edges");
+      }
     }
     Buf<u32> cdo, cio;
     SwapKeepArgs a{};
