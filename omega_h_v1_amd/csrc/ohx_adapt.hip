@@ -2427,7 +2427,32 @@ struct SwapEdgesOf {
   u32 const* goff1;    // interior edges generated per swapped edge
   u32 nsame1, nsame3;  // surviving edges, surviving tets
   u32* err;
+  // the element dual carried too (dual null: not built): the old dual,
+  // the tets' survivor offsets, per old tet face the product now across it
+  // (written for the cavities' outer faces), and per product face whose
+  // outside tet is in another cavity the old (tet, face) to resolve
+  u32* dual;
+  u32 const* old_dual;
+  u32 const* soff3;
+  u32* face_prod;
+  u32* deferred;
 };
+
+constexpr u32 DUAL_PENDING = 0xfffffffeu;
+
+// local face of a tet (vertices tv) with the vertices a, b, c
+__device__ inline u32 tet_face_of(u32 const* tv, u32 a, u32 b, u32 c) {
+  for (u32 f = 0; f < 4; ++f) {
+    u32 const* fv = canon(3, 2, 0, f);
+    u32 hits = 0;
+    for (u32 q = 0; q < 3; ++q) {
+      u32 v = tv[fv[q]];
+      hits += (v == a || v == b || v == c) ? 1u : 0u;
+    }
+    if (hits == 3) return f;
+  }
+  return 0;
+}
 
 // local edge of a tet between local vertices a and b
 __device__ inline u32 tet_edge_between(u32 a, u32 b) {
@@ -2502,6 +2527,7 @@ __global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
     u32 const* __restrict__ vt, u32 const* __restrict__ codes, SwapEdgesOf E) {
   __shared__ u32 s_ring[MAX_EDGE_SWAP][BLOCK];
   __shared__ u32 s_spoke[2][MAX_EDGE_SWAP][BLOCK];
+  __shared__ u32 s_tet[MAX_EDGE_SWAP][BLOCK];
   u32 const tid = threadIdx.x;
   u32 i = xcd_block() * BLOCK + tid;
   if (i >= ne || !indset[i]) return;  // no barrier follows
@@ -2534,6 +2560,7 @@ __global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
       if (tv[c] == edge_v[0]) l0 = c;
       if (tv[c] == edge_v[1]) l1 = c;
     }
+    s_tet[k][tid] = tet;
     s_ring[k][tid] = E.soff1[te[tet_edge_between(la, lb)]];
     s_spoke[0][k][tid] = E.soff1[te[tet_edge_between(la, l0)]];
     s_spoke[1][k][tid] = E.soff1[te[tet_edge_between(la, l1)]];
@@ -2570,10 +2597,95 @@ __global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
         if (id == INVALID) *E.err = ERR_REFLECT_NOT_FOUND;
         eo[le] = id;
       }
+      if (E.dual) {
+        // the tet across each face: the twin (same triangle, other end)
+        // across the triangle; across a face on a ring edge, the tet
+        // outside the ring tet's face there; across a face on an interior
+        // edge, the product of the other triangle holding that edge
+        u32 const self = first + 2 * t + j;
+        u32* dd = E.dual + 4 * (size_t) self;
+        for (u32 f = 0; f < 4; ++f) {
+          u32 const* fv = canon(3, 2, 0, f);
+          u32 r[2], nr = 0;
+          bool has_end = false;
+          for (u32 q = 0; q < 3; ++q) {
+            u32 l = lab[fv[q]];
+            if (l >= 8)
+              has_end = true;
+            else if (nr < 2)
+              r[nr++] = l;
+          }
+          u32 nb = INVALID;
+          if (!has_end) {
+            nb = first + 2 * t + (1 - j);
+          } else {
+            u32 const a = r[0], b = r[1];
+            bool const ring = a + 1 == b || a == b + rs - 1 || b + 1 == a || b == a + rs - 1;
+            if (ring) {
+              u32 const k = (a + 1 == b || a == b + rs - 1) ? a : b;
+              u32 const tk = s_tet[k][tid];
+              u32 const x = 1 - j;  // this tet's edge end
+              u32 const lf = tet_face_of(vt + 4 * (size_t) tk, edge_v[x], ring_v[k], ring_v[k + 1 < rs ? k + 1 : 0]);
+              E.face_prod[4 * (size_t) tk + lf] = self;
+              u32 const o = E.old_dual[4 * (size_t) tk + lf];
+              if (o == INVALID) {
+                nb = INVALID;
+              } else if (E.soff3[o] != E.soff3[o + 1]) {
+                nb = E.soff3[o];
+              } else {
+                nb = DUAL_PENDING;
+                E.deferred[4 * (size_t) (self - E.nsame3) + f] = 4 * tk + lf;
+              }
+            } else {
+              for (u32 t2 = 0; t2 < tpm; ++t2) {
+                if (t2 == t) continue;
+                unsigned char const* tr = tris[mesh[t2]];
+                bool ha = tr[0] == a || tr[1] == a || tr[2] == a;
+                bool hb = tr[0] == b || tr[1] == b || tr[2] == b;
+                if (ha && hb) nb = first + 2 * t2 + j;
+              }
+              if (nb == INVALID) *E.err = ERR_REFLECT_NOT_FOUND;
+            }
+          }
+          dd[f] = nb;
+        }
+      }
       u32 tmp = lab[0];
       lab[0] = lab[1];
       lab[1] = tmp;
     }
+  }
+}
+
+// The new dual's remaining entries: threads [0, nt_old) give a surviving
+// tet its neighbours (a surviving one renumbered, a removed one replaced by
+// the product now across that face); the ngen3 * 4 after them resolve a
+// product face whose outside tet was removed by another cavity.
+__global__ __launch_bounds__(BLOCK) void swap_dual_k(u32 nt_old, u32 ngen3, SwapEdgesOf E) {
+  u32 const s = xcd_block() * BLOCK + threadIdx.x;
+  auto across = [&](u32 n, u32 from) -> u32 {  // the product across n's face shared with from
+    for (u32 g = 0; g < 4; ++g)
+      if (E.old_dual[4 * (size_t) n + g] == from) return E.face_prod[4 * (size_t) n + g];
+    *E.err = ERR_REFLECT_NOT_FOUND;
+    return INVALID;
+  };
+  if (s < nt_old) {
+    u32 const o = E.soff3[s];
+    if (o == E.soff3[s + 1]) return;
+    for (u32 f = 0; f < 4; ++f) {
+      u32 const n = E.old_dual[4 * (size_t) s + f];
+      u32 r = INVALID;
+      if (n != INVALID) r = E.soff3[n] != E.soff3[n + 1] ? E.soff3[n] : across(n, s);
+      E.dual[4 * (size_t) o + f] = r;
+    }
+  } else if (s - nt_old < 4 * ngen3) {
+    u32 const q = s - nt_old;
+    u32* d = E.dual + 4 * (size_t) E.nsame3 + q;
+    if (*d != DUAL_PENDING) return;
+    u32 const key = E.deferred[q];
+    u32 const tk = key / 4;
+    u32 const n = E.old_dual[key];
+    *d = across(n, tk);
   }
 }
 
@@ -2902,7 +3014,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   // the tets' edges of the new mesh, from the old map (the old one exists:
   // the rings above read up(1,3)); the next pass asks for them first
   bool const edges_of_tets = m.dim == 3 && dlo == 1;
-  Buf<u32> d31;
+  Buf<u32> d31, dual_new;
   for (u32 d = dlo; d <= m.dim; ++d) {
     u32 n = count(m, d);
     u32 vpe = d + 1;
@@ -2911,6 +3023,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
     u32 ngen = tots[1 + 2 * (d - dlo)], nsame = tots[2 + 2 * (d - dlo)];
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     SwapEdgesOf E{};
+    Buf<u32> face_prod, deferred;
     if (d == 3 && edges_of_tets) {
       d31 = Buf<u32>((size_t) (nsame + ngen) * 6);
       E.out = d31.get();
@@ -2920,6 +3033,17 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       E.nsame1 = tots[2];
       E.nsame3 = nsame;
       E.err = dev_error_word();
+      if (m.dual) {
+        // the dual this pass's sliver marks read, carried to the next mesh
+        dual_new = Buf<u32>((size_t) (nsame + ngen) * 4);
+        face_prod = Buf<u32>((size_t) n * 4);
+        deferred = Buf<u32>((size_t) ngen * 4);
+        E.dual = dual_new.get();
+        E.old_dual = m.dual.get();
+        E.soff3 = soff.get();
+        E.face_prod = face_prod.get();
+        E.deferred = deferred.get();
+      }
     }
     {
       double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
@@ -2929,6 +3053,11 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
         hipLaunchKernelGGL(swap_product_edges_k, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, is, goff.get(),
                            u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), E);
         check_launch("swap_product_edges");
+      }
+      if (E.dual) {
+        hipLaunchKernelGGL(swap_dual_k, dim3(nblocks((size_t) n + (size_t) ngen * 4)), dim3(BLOCK), 0, stream(), n,
+                           ngen, E);
+        check_launch("swap_dual");
       }
     }
     Buf<u32> cdo, cio;
@@ -2969,6 +3098,7 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
     mo.up[1][3] = up_patch(6, u, ne, soffs[1].get(), tots[2], tots[1], soffs[3].get(), tots[6], tots[5], d31.get(),
                            cache ? &pc : nullptr);
     mo.down[3][1] = std::move(d31);
+    if (dual_new) mo.dual = std::move(dual_new);
     if (cache) {
       // the cache in the new numbering: a surviving edge that kept all its
       // tets and gained none keeps its entry

@@ -80,13 +80,14 @@ def _maps_match_fresh(m, what):
     for l, h in ups:
         for a, b, part in zip(m.up(l, h), fresh.up(l, h), ("offsets", "adjacency", "directions")):
             assert np.array_equal(a, b), "%s up(%d,%d) %s" % (what, l, h, part)
+    assert np.array_equal(m.dual(), fresh.dual()), what + " dual"
 
 
 def test_swapped_mesh_maps_match_derivation(product):
-    """A swap pass hands the next mesh its tets' edges and the edges' tets
-    built from the old mesh's maps instead of deriving them again; on the
-    warped 4^3 box, after every swap pass that changes the mesh, they equal
-    the derived ones."""
+    """A swap pass hands the next mesh its tets' edges, the edges' tets and
+    the element dual built from the old mesh's maps instead of deriving them
+    again; on the warped 4^3 box, after every swap pass that changes the
+    mesh, they equal the derived ones."""
     swaps = 0
     for label, m in P.CASES["warp3d_rot_n4"](product):
         if not label.startswith("warp"):
