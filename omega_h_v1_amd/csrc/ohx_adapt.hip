@@ -3091,6 +3091,23 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
       if (has_id) add_tag_u32(mo, d, "class_id", 1, std::move(cio));
     }
     if (d == m.dim) swap_transfer(m, mo, goff.get(), soff.get(), nsame, ngen);
+    if (d == m.dim && cache && eq_given) {
+      // the new mesh's element qualities: a survivor's is its old one, a
+      // product's is evaluated (the same kernel mesh_qualities runs)
+      Buf<double> neq = element_qualities(d, ngen, mo.down[d][0].get() + (size_t) nsame * vpe, coords(m),
+                                          count(m, 0), metric_of(m));
+      Buf<double> all((size_t) nsame + ngen);
+      double* ap = all.get();
+      double const* np = neq.get();
+      u32 const* so = soff.get();
+      double const* oq = eq_given;
+      parallel_for(n, OHX_LAMBDA(u32 t) {
+        u32 const o = so[t];
+        if (o != so[t + 1]) ap[o] = oq[t];
+      });
+      if (ngen) d2d(ap + nsame, np, ngen);
+      cache->eq = std::move(all);
+    }
   }
   if (d31) {
     // the edge-to-tet adjacency from the old one (the rings above read it)
@@ -3247,7 +3264,7 @@ int adapt(Mesh& m, double size_floor, double good_qual, u32 nlayers, u32 max_ops
     // sliver marks and the comparisons of the swap and collapse passes all
     // read them (a pass that changes nothing leaves them valid for the next)
     u32 const nel = count(m, m.dim);
-    Buf<double> eq = mesh_qualities(m);
+    Buf<double> eq = c.swap_cache.eq ? std::move(c.swap_cache.eq) : mesh_qualities(m);
     if (m.dim == 3) {
       // the minimum (over elements 1..n-1, as the serial doubles_min)
       // reaches the host with the swap pass's first wait instead of a wait

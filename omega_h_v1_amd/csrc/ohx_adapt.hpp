@@ -24,10 +24,14 @@ struct SwapCache {
   u32 ne = 0;  // edges of the mesh it describes (0: empty)
   Buf<double> q;
   Buf<u32> info;
+  // the new mesh's element qualities after a swap pass that changed it
+  // (survivors' carried, products' evaluated), for the shape loop to take
+  Buf<double> eq;
   void clear() {
     ne = 0;
     q.release();
     info.release();
+    eq.release();
   }
 };
 u32 swap_slivers(Mesh& m, double good_qual, u32 nlayers, double const* eq = nullptr, double const* gate = nullptr,
