@@ -1260,7 +1260,7 @@ struct CoarsenArgs {
   u32 const* ci;
   u32* out_cd;
   u32* out_ci;
-  CoarsenSides sides;
+  CoarsenSides sides, edges;  // down(d, d-1); the tets' edges (sliver passes)
 };
 
 // sides of dimension l of a d-simplex
@@ -1310,7 +1310,9 @@ __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
   if (a.esoff[i] != a.esoff[i + 1]) {
     o = a.esoff[i];
     for (u32 j = 0; j < vpe; ++j) a.out[(size_t) o * vpe + j] = a.vsoff[v[j]];
-    if (CoarsenSides const& s = a.sides; s.out) {
+    for (CoarsenSides const* sp : {&a.sides, &a.edges}) {
+      CoarsenSides const& s = *sp;
+      if (!s.out) continue;
       u32 const ns = simplex_sides(d, s.l);
       for (u32 c = 0; c < ns; ++c) s.out[(size_t) o * ns + c] = s.soff[s.old[(size_t) i * ns + c]];
     }
@@ -1319,7 +1321,9 @@ __global__ __launch_bounds__(BLOCK) void coarsen_outputs(CoarsenArgs a) {
     u32 const* bv = canon(d, d - 1, 0, opp(d, 0)[a.gd[i]]);
     for (u32 j = 0; j + 1 < vpe; ++j) a.out[(size_t) o * vpe + j] = a.vsoff[v[bv[j]]];
     a.out[(size_t) o * vpe + vpe - 1] = a.vsoff[a.gv[i]];
-    if (CoarsenSides const& s = a.sides; s.out) {
+    for (CoarsenSides const* sp : {&a.sides, &a.edges}) {
+      CoarsenSides const& s = *sp;
+      if (!s.out) continue;
       u32 const ns = simplex_sides(d, s.l);
       for (u32 c = 0; c < ns; ++c)
         s.out[(size_t) o * ns + c] = coarsen_product_side(d, c, a.gd[i], s.old + (size_t) i * ns, s);
@@ -1876,9 +1880,10 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   // the maps this pass read and the next collapse pass reads again,
   // carried to the new mesh instead of derived again: each entity's sides
   // (down(d, d-1)) here, the vertices' upward adjacencies below.  The tets'
-  // edges are not carried: only a swap pass reads them, and carrying them
-  // through every collapse pass cost more than one reflection before the
-  // swap (C3: 2.87 against 2.82 s per 8 cycles)
+  // edges and up(1,3) are carried by the sliver collapse passes only: a swap
+  // pass reads them, and carrying them through the collapse-by-size loop
+  // as well cost more than one reflection before the swap (C3: 2.87 against
+  // 2.82 s per 8 cycles)
   Buf<u32> side_maps[4][4];
   for (u32 dd = 0; dd < dim; ++dd) {
     u32 d = dim - dd;
@@ -1889,10 +1894,9 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     Buf<u32> vout((size_t) (nsame + ngen) * vpe);
     Buf<u32> cdo, cio;
     CoarsenArgs a{};
-    if (d >= 2 && m.down[d][d - 1]) {
-      u32 const l = d - 1, ns = simplex_sides(d, l);
+    auto carry_sides = [&](u32 l, CoarsenSides& cs) {
+      u32 const ns = simplex_sides(d, l);
       side_maps[d][l] = Buf<u32>((size_t) (nsame + ngen) * ns);
-      CoarsenSides& cs = a.sides;
       cs.out = side_maps[d][l].get();
       cs.old = m.down[d][l].get();
       cs.soff = plan[l].esoff.get();
@@ -1902,7 +1906,11 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
       cs.l = l;
       cs.nold = m.counts[l];
       cs.err = dev_error_word();
-    }
+    };
+    if (d >= 2 && m.down[d][d - 1]) carry_sides(d - 1, a.sides);
+    // a sliver collapse pass is followed by a swap pass, which reads the
+    // tets' edges (and up(1,3), carried below)
+    if (require_better && d == 3 && m.down[3][1]) carry_sides(1, a.edges);
     a.d = d;
     a.n = n;
     a.nsame = nsame;
@@ -1927,7 +1935,7 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
     // gathered and written, its survivor and product offsets, target and
     // direction, class tags in and out, and the carried sides (old ids in,
     // new ids gathered and written)
-    u32 const ns = a.sides.out ? simplex_sides(d, d - 1) : 0u;
+    u32 const ns = (a.sides.out ? simplex_sides(d, d - 1) : 0u) + (a.edges.out ? 6u : 0u);
     launch_timed("coarsen_outputs", (double) n * (vpe * 12 + 32 + ns * 12), coarsen_outputs, dim3(nblocks(n)),
                  dim3(BLOCK), a);
     set_ents(mo, d, nsame + ngen, std::move(vout));
@@ -1949,6 +1957,9 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   };
   carry_up(1);
   carry_up(dim);
+  if (require_better && dim == 3 && m.up[1][3] && mo.down[3][1])
+    mo.up[1][3] = up_patch(6, *m.up[1][3], m.counts[1], plan[1].esoff.get(), got[3], got[2], plan[3].esoff.get(),
+                           got[7], got[6], mo.down[3][1].get());
   say("collapsed", total, 1);
   overwrite(m, std::move(mo));
   return 1;
