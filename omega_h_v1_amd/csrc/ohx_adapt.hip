@@ -312,7 +312,7 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
 // (find_indset_resume: same rounds, same result).  The batch never runs
 // more rounds than the limit allows, so a set that needs more than the
 // limit always reaches the checked loop's failure.
-// Four rounds: measured on C3 (8 cycles), 87 % of the vertex sets and
+// Four rounds: measured on the 64^3 x 6 anisotropic box (8 cycles), 87 % of the vertex sets and
 // 95 % of the edge sets settle within four rounds (most within two), so a
 // longer first batch mostly launches rounds that only copy a settled state.
 static u32 first_batch_rounds() { return g_indset_max_rounds < 4 ? g_indset_max_rounds : 4; }
@@ -1882,8 +1882,8 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   // (down(d, d-1)) here, the vertices' upward adjacencies below.  The tets'
   // edges and up(1,3) are carried by the sliver collapse passes only: a swap
   // pass reads them, and carrying them through the collapse-by-size loop
-  // as well cost more than one reflection before the swap (C3: 2.87 against
-  // 2.82 s per 8 cycles)
+  // as well cost more than one reflection before the swap (64^3 x 6
+  // anisotropic box: 2.87 against 2.82 s per 8 cycles)
   Buf<u32> side_maps[4][4];
   for (u32 dd = 0; dd < dim; ++dd) {
     u32 d = dim - dd;
@@ -2194,7 +2194,7 @@ __device__ inline u32 edge_ring(u32 edge, u32 const* uo, u32 const* ua, u32 cons
 // and the triangulation code, which swap_cavity uses.  The kernel is built
 // for 4 waves per SIMD (at most 128 VGPRs; its 33 KB of LDS per block allow
 // 4 blocks per CU): with a metric that costs 28 bytes of scratch per lane
-// and measured 4 % less time at C3 than 3 waves without spills.
+// and measured 4 % less time on the 64^3 x 6 anisotropic box than 3 waves without spills.
 constexpr u32 SW_G = 8;
 constexpr u32 SW_EDGES = BLOCK / SW_G;
 constexpr u32 SW_MAXTRI = 35;
