@@ -5,9 +5,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "ohx_common.hpp"
 
@@ -126,20 +128,92 @@ void read_many(unsigned const* const* ptrs, unsigned n, unsigned* out) {
 // needed, and streams of different threads never hand blocks to each other
 // while their kernels may still read them.
 namespace {
+// Blocks come in power-of-two classes (256 B and up), indexed by log2.  The
+// free blocks are kept per stream and class in plain vectors (a stream's
+// entry found by a short linear search: one per driving thread), and the
+// live blocks' classes in an open-addressing table keyed by address, so an
+// allocation or a release costs a few array operations under the lock.
+constexpr unsigned ARENA_CLASSES = 64;
+struct StreamFree {
+  hipStream_t s;
+  std::vector<void*> lists[ARENA_CLASSES];
+};
+struct LiveTable {
+  // keys: block addresses (never null), values: class log2; linear probing
+  // with backward-shift deletion, so no tombstones
+  std::vector<void*> key;
+  std::vector<unsigned char> cls;
+  size_t n = 0;
+  static size_t hash(void* p, size_t mask) {
+    unsigned long long h = (unsigned long long) (size_t) p;
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    return (size_t) h & mask;
+  }
+  void grow() {
+    std::vector<void*> ok = std::move(key);
+    std::vector<unsigned char> oc = std::move(cls);
+    size_t cap = ok.empty() ? 1024 : ok.size() * 2;
+    key.assign(cap, nullptr);
+    cls.assign(cap, 0);
+    n = 0;
+    for (size_t i = 0; i < ok.size(); ++i)
+      if (ok[i]) put(ok[i], oc[i]);
+  }
+  void put(void* p, unsigned char c) {
+    if (2 * (n + 1) > key.size()) grow();
+    size_t mask = key.size() - 1, i = hash(p, mask);
+    while (key[i]) i = (i + 1) & mask;
+    key[i] = p;
+    cls[i] = c;
+    ++n;
+  }
+  // the class of p, removed from the table; -1 if p is not live
+  int take(void* p) {
+    if (key.empty()) return -1;
+    size_t mask = key.size() - 1, i = hash(p, mask);
+    while (key[i] && key[i] != p) i = (i + 1) & mask;
+    if (!key[i]) return -1;
+    int c = cls[i];
+    key[i] = nullptr;
+    --n;
+    // backward shift: move later entries of the cluster into the hole
+    size_t hole = i;
+    for (size_t j = (i + 1) & mask; key[j]; j = (j + 1) & mask) {
+      size_t home = hash(key[j], mask);
+      bool movable = hole <= j ? (home <= hole || home > j) : (home <= hole && home > j);
+      if (movable) {
+        key[hole] = key[j];
+        cls[hole] = cls[j];
+        key[j] = nullptr;
+        hole = j;
+      }
+    }
+    return c;
+  }
+};
 struct Arena {
   std::mutex mu;
-  std::map<std::pair<hipStream_t, size_t>, std::vector<void*>> free_lists;
-  std::unordered_map<void*, size_t> live;  // ptr -> class size
+  std::vector<std::unique_ptr<StreamFree>> free_by_stream;
+  LiveTable live;
   size_t in_use = 0, high = 0, cached = 0;
+  StreamFree& of(hipStream_t s) {
+    for (auto& f : free_by_stream)
+      if (f->s == s) return *f;
+    free_by_stream.push_back(std::make_unique<StreamFree>());
+    free_by_stream.back()->s = s;
+    return *free_by_stream.back();
+  }
 };
 Arena& arena() {
   static Arena* a = new Arena;  // never destroyed: blocks may outlive static teardown
   return *a;
 }
-size_t size_class(size_t bytes) {
-  size_t c = 256;
-  while (c < bytes) c <<= 1;
-  return c;
+unsigned size_class_log2(size_t bytes) {
+  unsigned k = 8;
+  while (((size_t) 1 << k) < bytes) ++k;
+  return k;
 }
 }  // namespace
 
@@ -147,14 +221,15 @@ void* dev_alloc(size_t bytes) {
   rt();
   hipStream_t s = stream();
   Arena& a = arena();
-  size_t c = size_class(bytes ? bytes : 1);
+  unsigned const k = size_class_log2(bytes ? bytes : 1);
+  size_t const c = (size_t) 1 << k;
   void* p = nullptr;
   {
     std::lock_guard<std::mutex> lock(a.mu);
-    auto it = a.free_lists.find({s, c});
-    if (it != a.free_lists.end() && !it->second.empty()) {
-      p = it->second.back();
-      it->second.pop_back();
+    std::vector<void*>& fl = a.of(s).lists[k];
+    if (!fl.empty()) {
+      p = fl.back();
+      fl.pop_back();
       a.cached -= c;
     }
   }
@@ -173,7 +248,7 @@ void* dev_alloc(size_t bytes) {
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   }
   std::lock_guard<std::mutex> lock(a.mu);
-  a.live[p] = c;
+  a.live.put(p, (unsigned char) k);
   a.in_use += c;
   if (a.in_use > a.high) a.high = a.in_use;
   return p;
@@ -184,25 +259,24 @@ void dev_free(void* p) {
   Arena& a = arena();
   hipStream_t s = stream();
   std::lock_guard<std::mutex> lock(a.mu);
-  auto it = a.live.find(p);
-  if (it == a.live.end()) fail("dev_free of an unknown pointer", __FILE__, __LINE__);
-  size_t c = it->second;
-  a.live.erase(it);
+  int const k = a.live.take(p);
+  if (k < 0) fail("dev_free of an unknown pointer", __FILE__, __LINE__);
+  size_t const c = (size_t) 1 << k;
   a.in_use -= c;
-  a.free_lists[{s, c}].push_back(p);
+  a.of(s).lists[k].push_back(p);
   a.cached += c;
 }
 
 void trim_stream(hipStream_t s) {
   Arena& a = arena();
   std::lock_guard<std::mutex> lock(a.mu);
-  for (auto& kv : a.free_lists) {
-    if (kv.first.first != s) continue;
-    for (void* p : kv.second) {
+  StreamFree& f = a.of(s);
+  for (unsigned k = 0; k < ARENA_CLASSES; ++k) {
+    for (void* p : f.lists[k]) {
       (void) hipFree(p);
-      a.cached -= kv.first.second;
+      a.cached -= (size_t) 1 << k;
     }
-    kv.second.clear();
+    f.lists[k].clear();
   }
 }
 
