@@ -181,6 +181,28 @@ def test_c3_full_size_mixed_takes_fp64_decisions(product):
     assert d.ohx_indset_rounds_max(1) > 12
 
 
+def test_c3_full_size_carried_maps_match_derivation(product):
+    """After the first cycle of the full-size workload (hundreds of swap and
+    collapse passes, each handing the maps it read to the next mesh instead
+    of deriving them again), the final mesh's tet-to-edge map, edge-to-tet
+    adjacency, element dual and the vertices' upward adjacencies equal the
+    ones a copy holding only the entities' vertices derives."""
+    d = product.dll
+    d.ohx_set_precision(1)
+    m = ML.Mesh.box_on_device(product, 3, 64)
+    m.derive_model_full()
+    _c3_metric(m)
+    r = m.adapt(ADAPT[0], ADAPT[1], ADAPT[2], 1000)
+    d.ohx_set_precision(0)
+    assert r == 1
+    fresh = m.clone()
+    assert np.array_equal(m.down(3, 1), fresh.down(3, 1))
+    assert np.array_equal(m.dual(), fresh.dual())
+    for l, h in ((1, 3), (0, 1), (0, 3)):
+        for a, b in zip(m.up(l, h), fresh.up(l, h)):
+            assert np.array_equal(a, b), "up(%d,%d)" % (l, h)
+
+
 # ------------------------------------------------------------------ C4
 def _c4_mesh(lib, s):
     x, c = ML.box(3, s["n"])
