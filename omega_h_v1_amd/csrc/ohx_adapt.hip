@@ -2475,42 +2475,47 @@ __device__ inline u32 tet_edge_between(u32 a, u32 b) {
 }
 
 // swap_cavity (swap_topology.cpp:46-75) with get_swap_ents
-// (edge_swap.cpp:282-373): the ring again (integer work) and the
-// triangulation code that swap_quality chose, then the products
-__global__ __launch_bounds__(BLOCK) void swap_cavity_k(
-    u32 ne, u32 d, u32 const* __restrict__ indset, u32 const* __restrict__ goff, u32 const* __restrict__ uo,
-    u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
-    u32 const* __restrict__ vt, u32 const* __restrict__ codes, u32* __restrict__ out) {
-  u32 i = xcd_block() * BLOCK + threadIdx.x;
-  if (i >= ne || !indset[i]) return;
+// (edge_swap.cpp:282-373), every dimension in one launch: the ring again
+// (integer work) and the triangulation code that swap_quality chose, then
+// the products' vertices (interior edges, their faces, the tets: triangle t
+// of the triangulation with edge end 1, then with its first two corners
+// exchanged and edge end 0), written where out_d is given.  With E.out, the
+// product tets' edges too: a product tet's vertices are labelled by where
+// they come from (ring slot, or edge end); an edge between adjacent ring
+// slots is a ring edge, one between a slot and an edge end a spoke, both
+// edges of the old tet at that ring slot (the one holding ring vertices k
+// and k + 1), and any other is one of the new interior edges.  The cavity's
+// ring edges and spokes are staged in LDS per thread (indexed by slot).
+__global__ __launch_bounds__(BLOCK) void swap_products_k(
+    u32 ne, u32 const* __restrict__ indset, u32 const* __restrict__ goff1, u32 const* __restrict__ goff2,
+    u32 const* __restrict__ goff3, u32 const* __restrict__ uo, u32 const* __restrict__ ua,
+    u32 const* __restrict__ ud, u32 const* __restrict__ ev, u32 const* __restrict__ vt,
+    u32 const* __restrict__ codes, u32* __restrict__ out1, u32* __restrict__ out2, u32* __restrict__ out3,
+    SwapEdgesOf E) {
+  __shared__ u32 s_ring[MAX_EDGE_SWAP][BLOCK];
+  __shared__ u32 s_spoke[2][MAX_EDGE_SWAP][BLOCK];
+  __shared__ u32 s_tet[MAX_EDGE_SWAP][BLOCK];
+  u32 const tid = threadIdx.x;
+  u32 i = xcd_block() * BLOCK + tid;
+  if (i >= ne || !indset[i]) return;  // no barrier follows
   u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
   u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
   if (rs < 3 || rs > MAX_EDGE_SWAP) return;  // cannot happen for an indset edge
   u32 const code = codes[i];
-  u32 tpm = k_sw_sizes[rs];
+  u32 const tpm = k_sw_sizes[rs];
   unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs] + code * tpm;
   unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
-  u32 nint = k_sw_nint[rs];
+  u32 const nint = k_sw_nint[rs];
   unsigned char const* ie = k_sw_int + k_sw_int_off[rs] + code * nint * 2;
-  u32* o = out + (size_t) goff[i] * (d + 1);
-  if (d == 3) {
-    for (u32 t = 0; t < tpm; ++t) {
-      u32 tv[4];
-      for (u32 j = 0; j < 3; ++j) tv[j] = ring_v[tris[mesh[t]][j]];
-      for (u32 j = 0; j < 2; ++j) {
-        tv[3] = edge_v[1 - j];
-        for (u32 k = 0; k < 4; ++k) *o++ = tv[k];
-        u32 tmp = tv[0];
-        tv[0] = tv[1];
-        tv[1] = tmp;
-      }
-    }
-  } else if (d == 1) {
+  if (out1) {
+    u32* o = out1 + (size_t) goff1[i] * 2;
     for (u32 k = 0; k < nint; ++k) {
       o[2 * k] = ring_v[ie[2 * k]];
       o[2 * k + 1] = ring_v[ie[2 * k + 1]];
     }
-  } else {
+  }
+  if (out2) {
+    u32* o = out2 + (size_t) goff2[i] * 3;
     for (u32 k = 0; k < nint; ++k)
       for (u32 j = 0; j < 2; ++j) {
         o[k * 6 + j * 3 + 0] = ring_v[ie[2 * k]];
@@ -2521,32 +2526,8 @@ __global__ __launch_bounds__(BLOCK) void swap_cavity_k(
     for (u32 t = 0; t < tpm; ++t)
       for (u32 j = 0; j < 3; ++j) o[t * 3 + j] = ring_v[tris[mesh[t]][j]];
   }
-}
-
-// The product tets' edges of each swapped edge, in the order swap_cavity_k
-// writes the tets (triangle t of the chosen triangulation with edge end 1,
-// then with its first two corners exchanged and edge end 0).  A product
-// tet's vertices are labelled by where they come from (ring slot, or edge
-// end); an edge between adjacent ring slots is a ring edge, one between a
-// slot and an edge end a spoke, both edges of the old tet at that ring slot
-// (the one holding ring vertices k and k + 1), and any other is one of the
-// new interior edges.  The cavity's ring edges and spokes are staged in
-// LDS per thread (indexed by slot).
-__global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
-    u32 ne, u32 const* __restrict__ indset, u32 const* __restrict__ goff3, u32 const* __restrict__ uo,
-    u32 const* __restrict__ ua, u32 const* __restrict__ ud, u32 const* __restrict__ ev,
-    u32 const* __restrict__ vt, u32 const* __restrict__ codes, SwapEdgesOf E) {
-  __shared__ u32 s_ring[MAX_EDGE_SWAP][BLOCK];
-  __shared__ u32 s_spoke[2][MAX_EDGE_SWAP][BLOCK];
-  __shared__ u32 s_tet[MAX_EDGE_SWAP][BLOCK];
-  u32 const tid = threadIdx.x;
-  u32 i = xcd_block() * BLOCK + tid;
-  if (i >= ne || !indset[i]) return;  // no barrier follows
-  u32 edge_v[2], ring_v[MAX_EDGE_SWAP];
-  u32 rs = edge_ring(i, uo, ua, ud, ev, vt, edge_v, ring_v);
-  if (rs < 3 || rs > MAX_EDGE_SWAP) return;
   u32 const f = uo[i], e = uo[i + 1];
-  for (u32 k = 0; k < rs; ++k) {
+  for (u32 k = 0; E.out && k < rs; ++k) {
     u32 const a = ring_v[k], b = ring_v[k + 1 < rs ? k + 1 : 0];
     u32 tet = INVALID;
     for (u32 q = f; q < e && tet == INVALID; ++q) {
@@ -2576,18 +2557,28 @@ __global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
     s_spoke[0][k][tid] = E.soff1[te[tet_edge_between(la, l0)]];
     s_spoke[1][k][tid] = E.soff1[te[tet_edge_between(la, l1)]];
   }
-  u32 const code = codes[i];
-  u32 const tpm = k_sw_sizes[rs];
-  unsigned char const* mesh = k_sw_meshes + k_sw_mesh_off[rs] + code * tpm;
-  unsigned char const(*tris)[3] = k_sw_tris + k_sw_tri_off[rs];
-  u32 const nint = k_sw_nint[rs];
-  unsigned char const* ie = k_sw_int + k_sw_int_off[rs] + code * nint * 2;
+  if (!out3) return;
+  u32* o3 = out3 + (size_t) goff3[i] * 4;
   u32 const first = E.nsame3 + goff3[i];
   for (u32 t = 0; t < tpm; ++t) {
-    u32 lab[4];  // ring slot, or 8 + edge end
-    for (u32 j = 0; j < 3; ++j) lab[j] = tris[mesh[t]][j];
+    u32 lab[4], tv[4];  // labels: ring slot, or 8 + edge end; vertices
+    for (u32 j = 0; j < 3; ++j) {
+      lab[j] = tris[mesh[t]][j];
+      tv[j] = ring_v[lab[j]];
+    }
     for (u32 j = 0; j < 2; ++j) {
       lab[3] = 8 + (1 - j);
+      tv[3] = edge_v[1 - j];
+      for (u32 k = 0; k < 4; ++k) *o3++ = tv[k];
+      if (!E.out) {
+        u32 tmp = tv[0];
+        tv[0] = tv[1];
+        tv[1] = tmp;
+        tmp = lab[0];
+        lab[0] = lab[1];
+        lab[1] = tmp;
+        continue;
+      }
       u32* eo = E.out + 6 * (size_t) (first + 2 * t + j);
       for (u32 le = 0; le < 6; ++le) {
         u32 const* c = canon(3, 1, 0, le);
@@ -2661,7 +2652,10 @@ __global__ __launch_bounds__(BLOCK) void swap_product_edges_k(
           dd[f] = nb;
         }
       }
-      u32 tmp = lab[0];
+      u32 tmp = tv[0];
+      tv[0] = tv[1];
+      tv[1] = tmp;
+      tmp = lab[0];
       lab[0] = lab[1];
       lab[1] = tmp;
     }
@@ -3025,55 +3019,59 @@ static u32 swap_common(Mesh& m, Buf<u32>& cand, double const* eq_given = nullptr
   // the tets' edges of the new mesh, from the old map (the old one exists:
   // the rings above read up(1,3)); the next pass asks for them first
   bool const edges_of_tets = m.dim == 3 && dlo == 1;
-  Buf<u32> d31, dual_new;
+  Buf<u32> d31, dual_new, face_prod, deferred;
+  // every dimension's products in one launch (the ring of each swapped edge
+  // found once), then the survivors dimension by dimension
+  Buf<u32> vouts[4];
+  u32* pouts[4] = {nullptr, nullptr, nullptr, nullptr};
+  double prof_bytes = (double) ne * 12;
+  for (u32 d = dlo; d <= m.dim; ++d) {
+    u32 ngen = tots[1 + 2 * (d - dlo)], nsame = tots[2 + 2 * (d - dlo)];
+    vouts[d] = Buf<u32>((size_t) (nsame + ngen) * (d + 1));
+    pouts[d] = vouts[d].get() + (size_t) nsame * (d + 1);
+    prof_bytes += (double) ngen * (d + 1) * 4;
+  }
+  SwapEdgesOf E{};
+  u32 const nt_old = count(m, 3), ngen3 = tots[1 + 2 * (3 - dlo)], nsame3 = tots[2 + 2 * (3 - dlo)];
+  if (edges_of_tets) {
+    d31 = Buf<u32>((size_t) (nsame3 + ngen3) * 6);
+    E.out = d31.get();
+    E.old = ask_down(m, 3, 1);
+    E.soff1 = soffs[1].get();
+    E.goff1 = goffs[1].get();
+    E.nsame1 = tots[2];
+    E.nsame3 = nsame3;
+    E.err = dev_error_word();
+    if (m.dual) {
+      // the dual this pass's sliver marks read, carried to the next mesh
+      dual_new = Buf<u32>((size_t) (nsame3 + ngen3) * 4);
+      face_prod = Buf<u32>((size_t) nt_old * 4);
+      deferred = Buf<u32>((size_t) ngen3 * 4);
+      E.dual = dual_new.get();
+      E.old_dual = m.dual.get();
+      E.soff3 = soffs[3].get();
+      E.face_prod = face_prod.get();
+      E.deferred = deferred.get();
+    }
+  }
+  launch_timed("swap_cavity", prof_bytes, swap_products_k, dim3(nblocks(ne)), dim3(BLOCK), ne, is,
+               goffs[1].get(), goffs[2].get(), goffs[3].get(), u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt,
+               codes.get(), pouts[1], pouts[2], pouts[3], E);
+  if (E.dual) {
+    hipLaunchKernelGGL(swap_dual_k, dim3(nblocks((size_t) nt_old + (size_t) ngen3 * 4)), dim3(BLOCK), 0, stream(),
+                       nt_old, ngen3, E);
+    check_launch("swap_dual");
+  }
   for (u32 d = dlo; d <= m.dim; ++d) {
     u32 n = count(m, d);
     u32 vpe = d + 1;
     Buf<u32>& goff = goffs[d];
     Buf<u32>& soff = soffs[d];
     u32 ngen = tots[1 + 2 * (d - dlo)], nsame = tots[2 + 2 * (d - dlo)];
-    Buf<u32> vout((size_t) (nsame + ngen) * vpe);
-    SwapEdgesOf E{};
-    Buf<u32> face_prod, deferred;
-    if (d == 3 && edges_of_tets) {
-      d31 = Buf<u32>((size_t) (nsame + ngen) * 6);
-      E.out = d31.get();
-      E.old = ask_down(m, 3, 1);
-      E.soff1 = soffs[1].get();
-      E.goff1 = goffs[1].get();
-      E.nsame1 = tots[2];
-      E.nsame3 = nsame;
-      E.err = dev_error_word();
-      if (m.dual) {
-        // the dual this pass's sliver marks read, carried to the next mesh
-        dual_new = Buf<u32>((size_t) (nsame + ngen) * 4);
-        face_prod = Buf<u32>((size_t) n * 4);
-        deferred = Buf<u32>((size_t) ngen * 4);
-        E.dual = dual_new.get();
-        E.old_dual = m.dual.get();
-        E.soff3 = soff.get();
-        E.face_prod = face_prod.get();
-        E.deferred = deferred.get();
-      }
-    }
-    {
-      double prof_bytes = (double) ne * 12 + (double) ngen * vpe * 4;
-      launch_timed("swap_cavity", prof_bytes, swap_cavity_k, dim3(nblocks(ne)), dim3(BLOCK), ne, d, is, goff.get(),
-                   u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), vout.get() + (size_t) nsame * vpe);
-      if (E.out) {
-        hipLaunchKernelGGL(swap_product_edges_k, dim3(nblocks(ne)), dim3(BLOCK), 0, stream(), ne, is, goff.get(),
-                           u.offsets.get(), u.adj.get(), u.dirs.get(), ev, vt, codes.get(), E);
-        check_launch("swap_product_edges");
-      }
-      if (E.dual) {
-        hipLaunchKernelGGL(swap_dual_k, dim3(nblocks((size_t) n + (size_t) ngen * 4)), dim3(BLOCK), 0, stream(), n,
-                           ngen, E);
-        check_launch("swap_dual");
-      }
-    }
+    Buf<u32> vout = std::move(vouts[d]);
     Buf<u32> cdo, cio;
     SwapKeepArgs a{};
-    a.E = E;
+    if (d == 3) a.E = E;
     a.n = n;
     a.ne = ne;
     a.vpe = vpe;
