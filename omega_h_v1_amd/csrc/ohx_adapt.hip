@@ -1208,6 +1208,26 @@ __global__ __launch_bounds__(BLOCK) void coarsen_size_code_f64(u32 n, u32 const*
   wave_flag_value(words + W_ANY, c != 0, tag);
 }
 
+// size codes carried to a collapse pass's new mesh: threads [0, ne_old)
+// give a surviving edge its old code; the next ngen measure a product edge
+__global__ __launch_bounds__(BLOCK) void carry_size_codes_k(u32 ne_old, u32 ngen, u32 nsame,
+                                                            u32 const* __restrict__ soff,
+                                                            u32 const* __restrict__ old_codes,
+                                                            u32 const* __restrict__ ev, double const* __restrict__ x,
+                                                            double const* __restrict__ h,
+                                                            double const* __restrict__ mt, double floor_,
+                                                            u32* __restrict__ codes) {
+  u32 t = xcd_block() * BLOCK + threadIdx.x;
+  if (t < ne_old) {
+    u32 o = soff[t];
+    if (o != soff[t + 1]) codes[o] = old_codes[t];
+  } else if (t - ne_old < ngen) {
+    u32 i = nsame + (t - ne_old);
+    double sz = edge_size(ev[2 * i], ev[2 * i + 1], x, h, mt);
+    codes[i] = (sz < floor_) ? (u32) COLLAPSE_BOTH : (u32) DONT_COLLAPSE;
+  }
+}
+
 // tags_subset (subset.cpp:15-43): every tag of the surviving vertices,
 // one thread per old vertex, all tags in one launch
 __global__ __launch_bounds__(BLOCK) void subset_tags(u32 n, u32 const* __restrict__ soff, TagList L) {
@@ -1576,7 +1596,8 @@ static std::unique_ptr<Up> up_patch(u32 deg, Up const& old, u32 nl_old, u32 cons
 }
 
 static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool require_better,
-                          double const* qfe = nullptr, double const* eq_given = nullptr) {
+                          double const* qfe = nullptr, double const* eq_given = nullptr,
+                          SizeCodeCache* sizes = nullptr, double size_floor = 0) {
   check_index_space(m);
   u32 dim = m.dim;
   u32 ne = count(m, 1);
@@ -1957,6 +1978,19 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   };
   carry_up(1);
   carry_up(dim);
+  if (sizes && sizes->ne == m.counts[1]) {
+    // the new mesh's size codes: survivors' carried, products measured
+    u32 const ne_old = m.counts[1], ngen1 = got[2], nsame1 = got[3];
+    Buf<u32> nc((size_t) nsame1 + ngen1);
+    hipLaunchKernelGGL(carry_size_codes_k, dim3(nblocks((size_t) ne_old + ngen1)), dim3(BLOCK), 0, stream(), ne_old,
+                       ngen1, nsame1, plan[1].esoff.get(), sizes->codes.get(), mo.down[1][0].get(), coords(mo),
+                       size_of(mo), metric_of(mo), size_floor, nc.get());
+    check_launch("carry_size_codes");
+    sizes->ne = nsame1 + ngen1;
+    sizes->codes = std::move(nc);
+  } else if (sizes) {
+    sizes->clear();
+  }
   if (require_better && dim == 3 && m.up[1][3] && mo.down[3][1])
     mo.up[1][3] = up_patch(6, *m.up[1][3], m.counts[1], plan[1].esoff.get(), got[3], got[2], plan[3].esoff.get(),
                            got[7], got[6], mo.down[3][1].get());
@@ -1965,16 +1999,32 @@ static u32 coarsen_common(Mesh& m, Buf<u32>& codes, double qual_floor, bool requ
   return 1;
 }
 
-u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
+u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor, SizeCodeCache* cache) {
   u32 ne = count(m, 1);
   double const* h = size_of(m);
   double const* mt = metric_of(m);
   Buf<u32> codes(ne);
   u32 tag = next_any_tag();
-  if (ne) {
+  if (g_batch && cache) cache->clear();
+  if (cache && cache->ne != ne) cache->clear();
+  if (ne && cache && cache->ne) {
+    // the codes carried from the pass before (the pass consumes its copy)
+    u32* c = codes.get();
+    u32 const* cc = cache->codes.get();
+    u32* words = scratch_words();
+    parallel_for(ne, OHX_LAMBDA(u32 i) {
+      u32 const v = cc[i];
+      c[i] = v;
+      wave_flag_value(words + W_ANY, v != 0, tag);
+    });
+  } else if (ne) {
     double prof_bytes = (double) ne * 12 + (double) count(m, 0) * 32;
     launch_timed("coarsen_size_code_f64", prof_bytes, coarsen_size_code_f64, dim3(nblocks(ne)), dim3(BLOCK), ne,
                        ask_down(m, 1, 0), coords(m), h, mt, size_floor, codes.get(), scratch_words(), tag);
+    if (cache && !g_batch) {
+      cache->ne = ne;
+      cache->codes = copy(codes.get(), ne);
+    }
   }
   // check_coarsen_noop (coarsen_common.cpp:109-118), kept as its own read:
   // without it the last, unproductive pass of every coarsening loop would
@@ -1986,7 +2036,7 @@ u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor) {
     Buf<double> qfe = batch_floors(m, 1);
     return coarsen_common(m, codes, qual_floor, false, qfe.get());
   }
-  return coarsen_common(m, codes, qual_floor, false);
+  return coarsen_common(m, codes, qual_floor, false, nullptr, nullptr, cache, size_floor);
 }
 
 // coarsen_slivers (coarsen.cpp:38-70)
@@ -3177,6 +3227,7 @@ struct Ctx {
   u32 ops = 0, max_ops = 0;
   AdaptStats st;
   SwapCache swap_cache;  // the shape loop's swap choices (cleared by other passes)
+  SizeCodeCache size_codes;  // the collapse loop's size codes (cleared by other passes)
   // passes that changed the mesh: events around them on the stream, read
   // once at the end (a pass's time is the stream's, from the end of the work
   // before it to the end of its own, host waits included)
@@ -3238,9 +3289,10 @@ u32 run_pass(Mesh& m, Ctx& c, Kind k, double a, double b, u32 n, double const* e
   OHX_CHECK(hipEventRecord(e0, stream()));
   u32 ok = 0;
   if (k != SWAP) c.swap_cache.clear();
+  if (k != COARSEN) c.size_codes.clear();
   switch (k) {
     case REFINE: ok = refine_by_size(m, a); break;
-    case COARSEN: ok = coarsen_by_size(m, a, b); break;
+    case COARSEN: ok = coarsen_by_size(m, a, b, &c.size_codes); break;
     case SWAP: ok = swap_slivers(m, a, n, eq, gate, &c.swap_cache); break;
     case COARSEN_SLIVERS: ok = coarsen_slivers(m, a, n, eq); break;
   }

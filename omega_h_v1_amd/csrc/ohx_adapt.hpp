@@ -11,7 +11,19 @@ Buf<u32> find_indset(Mesh& m, u32 ent_dim, u32 const* filter, double const* good
 
 u32 refine_by_size(Mesh& m, double qual_floor);
 void uniformly_refine(Mesh& m);
-u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor);
+// The size codes of the collapse-by-size loop, per edge of the current
+// mesh: a surviving edge keeps its vertices, their coordinates and the
+// metric, so its code (its length against the floor) stays what it was;
+// only the product edges of a pass are measured again.
+struct SizeCodeCache {
+  u32 ne = 0;  // edges of the mesh it describes (0: empty)
+  Buf<u32> codes;
+  void clear() {
+    ne = 0;
+    codes.release();
+  }
+};
+u32 coarsen_by_size(Mesh& m, double qual_floor, double size_floor, SizeCodeCache* cache = nullptr);
 // eq: the mesh's element qualities when the caller has them (else computed)
 u32 coarsen_slivers(Mesh& m, double qual_floor, u32 nlayers, double const* eq = nullptr);
 // The swap choices of the edges evaluated by earlier swap passes of one
